@@ -1,0 +1,289 @@
+"""Write the golden fixtures in tests/golden/ from the compiled reference.
+
+Run in the build container (needs /root/reference to build oracle/_ref):
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each fixture holds inputs and the reference's outputs for one hot-path function family; the
+reference is called through oracle/ref_py.py.  Large outputs are stored as a sha256 of the
+exact bytes plus slices for diagnostics.  manifest.json records parameters, seeds and the
+sha256 of the reference sources the library was built from.
+"""
+import ctypes as C
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+# The reference's z-slab Mur faces read plane k=1 / k=K-2 from the slab owner; with slabs thinner
+# than 2 planes (K < 2*threads) that read races with a neighbour thread (timefield3d.c:2799-2848),
+# so fixtures are generated with few threads and K >= 16.
+os.environ["OMP_NUM_THREADS"] = "4"
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import ref_py as R  # noqa: E402
+
+REF_DIR = "/root/reference"
+DISC = 0.02
+DT = 0.99 * DISC / np.sqrt(3)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def save(name, **arrays):
+    np.savez_compressed(os.path.join(HERE, name), **arrays)
+
+
+def new_tf(L, I, J, K):
+    tf = L.timefield3d_new(I, J, K, DISC)
+    return tf, R.tf_arrays(tf)
+
+
+def f1_f2(L, manifest):
+    """F1: random eps/mu/fields, 50 reference steps.  F2: adds random sigma and sources."""
+    for name, (I, J, K), steps, lossy, seed in (("F1_step_random", (20, 14, 16), 50, False, 101),
+                                                ("F2_step_sigma_sources", (14, 12, 16), 30, True, 202)):
+        rng = np.random.default_rng(seed)
+        tf, A = new_tf(L, I, J, K)
+        A["ep"][:] = rng.uniform(1.0, 12.1, A["ep"].size).astype(np.float32)
+        A["mu"][:] = rng.uniform(1.0, 2.0, A["mu"].size).astype(np.float32)
+        f0 = rng.uniform(-1.0, 1.0, A["fields"].size).astype(np.float32)
+        A["fields"][:] = f0
+        extra = {}
+        if lossy:
+            A["sig"][:] = rng.uniform(0.0, 5.0, A["sig"].size).astype(np.float32)
+            A["sources"][:] = rng.uniform(-1.0, 1.0, A["sources"].size).astype(np.float32)
+            extra = {"sig": A["sig"].copy(), "sources": A["sources"].copy()}
+        for _ in range(steps):
+            L.timefield3d_apply_timestep_omp_full(tf, DT)
+        save(name, dims=np.array([I, J, K]), disc=DISC, dt=DT, steps=steps, ep=A["ep"].copy(), mu=A["mu"].copy(),
+             fields0=f0, fields=A["fields"].copy(), **extra)
+        manifest[name] = {"dims": [I, J, K], "steps": steps, "seed": seed, "sigma_and_sources": lossy,
+                          "calls": "timefield3d_apply_timestep_omp_full"}
+        L.timefield3d_free(tf)
+
+
+def dipole_J(n):
+    """Gaussian-modulated sine, lambda0 = 1.55 um, bandwidth 0.2 um, t0 = 250 dt."""
+    lam0, dlam = 1.55, 0.2
+    w0 = 2 * np.pi / lam0
+    sig_t = lam0 * lam0 / (2 * np.pi * dlam)
+    t = n * DT
+    t0 = 250 * DT
+    return np.exp(-(t - t0) ** 2 / (2 * sig_t ** 2)) * np.sin(w0 * t)
+
+
+def f3(L, manifest):
+    """F3: 64^3 vacuum, Mur, z-polarised point dipole at the centre, 500 steps; Ez at the end and
+    the DFT of Ez on plane z=48 at 5 wavelengths 1.50..1.60 um (reference DFT routine applied to a
+    one-plane view of the grid, one fourier buffer per wavelength)."""
+    N, steps = 64, 500
+    tf, A = new_tf(L, N, N, N)
+    c0 = 32 + N * (32 + N * 32)
+    lams = np.linspace(1.50, 1.60, 5)
+    omegas = 2 * np.pi / lams
+    plane = N * N
+    bufs = [np.zeros(12 * plane) for _ in lams]
+    fields_ptr = C.cast(tf.fields, C.c_void_p).value
+    Js = np.zeros(steps)
+    for n in range(steps):
+        Js[n] = np.float32(dipole_J(n))
+        A["sources"][6 * c0 + 2] = Js[n]
+        L.timefield3d_apply_timestep_omp_full(tf, DT)
+        for q, w in enumerate(omegas):
+            view = R.TimeField(N, N, 1, DISC, tf.ep, tf.mu, tf.sig,
+                               C.cast(fields_ptr + 8 * 6 * plane * 48, C.POINTER(C.c_double)), tf.old_fields,
+                               C.cast(bufs[q].ctypes.data, C.POINTER(R.Complex)), tf.sources)
+            L.timefield3d_update_fourier_fields(view, n * DT * w)
+    F = A["fields"].reshape(N, N, N, 6)
+    dft_ez = np.stack([b.reshape(N, N, 6, 2)[:, :, 2, :] for b in bufs])  # [lam, y, x, re/im]
+    save("F3_dipole_mur_64.npz", dims=np.array([N, N, N]), disc=DISC, dt=DT, steps=steps, J=Js,
+         omegas=omegas, fields_sha=sha(A["fields"]), ez_z32=F[32, :, :, 2].copy(), ez_z48=F[48, :, :, 2].copy(),
+         ez_sum_sq=np.sum(F[..., 2] ** 2), dft_ez_z48=dft_ez)
+    manifest["F3_dipole_mur_64"] = {"dims": [N, N, N], "steps": steps, "source": "J_z at (32,32,32), float32",
+                                   "lambdas_um": lams.tolist(), "t0_steps": 250,
+                                   "phase": "omega * n * dt after step n"}
+    L.timefield3d_free(tf)
+
+
+def f4(L, manifest):
+    """F4: Si strip, Gaussian modes from mode_gaussian; forward run with mode_source, y-slice DFT and
+    three overlap monitors; then mode_smul + zero_fields + a time-reversed adjoint run."""
+    I, J, K = 40, 40, 60
+    om = 2 * np.pi / 1.55
+    tf, A = new_tf(L, I, J, K)
+    L.timefield3d_set_background(tf, 1.444 ** 2, 1.0)
+    L.timefield3d_set_waveguide(tf, 3.48 ** 2, 14, 26, 0, K, 17, 23)
+    md = L.mode_gaussian(30, 30, DISC, R.Complex(1.444 ** 2, 0), R.Complex(1, 0), om, 4.0, 0)
+    md2 = L.mode_gaussian(24, 24, DISC, R.Complex(1.444 ** 2, 0), R.Complex(1, 0), om, 3.0, 1)
+    mode1_0 = R.mode_fields(md).copy()
+    mode2_0 = R.mode_fields(md2).copy()
+    mons = [(15, 0, md), (15, 1, md), (45, 0, md2)]
+    cums = [R.Complex(0, 0) for _ in mons]
+    fwd_steps = 120
+    inst = np.zeros((fwd_steps, len(mons), 2))
+    for n in range(fwd_steps):
+        amp = np.exp(-(n - 60) ** 2 / (2 * 25.0 ** 2))
+        ph = n * DT * om
+        L.timefield3d_mode_source(tf, I // 2, J // 2, 10, md, ph, amp, 0)
+        L.timefield3d_apply_timestep_omp_full(tf, DT)
+        L.timefield3d_update_fourier_fields_y_slice_omp(tf, ph, J // 2)
+        for q, (z, rev, mm) in enumerate(mons):
+            r = L.timefield3d_mode_overlap(tf, I // 2, J // 2, z, mm, ph, rev, C.byref(cums[q]))
+            inst[n, q] = (r.re, r.im)
+    FF = A["fourier"].reshape(K, J, I, 6, 2)[:, J // 2]
+    fwd = dict(fwd_fields_sha=sha(A["fields"]), fwd_ex_plane=A["fields"].reshape(K, J, I, 6)[:, J // 2, :, 0].copy(),
+               fwd_src_plane=A["sources"].reshape(K, J, I, 6)[10].copy(), fwd_inst=inst,
+               fwd_cum=np.array([[c.re, c.im] for c in cums]), fwd_dft_y_exey=FF[:, :, 0:2, :].copy(),
+               mode1_after_fwd=R.mode_fields(md).copy())
+    L.mode_smul(R.Complex(0.3, 0.8), md)
+    mode1_rot = R.mode_fields(md).copy()
+    L.timefield3d_zero_fields(tf)
+    adj_steps = 50
+    for n in range(adj_steps, 0, -1):
+        ph = (n + 1) * DT * om
+        L.timefield3d_mode_source_adjoint(tf, I // 2, J // 2, 45, md, ph, 0)
+        L.timefield3d_apply_timestep_omp_full(tf, DT)
+    save("F4_mode_source_overlap.npz", dims=np.array([I, J, K]), disc=DISC, dt=DT, omega=om, ep=A["ep"].copy(),
+         mu=A["mu"].copy(), mode1=mode1_0, mode2=mode2_0, mode1_rot=mode1_rot, fwd_steps=fwd_steps,
+         adj_steps=adj_steps, adj_fields_sha=sha(A["fields"]),
+         adj_hy_plane=A["fields"].reshape(K, J, I, 6)[:, J // 2, :, 4].copy(),
+         adj_src_plane=A["sources"].reshape(K, J, I, 6)[45].copy(), **fwd)
+    manifest["F4_mode_source_overlap"] = {
+        "dims": [I, J, K], "background_eps": 1.444 ** 2, "core": "eps 3.48^2, x 14..26, y 17..23, all z",
+        "mode1": "mode_gaussian(30,30,0.02,eps_bg,1,omega,4.0,0)", "mode2": "mode_gaussian(24,24,...,3.0,1)",
+        "source": "plane z=10 centre (20,20), amplitude exp(-(n-60)^2/(2*25^2)), phase n*dt*omega",
+        "monitors": "(z=15,rev 0,mode1), (z=15,rev 1,mode1), (z=45,rev 0,mode2)", "dft": "y slice j=20",
+        "adjoint": "mode_smul(0.3+0.8i, mode1); zero_fields; n=50..1 source_adjoint z=45 phase (n+1)*dt*omega"}
+    L.mode_free(md)
+    L.mode_free(md2)
+    L.timefield3d_free(tf)
+
+
+def f5(L, manifest):
+    """F5: mode_power_normalize applied 1x and 3x to a seeded random 7x5 mode."""
+    rng = np.random.default_rng(505)
+    md = L.mode_new(7, 5, 0.05)
+    mf = R.mode_fields(md)
+    mf[:] = rng.uniform(-1, 1, mf.size)
+    m0 = mf.copy()
+    p = L.mode_power_inner_product(md, md)
+    L.mode_power_normalize(md)
+    m1 = mf.copy()
+    L.mode_power_normalize(md)
+    L.mode_power_normalize(md)
+    m3 = mf.copy()
+    save("F5_power_normalize.npz", mi=7, mj=5, mdisc=0.05, mode0=m0, P0=np.array([p.re, p.im]), mode1=m1, mode3=m3)
+    manifest["F5_power_normalize"] = {"mode": "7x5, disc 0.05, fields U[-1,1] seed 505"}
+    L.mode_free(md)
+
+
+def pattern_fields(T, n, a, b):
+    """Deterministic field snapshots from integer arithmetic only: ((a*idx + b*t) % 97)/48.5 - 1."""
+    idx = np.arange(n, dtype=np.int64)
+    return np.stack([(((a * idx + b * t) % 97) / 48.5 - 1.0) for t in range(T)]).astype(np.float32)
+
+
+def f6(L, manifest):
+    """F6: taper edge store, ep-gradient, control-point projection, symmetric/asymmetric updates."""
+    rng = np.random.default_rng(606)
+    I, J, K = 30, 20, 40
+    tf, A = new_tf(L, I, J, K)
+    seg, start, end, bottom, top, T = 5, 5, 35, 8, 12, 20
+    left = np.array([12, 11, 10, 9, 9, 8], np.int32)
+    right = np.array([17, 18, 19, 21, 21, 22], np.int32)
+    tp = L.adjoint3d_taper_new(3.51 ** 2, 1.0, seg, start, end, left.ctypes.data_as(C.POINTER(C.c_int)),
+                               right.ctypes.data_as(C.POINTER(C.c_int)), bottom, top, T)
+    H, Ln = top - bottom, end - start
+    fields_seq = pattern_fields(T, A["fields"].size, 7, 13)
+    for t in range(T):
+        A["fields"][:] = fields_seq[t]
+        L.adjoint3d_taper_store_forward_fields(tp, tf, t)
+    adj_seq = pattern_fields(T, A["fields"].size, 11, 5)
+    for t in range(T):
+        A["fields"][:] = adj_seq[t]
+        L.adjoint3d_taper_store_adjoint_fields(tp, tf, t)
+    nh = 6 * 2 * H * Ln * T
+    fwd = np.ctypeslib.as_array(tp.forward_fields, shape=(nh,)).copy()
+    adj = np.ctypeslib.as_array(tp.adjoint_fields, shape=(nh,)).copy()
+    L.adjoint3d_taper_calculate_ep_gradient(tp)
+    epg = np.ctypeslib.as_array(tp.ep_gradient, shape=(2 * H * Ln,)).copy()
+    ep_diff = 3.51 * 3.51 - 1.447 * 1.447
+    L.adjoint3d_taper_calculate_gradients_from_ep_gradient(tp, ep_diff)
+    lg = np.ctypeslib.as_array(tp.left_grad, shape=(seg + 1,)).copy()
+    rg = np.ctypeslib.as_array(tp.right_grad, shape=(seg + 1,)).copy()
+    updates = []
+    for mode_name, goal in (("asymmetric", 0.05), ("asymmetric", 1e-7), ("symmetric", 0.05), ("symmetric", 1e-7)):
+        t2 = L.adjoint3d_taper_new(3.51 ** 2, 1.0, seg, start, end, left.ctypes.data_as(C.POINTER(C.c_int)),
+                                   right.ctypes.data_as(C.POINTER(C.c_int)), bottom, top, 1)
+        np.ctypeslib.as_array(t2.left_grad, shape=(seg + 1,))[:] = lg
+        np.ctypeslib.as_array(t2.right_grad, shape=(seg + 1,))[:] = rg
+        fn = L.adjoint3d_taper_update_asymmetric if mode_name == "asymmetric" else L.adjoint3d_taper_update_symmetric
+        flag = fn(t2, goal, DISC, 3)
+        updates.append((flag, np.ctypeslib.as_array(t2.left, shape=(seg + 1,)).copy(),
+                        np.ctypeslib.as_array(t2.right, shape=(seg + 1,)).copy(),
+                        np.ctypeslib.as_array(t2.left_grad, shape=(seg + 1,)).copy(),
+                        np.ctypeslib.as_array(t2.right_grad, shape=(seg + 1,)).copy()))
+    # rasterisation of the taper into a background + a waveguide
+    tf2, A2 = new_tf(L, 40, 30, 50)
+    L.timefield3d_set_background(tf2, 1.447 ** 2, 1.0)
+    L.timefield3d_set_top_half_plane(tf2, 1.93 ** 2, 1.0, 20)
+    L.timefield3d_set_taper(tf2, 3.51 ** 2, 1.0, 20, 20, 14, 26, 0, 10, 12, 16)
+    L.adjoint3d_taper_insert(tf2, tp)
+    L.timefield3d_set_waveguide(tf2, 3.51 ** 2, 10, 30, 35, 50, 12, 16)
+    save("F6_taper.npz", dims=np.array([I, J, K]), seg=seg, start=start, end=end, bottom=bottom, top=top, T=T,
+         left=left, right=right, fwd_hist=fwd, adj_hist=adj, ep_grad=epg,
+         ep_diff=ep_diff, left_grad=lg, right_grad=rg, goals=np.array([0.05, 1e-7, 0.05, 1e-7]),
+         upd_flag=np.array([u[0] for u in updates]), upd_left=np.stack([u[1] for u in updates]),
+         upd_right=np.stack([u[2] for u in updates]), upd_lg=np.stack([u[3] for u in updates]),
+         upd_rg=np.stack([u[4] for u in updates]), raster_dims=np.array([40, 30, 50]), raster_ep=A2["ep"].copy(),
+         raster_mu=A2["mu"].copy())
+    manifest["F6_taper"] = {"grid": [I, J, K], "seg": seg, "start": start, "end": end, "bottom": bottom,
+                            "top": top, "T": T, "min_width": 3,
+                            "fields": "pattern_fields(T, 6n, 7, 13) forward, (11, 5) adjoint",
+                            "updates": "asym 0.05, asym 1e-7, sym 0.05, sym 1e-7 on copies",
+                            "raster": "background 1.447^2; top half plane 1.93^2 from y=20; "
+                                      "set_taper(3.51^2,1,20,20,14,26,0,10,12,16); insert; "
+                                      "waveguide(3.51^2,10,30,35,50,12,16)"}
+    L.timefield3d_free(tf)
+    L.timefield3d_free(tf2)
+
+
+def f9(L, manifest):
+    """F9: complex_exp (Taylor series) on the imaginary axis and off it, complex_angle, complex_sqrt."""
+    rng = np.random.default_rng(909)
+    xs = np.concatenate([rng.uniform(0, 2 * np.pi, 150), rng.uniform(-20, 20, 50)])
+    zs = rng.uniform(-3, 3, (60, 2))
+    e_im = np.array([[r.re, r.im] for r in (L.complex_exp(R.Complex(0, x)) for x in xs)])
+    e_z = np.array([[r.re, r.im] for r in (L.complex_exp(R.Complex(a, b)) for a, b in zs)])
+    ang = np.array([L.complex_angle(R.Complex(a, b)) for a, b in zs])
+    sq = np.array([[r.re, r.im] for r in (L.complex_sqrt(R.Complex(a, b)) for a, b in zs)])
+    save("F9_complex.npz", xs=xs, exp_imag=e_im, zs=zs, exp_z=e_z, angle=ang, sqrt=sq)
+    manifest["F9_complex"] = {"inputs": "150 U[0,2pi) + 50 U[-20,20) imaginary; 60 U[-3,3)^2 complex"}
+
+
+def main():
+    L = R.lib()
+    manifest = {"reference_sources_sha256": {}, "compiler": "gcc -O3 -fopenmp -fPIC (oracle/Makefile ref)",
+                "disc_um": DISC, "dt": DT}
+    for fn in ("complex.c", "matrix.c", "mode.c", "timefield3d.c", "adjoint3d.c"):
+        with open(os.path.join(REF_DIR, fn), "rb") as fh:
+            manifest["reference_sources_sha256"][fn] = hashlib.sha256(fh.read()).hexdigest()
+    f1_f2(L, manifest)
+    f3(L, manifest)
+    f4(L, manifest)
+    f5(L, manifest)
+    f6(L, manifest)
+    f9(L, manifest)
+    with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1, sort_keys=True)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
