@@ -1,0 +1,223 @@
+"""Python view of libfdtd3d.so, the MI355X FDTD library with the reference's C surface.
+
+The product is the C-ABI shared library (include/*.h); this module only loads it through
+ctypes and mirrors the reference struct layouts (timefield3d.h:15-27, mode.h:13-22,
+matrix.h:12-16, complex.h:11-15, adjoint3d.h:16-55) so that tests and bench.py call exactly
+what a C driver calls.  There is no fallback: if the library is missing or does not load,
+every entry point raises.
+
+Import with ``importlib.import_module("3d_fdtd_amd")`` (the package name starts with a digit).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libfdtd3d.so")
+
+
+class Complex(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class Matrix(C.Structure):
+    _fields_ = [("i", C.c_int), ("j", C.c_int), ("m", C.POINTER(Complex))]
+
+
+class Mode(C.Structure):
+    _fields_ = [("i", C.c_int), ("j", C.c_int), ("disc", C.c_double), ("pml", Matrix), ("ep", Matrix),
+                ("mu", Matrix), ("fields", Matrix)]
+
+
+class TimeField(C.Structure):
+    _fields_ = [("i", C.c_int), ("j", C.c_int), ("k", C.c_int), ("disc", C.c_double),
+                ("ep", C.POINTER(C.c_float)), ("mu", C.POINTER(C.c_float)), ("sig", C.POINTER(C.c_float)),
+                ("fields", C.POINTER(C.c_double)), ("old_fields", C.POINTER(C.c_double)),
+                ("fourier_fields", C.POINTER(Complex)), ("sources", C.POINTER(C.c_float))]
+
+
+class Taper(C.Structure):
+    _fields_ = [("ep", C.c_double), ("mu", C.c_double), ("seg_num", C.c_int), ("start", C.c_int),
+                ("end", C.c_int), ("bottom", C.c_int), ("top", C.c_int), ("duration", C.c_int),
+                ("left", C.POINTER(C.c_int)), ("right", C.POINTER(C.c_int)),
+                ("forward_fields", C.POINTER(C.c_double)), ("adjoint_fields", C.POINTER(C.c_double)),
+                ("ep_gradient", C.POINTER(C.c_double)), ("left_grad", C.POINTER(C.c_double)),
+                ("right_grad", C.POINTER(C.c_double))]
+
+
+class Block(C.Structure):
+    _fields_ = [("t", C.c_int), ("x_start", C.c_int), ("x_end", C.c_int), ("y_start", C.c_int),
+                ("y_end", C.c_int), ("z_start", C.c_int), ("z_end", C.c_int),
+                ("forward_fields", C.POINTER(C.c_double)), ("gradient", C.POINTER(C.c_double))]
+
+
+_d, _i, _f = C.c_double, C.c_int, C.c_float
+_P = C.POINTER
+
+# name -> (argtypes, restype); the C declarations are in include/*.h
+SIGNATURES = {
+    # timefield3d.h
+    "timefield3d_new": ([_i, _i, _i, _d], TimeField),
+    "timefield3d_free": ([TimeField], None),
+    "timefield3d_copy": ([TimeField], TimeField),
+    "timefield3d_zero_fields": ([TimeField], None),
+    "timefield3d_get_fields": ([TimeField], _P(_d)),
+    "timefield3d_get_edge_fields": ([TimeField], _P(_d)),
+    "timefield3d_update_old_edge_fields": ([TimeField], None),
+    "timefield3d_set_background": ([TimeField, _f, _f], None),
+    "timefield3d_set_top_half_plane": ([TimeField, _f, _f, _i], None),
+    "timefield3d_set_background_permittivity": ([TimeField, _f], None),
+    "timefield3d_set_background_permeability": ([TimeField, _f], None),
+    "timefield3d_set_background_electric_conductivity": ([TimeField, _f], None),
+    "timefield3d_set_background_magnetic_conductivity": ([TimeField, _f], None),
+    "timefield3d_set_background_complex_permittivity": ([TimeField, _d, _f, _f], None),
+    "timefield3d_set_prism_permittivity": ([TimeField, _f, _i, _i, _i, _i, _i, _i], None),
+    "timefield3d_set_prism_electric_conductivity": ([TimeField, _f, _i, _i, _i, _i, _i, _i], None),
+    "timefield3d_set_prism_complex_permittivity": ([TimeField, _d, _f, _f, _i, _i, _i, _i, _i, _i], None),
+    "timefield3d_set_waveguide": ([TimeField, _f, _i, _i, _i, _i, _i, _i], None),
+    "timefield3d_set_taper": ([TimeField, _d, _d, _i, _i, _i, _i, _i, _i, _i, _i], None),
+    "timefield3d_apply_H_timestep": ([TimeField, _d], None),
+    "timefield3d_apply_E_timestep": ([TimeField, _d], None),
+    "timefield3d_apply_H_timestep_zsection": ([TimeField, _d, _i, _i], None),
+    "timefield3d_apply_E_timestep_zsection": ([TimeField, _d, _i, _i], None),
+    "timefield3d_apply_H_first_order_abc": ([TimeField, _P(_d), _d], None),
+    "timefield3d_apply_E_first_order_abc": ([TimeField, _P(_d), _d], None),
+    "timefield3d_apply_H_first_order_abc_zsection": ([TimeField, _d, _i, _i], None),
+    "timefield3d_apply_E_first_order_abc_zsection": ([TimeField, _d, _i, _i], None),
+    "timefield3d_apply_timestep": ([TimeField, _d], None),
+    "timefield3d_apply_timestep_omp": ([TimeField, _d], None),
+    "timefield3d_apply_timestep_omp_full": ([TimeField, _d], None),
+    "timefield3d_update_fourier_fields": ([TimeField, _d], None),
+    "timefield3d_update_fourier_fields_y_slice": ([TimeField, _d, _i], None),
+    "timefield3d_update_fourier_fields_zsection": ([TimeField, _d, _i, _i], None),
+    "timefield3d_update_fourier_fields_y_slice_zsection": ([TimeField, _d, _i, _i, _i], None),
+    "timefield3d_update_fourier_fields_omp": ([TimeField, _d], None),
+    "timefield3d_update_fourier_fields_y_slice_omp": ([TimeField, _d, _i], None),
+    "timefield3d_mode_source": ([TimeField, _i, _i, _i, Mode, _d, _d, _i], None),
+    "timefield3d_mode_source_adjoint": ([TimeField, _i, _i, _i, Mode, _d, _i], None),
+    "timefield3d_TE_gaussian_source": ([TimeField, _i, _i, _i, _d, _d, _d, _d], None),
+    "timefield3d_TM_gaussian_source": ([TimeField, _i, _i, _i, _d, _d, _d, _d], None),
+    "timefield3d_mode_overlap": ([TimeField, _i, _i, _i, Mode, _d, _i, _P(Complex)], Complex),
+    "timefield3d_file_write_permittivity_y_slice": ([TimeField, C.c_char_p, _i, _i, _i, _i, _i, _d, _d], None),
+    # mode.h
+    "mode_new": ([_i, _i, _d], Mode),
+    "mode_free": ([Mode], None),
+    "mode_copy": ([Mode], Mode),
+    "mode_smul": ([Complex, Mode], None),
+    "mode_gaussian": ([_i, _i, _d, Complex, Complex, _d, _d, _i], Mode),
+    "mode_power_inner_product": ([Mode, Mode], Complex),
+    "mode_power_normalize": ([Mode], None),
+    # complex.h
+    "complex_exp": ([Complex], Complex),
+    "complex_angle": ([Complex], _d),
+    "complex_sqrt": ([Complex], Complex),
+    "complex_div": ([Complex, Complex], Complex),
+    # adjoint3d.h
+    "ajoint3d_new": ([_i, _i, _i, _i, _i, _i, _i], Block),
+    "adjoint3d_free": ([Block], None),
+    "adjoint3d_store_fwd_fields": ([Block, TimeField, _i], None),
+    "adjoint3d_update_gradient": ([Block, TimeField, _i], None),
+    "adjoint3d_reset_gradient": ([Block], None),
+    "adjoint3d_reset_forward_fields": ([Block], None),
+    "adjoint3d_taper_new": ([_d, _d, _i, _i, _i, _P(_i), _P(_i), _i, _i, _i], Taper),
+    "adjoint3d_taper_insert": ([TimeField, Taper], None),
+    "adjoint3d_taper_store_forward_fields": ([Taper, TimeField, _i], None),
+    "adjoint3d_taper_store_adjoint_fields": ([Taper, TimeField, _i], None),
+    "adjoint3d_taper_calculate_ep_gradient": ([Taper], None),
+    "adjoint3d_taper_calculate_gradients_from_ep_gradient": ([Taper, _d], None),
+    "adjoint3d_taper_update_asymmetric": ([Taper, _d, _d, _i], _i),
+    "adjoint3d_taper_update_symmetric": ([Taper, _d, _d, _i], _i),
+    # fdtd3d_ext.h
+    "fdtd3d_ext_sync_to_device": ([TimeField], _i),
+    "fdtd3d_ext_sync_from_device": ([TimeField], _i),
+    "fdtd3d_ext_upload_source_plane": ([TimeField, _i], _i),
+    "fdtd3d_ext_timer_start": ([TimeField], _i),
+    "fdtd3d_ext_timer_stop": ([TimeField, _P(_d)], _i),
+    "fdtd3d_ext_synchronize": ([TimeField], _i),
+    "fdtd3d_ext_steps": ([TimeField, _d, _i], _i),
+    "fdtd3d_ext_last_error": ([], C.c_char_p),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libfdtd3d.so (built by __graft_entry__.build()); raise if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (no CPU fallback exists)")
+    L = C.CDLL(LIB_PATH)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def check(rc, what="fdtd3d call"):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {lib().fdtd3d_ext_last_error().decode()}")
+
+
+def _view(ptr, count, ctype, dtype):
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,)).view(dtype)
+
+
+def arrays(tf):
+    """numpy views (no copy) of a grid's host arrays, reference layout."""
+    n = tf.i * tf.j * tf.k
+    return {
+        "ep": _view(tf.ep, 3 * n, C.c_float, np.float32),
+        "mu": _view(tf.mu, 3 * n, C.c_float, np.float32),
+        "sig": _view(tf.sig, 6 * n, C.c_float, np.float32),
+        "fields": _view(tf.fields, 6 * n, C.c_double, np.float64),
+        "fourier": _view(tf.fourier_fields, 12 * n, C.c_double, np.float64),
+        "sources": _view(tf.sources, 6 * n, C.c_float, np.float32),
+    }
+
+
+def mode_fields(md):
+    """Interleaved (re, im) view of a mode's 6*i*j complex fields."""
+    return _view(md.fields.m, 12 * md.i * md.j, C.c_double, np.float64)
+
+
+def mode_from_array(mi, mj, disc, values):
+    """New mode whose fields are the interleaved complex array `values` (12*mi*mj doubles)."""
+    md = lib().mode_new(mi, mj, disc)
+    mode_fields(md)[:] = values
+    return md
+
+
+class Grid:
+    """Owns one timefield3d (host mirrors + device state)."""
+
+    def __init__(self, I, J, K, disc):
+        self.tf = lib().timefield3d_new(I, J, K, disc)
+        self.I, self.J, self.K, self.disc = I, J, K, disc
+        self.a = arrays(self.tf)
+
+    def free(self):
+        if self.tf is not None:
+            lib().timefield3d_free(self.tf)
+            self.tf = None
+
+    def push(self):
+        check(lib().fdtd3d_ext_sync_to_device(self.tf), "sync_to_device")
+
+    def pull(self):
+        check(lib().fdtd3d_ext_sync_from_device(self.tf), "sync_from_device")
+        return self.a
+
+    def step(self, dt, n=1):
+        if n == 1:
+            lib().timefield3d_apply_timestep_omp_full(self.tf, dt)
+        else:
+            check(lib().fdtd3d_ext_steps(self.tf, dt, n), "steps")
+
+    def fields(self):
+        self.pull()
+        return self.a["fields"]

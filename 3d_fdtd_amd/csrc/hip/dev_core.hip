@@ -1,0 +1,424 @@
+// Device grid lifecycle, materials, host<->device transfers, timers.
+// Replaces the storage side of timefield3d.c:6-136 with device-resident FP32 structure-of-arrays
+// fields and a material-id volume (see fdtd3d_dev.h for the C-ABI contract).
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "dev_internal.h"
+
+namespace fdtd {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int fail(const char *where, hipError_t e) {
+    set_error(std::string(where) + ": " + hipGetErrorString(e));
+    return -1;
+}
+
+namespace {
+
+struct MatKey {
+    float v[12];
+    bool operator==(const MatKey &o) const { return std::memcmp(v, o.v, sizeof v) == 0; }
+};
+
+struct MatKeyHash {
+    size_t operator()(const MatKey &k) const {
+        uint64_t h = 1469598103934665603ull;
+        const unsigned char *p = reinterpret_cast<const unsigned char *>(k.v);
+        for (size_t i = 0; i < sizeof k.v; i++) h = (h ^ p[i]) * 1099511628211ull;
+        return (size_t)h;
+    }
+};
+
+// AoS double[6N] -> six SoA float arrays.
+__global__ void k_aos_to_soa(const double *__restrict__ aos, float *__restrict__ base, long long N, long long Nal) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+#pragma unroll
+    for (int m = 0; m < 6; m++) base[m * Nal + c] = (float)aos[6 * c + m];
+}
+
+__global__ void k_soa_to_aos(const float *__restrict__ base, double *__restrict__ aos, long long N, long long Nal) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+#pragma unroll
+    for (int m = 0; m < 6; m++) aos[6 * c + m] = (double)base[m * Nal + c];
+}
+
+}  // namespace
+
+int ensure_coefficients(fdtd_dev *d, double dt) {
+    if (!d->have_mats) {
+        set_error("materials were never uploaded");
+        return -1;
+    }
+    if (d->coef_dt == dt) return 0;
+    std::vector<float> tab((size_t)2 * d->nmat * kCoefStride, 0.0f);
+    const double inv_dx = 1 / d->disc;
+    for (int id = 0; id < d->nmat; id++) {
+        const float *pr = &d->props[(size_t)id * kPropStride];
+        for (int scheme = 0; scheme < 2; scheme++) {
+            float *r = &tab[((size_t)scheme * d->nmat + id) * kCoefStride];
+            for (int m = 0; m < 3; m++) {
+                double ep = pr[m], mu = pr[3 + m], se = pr[6 + m], sm = pr[9 + m];
+                if (scheme == 0) {
+                    // semi-implicit (timefield3d.c:1823-1833 and 1743-1753; sigma* counted twice)
+                    double inv_e = 1.0 / (ep / dt + 0.5 * se), keep_e = ep / dt - 0.5 * se;
+                    double inv_h = 1.0 / (mu / dt + 0.5 * sm), keep_h = mu / dt - 0.5 * sm;
+                    r[0 + m] = (float)(inv_e * keep_e);
+                    r[3 + m] = (float)(inv_e * inv_dx);
+                    r[6 + m] = (float)inv_e;
+                    r[9 + m] = (float)(inv_h * (keep_h - sm));
+                    r[12 + m] = (float)(inv_h * inv_dx);
+                    r[15 + m] = (float)inv_h;
+                } else {
+                    // explicit (timefield3d.c:1593-1603 and 1664-1674)
+                    r[0 + m] = (float)(1.0 - dt * se / ep);
+                    r[3 + m] = (float)(dt * inv_dx / ep);
+                    r[6 + m] = (float)(dt / ep);
+                    r[9 + m] = (float)(1.0 - dt * sm / mu);
+                    r[12 + m] = (float)(dt * inv_dx / mu);
+                    r[15 + m] = (float)(dt / mu);
+                }
+            }
+            // Mur: E comp t on a face normal to axis a pairs ep_t with mu_d, H comp t pairs
+            // ep_d with mu_t, d the third axis (timefield3d.c:2275-2721).
+            for (int a = 0; a < 3; a++) {
+                for (int q = 0; q < 2; q++) {
+                    int t = (a + 1 + q) % 3, dd = 3 - a - t;
+                    double vE = 1 / std::sqrt((double)pr[t] * (double)pr[3 + dd]);
+                    double vH = 1 / std::sqrt((double)pr[dd] * (double)pr[3 + t]);
+                    r[18 + 2 * a + q] = (float)((vE * dt - d->disc) / (vE * dt + d->disc));
+                    r[24 + 2 * a + q] = (float)((vH * dt - d->disc) / (vH * dt + d->disc));
+                }
+            }
+        }
+    }
+    FDTD_TRY(hipMemcpyAsync(d->d_coef, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    d->coef_dt = dt;
+    return 0;
+}
+
+const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, double mdisc, int *err) {
+    *err = 0;
+    const size_t n = (size_t)12 * mi * mj;
+    d->use_clock++;
+    for (auto &mc : d->modes) {
+        if (mc.host_key != mode || mc.mi != mi || mc.mj != mj || mc.mdisc != mdisc) continue;
+        // Re-upload whenever the caller's mode changed (in-place renormalisation, mode_smul).
+        mc.last_use = d->use_clock;
+        if (std::memcmp(mode, mc.shadow.data(), n * sizeof(double)) == 0) return mc.dev;
+        std::memcpy(mc.shadow.data(), mode, n * sizeof(double));
+        hipError_t e = hipMemcpyAsync(mc.dev, mode, n * sizeof(double), hipMemcpyHostToDevice, d->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+        if (e != hipSuccess) {
+            *err = fail("mode upload", e);
+            return nullptr;
+        }
+        return mc.dev;
+    }
+    // new entry; keep at most 16 cached modes per grid (evict the least recently used)
+    if (d->modes.size() >= 16) {
+        size_t victim = 0;
+        for (size_t q = 1; q < d->modes.size(); q++)
+            if (d->modes[q].last_use < d->modes[victim].last_use) victim = q;
+        (void)hipFree(d->modes[victim].dev);
+        d->modes.erase(d->modes.begin() + victim);
+    }
+    ModeCache mc;
+    mc.host_key = mode;
+    mc.mi = mi;
+    mc.mj = mj;
+    mc.mdisc = mdisc;
+    mc.shadow.assign(mode, mode + n);
+    mc.last_use = d->use_clock;
+    hipError_t e = hipMalloc(&mc.dev, n * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyAsync(mc.dev, mode, n * sizeof(double), hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess) {
+        *err = fail("mode upload", e);
+        return nullptr;
+    }
+    d->modes.push_back(std::move(mc));
+    return d->modes.back().dev;
+}
+
+}  // namespace fdtd
+
+using namespace fdtd;
+
+extern "C" {
+
+const char *fdtd_dev_last_error(void) { return g_err.c_str(); }
+
+fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
+    if (I < 3 || J < 3 || K < 3) {
+        set_error("grid dimensions must be at least 3 in every axis");
+        return nullptr;
+    }
+    fdtd_dev *d = new fdtd_dev();
+    d->I = I;
+    d->J = J;
+    d->K = K;
+    d->P = (long long)I * J;
+    d->N = d->P * K;
+    d->disc = disc;
+    d->nmat = 0;
+    d->mat_bytes = 1;
+    d->mat = nullptr;
+    d->d_props = nullptr;
+    d->d_coef = nullptr;
+    d->coef_dt = NAN;
+    d->have_mats = false;
+    d->fourier = nullptr;
+    d->partials = nullptr;
+    d->red_out = nullptr;
+    d->partial_cap = 0;
+    d->use_clock = 0;
+    d->field_base = nullptr;
+    d->shell = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    // fields: 6 arrays, each N floats, each starting on a 256-byte boundary
+    d->Nal = (d->N + 63) / 64 * 64;
+    if (e == hipSuccess) e = hipMalloc(&d->field_base, (size_t)6 * d->Nal * sizeof(float));
+    if (e == hipSuccess) e = hipMemsetAsync(d->field_base, 0, (size_t)6 * d->Nal * sizeof(float), d->stream);
+    for (int m = 0; m < 6; m++) d->F[m] = d->field_base + m * d->Nal;
+    // shells: faces x0,y0,z0,xend,yend,zend; each 2 layers x 4 components x area
+    const long long area[3] = {(long long)J * K, (long long)K * I, (long long)I * J};
+    long long off = 0;
+    for (int f = 0; f < 6; f++) {
+        d->shell_off[f] = off;
+        off += 8 * area[f % 3];
+    }
+    if (e == hipSuccess) e = hipMalloc(&d->shell, (size_t)off * sizeof(float));
+    if (e == hipSuccess) e = hipMemsetAsync(d->shell, 0, (size_t)off * sizeof(float), d->stream);
+    if (e == hipSuccess) e = hipMalloc(&d->red_out, sizeof(double2) * 4);
+    if (e == hipSuccess) e = hipEventCreate(&d->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&d->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess) {
+        fail("fdtd_dev_create", e);
+        fdtd_dev_destroy(d);
+        return nullptr;
+    }
+    return d;
+}
+
+void fdtd_dev_destroy(fdtd_dev *d) {
+    if (!d) return;
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    (void)hipFree(d->field_base);
+    (void)hipFree(d->mat);
+    (void)hipFree(d->d_props);
+    (void)hipFree(d->d_coef);
+    (void)hipFree(d->shell);
+    (void)hipFree(d->fourier);
+    (void)hipFree(d->partials);
+    (void)hipFree(d->red_out);
+    for (auto &p : d->src) (void)hipFree(p.J);
+    for (auto &m : d->modes) (void)hipFree(m.dev);
+    if (d->ev0) (void)hipEventDestroy(d->ev0);
+    if (d->ev1) (void)hipEventDestroy(d->ev1);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const float *sig) {
+    // kernels already enqueued may still read the id volume and the tables
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    std::unordered_map<MatKey, int, MatKeyHash> ids;
+    std::vector<uint16_t> cell_id((size_t)d->N);
+    std::vector<float> props;
+    MatKey prev;
+    int prev_id = -1;
+    for (long long c = 0; c < d->N; c++) {
+        MatKey k;
+        std::memcpy(k.v, ep + 3 * c, 3 * sizeof(float));
+        std::memcpy(k.v + 3, mu + 3 * c, 3 * sizeof(float));
+        std::memcpy(k.v + 6, sig + 6 * c, 6 * sizeof(float));
+        int id;
+        if (prev_id >= 0 && k == prev) {
+            id = prev_id;
+        } else {
+            auto it = ids.find(k);
+            if (it == ids.end()) {
+                id = (int)ids.size();
+                if (id >= 65536) {
+                    set_error("more than 65536 distinct (ep, mu, sigma) tuples are not supported");
+                    return -2;
+                }
+                ids.emplace(k, id);
+                props.resize((size_t)(id + 1) * kPropStride, 0.0f);
+                std::memcpy(&props[(size_t)id * kPropStride], k.v, sizeof k.v);
+            } else {
+                id = it->second;
+            }
+            prev = k;
+            prev_id = id;
+        }
+        cell_id[c] = (uint16_t)id;
+    }
+    int nmat = (int)ids.size();
+    int bytes = nmat <= 256 ? 1 : 2;
+    if (d->mat && bytes != d->mat_bytes) {
+        FDTD_TRY(hipFree(d->mat));
+        d->mat = nullptr;
+    }
+    if (!d->mat) FDTD_TRY(hipMalloc(&d->mat, (size_t)d->N * bytes));
+    if (bytes == 1) {
+        std::vector<uint8_t> c8((size_t)d->N);
+        for (long long c = 0; c < d->N; c++) c8[c] = (uint8_t)cell_id[c];
+        FDTD_TRY(hipMemcpy(d->mat, c8.data(), (size_t)d->N, hipMemcpyHostToDevice));
+    } else {
+        FDTD_TRY(hipMemcpy(d->mat, cell_id.data(), (size_t)d->N * 2, hipMemcpyHostToDevice));
+    }
+    if (nmat != d->nmat || !d->d_props) {
+        (void)hipFree(d->d_props);
+        (void)hipFree(d->d_coef);
+        d->d_props = nullptr;
+        d->d_coef = nullptr;
+        FDTD_TRY(hipMalloc(&d->d_props, (size_t)nmat * kPropStride * sizeof(float)));
+        FDTD_TRY(hipMalloc(&d->d_coef, (size_t)2 * nmat * kCoefStride * sizeof(float)));
+    }
+    FDTD_TRY(hipMemcpy(d->d_props, props.data(), props.size() * sizeof(float), hipMemcpyHostToDevice));
+    d->props = std::move(props);
+    d->nmat = nmat;
+    d->mat_bytes = bytes;
+    d->coef_dt = NAN;
+    d->have_mats = true;
+    return 0;
+}
+
+int fdtd_dev_set_fields(fdtd_dev *d, const double *fields) {
+    double *tmp = nullptr;
+    FDTD_TRY(hipMalloc(&tmp, (size_t)6 * d->N * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, fields, (size_t)6 * d->N * sizeof(double), hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) {
+        unsigned nb = (unsigned)((d->N + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(nb), dim3(kBlock), 0, d->stream, tmp, d->field_base, d->N, d->Nal);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail("fdtd_dev_set_fields", e);
+    return 0;
+}
+
+int fdtd_dev_get_fields(fdtd_dev *d, double *fields) {
+    double *tmp = nullptr;
+    FDTD_TRY(hipMalloc(&tmp, (size_t)6 * d->N * sizeof(double)));
+    unsigned nb = (unsigned)((d->N + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(nb), dim3(kBlock), 0, d->stream, d->field_base, tmp, d->N, d->Nal);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(fields, tmp, (size_t)6 * d->N * sizeof(double), hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail("fdtd_dev_get_fields", e);
+    return 0;
+}
+
+int fdtd_dev_set_sources(fdtd_dev *d, const float *sources) {
+    // A z plane is kept on the device when any of its source values is non-zero.
+    std::vector<int> planes;
+    for (int k = 0; k < d->K; k++) {
+        const float *p = sources + 6 * d->P * k;
+        bool nz = false;
+        for (long long q = 0; q < 6 * d->P && !nz; q++) nz = p[q] != 0.0f;
+        if (nz) planes.push_back(k);
+    }
+    if ((int)planes.size() > kMaxSrcPlanes) {
+        set_error("sources on more than 8 distinct z planes are not supported by the device path");
+        return -3;
+    }
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    for (auto &p : d->src) (void)hipFree(p.J);
+    d->src.clear();
+    std::vector<float> soa((size_t)6 * d->P);
+    for (int k : planes) {
+        const float *p = sources + 6 * d->P * k;
+        for (long long c = 0; c < d->P; c++)
+            for (int m = 0; m < 6; m++) soa[m * d->P + c] = p[6 * c + m];
+        SrcPlane sp;
+        sp.k = k;
+        FDTD_TRY(hipMalloc(&sp.J, (size_t)6 * d->P * sizeof(float)));
+        FDTD_TRY(hipMemcpy(sp.J, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
+        d->src.push_back(sp);
+    }
+    return 0;
+}
+
+int fdtd_dev_get_sources(fdtd_dev *d, float *sources) {
+    std::memset(sources, 0, (size_t)6 * d->N * sizeof(float));
+    std::vector<float> soa((size_t)6 * d->P);
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    for (auto &sp : d->src) {
+        FDTD_TRY(hipMemcpy(soa.data(), sp.J, soa.size() * sizeof(float), hipMemcpyDeviceToHost));
+        float *p = sources + 6 * d->P * sp.k;
+        for (long long c = 0; c < d->P; c++)
+            for (int m = 0; m < 6; m++) p[6 * c + m] = soa[m * d->P + c];
+    }
+    return 0;
+}
+
+int fdtd_dev_zero(fdtd_dev *d) {
+    FDTD_TRY(hipMemsetAsync(d->field_base, 0, (size_t)6 * d->Nal * sizeof(float), d->stream));
+    long long shell_n = d->shell_off[5] + 8LL * d->I * d->J;
+    FDTD_TRY(hipMemsetAsync(d->shell, 0, (size_t)shell_n * sizeof(float), d->stream));
+    for (auto &p : d->src) FDTD_TRY(hipMemsetAsync(p.J, 0, (size_t)6 * d->P * sizeof(float), d->stream));
+    if (d->fourier) FDTD_TRY(hipMemsetAsync(d->fourier, 0, (size_t)6 * d->N * sizeof(double2), d->stream));
+    return 0;
+}
+
+int fdtd_dev_get_fourier(fdtd_dev *d, double *out) {
+    if (!d->fourier) {
+        std::memset(out, 0, (size_t)12 * d->N * sizeof(double));
+        return 0;
+    }
+    std::vector<double2> soa((size_t)6 * d->N);
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    FDTD_TRY(hipMemcpy(soa.data(), d->fourier, soa.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    for (long long c = 0; c < d->N; c++)
+        for (int m = 0; m < 6; m++) {
+            out[2 * (6 * c + m)] = soa[m * d->N + c].x;
+            out[2 * (6 * c + m) + 1] = soa[m * d->N + c].y;
+        }
+    return 0;
+}
+
+int fdtd_dev_set_fourier(fdtd_dev *d, const double *in) {
+    if (!d->fourier) FDTD_TRY(hipMalloc(&d->fourier, (size_t)6 * d->N * sizeof(double2)));
+    std::vector<double2> soa((size_t)6 * d->N);
+    for (long long c = 0; c < d->N; c++)
+        for (int m = 0; m < 6; m++) soa[m * d->N + c] = make_double2(in[2 * (6 * c + m)], in[2 * (6 * c + m) + 1]);
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    FDTD_TRY(hipMemcpy(d->fourier, soa.data(), soa.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int fdtd_dev_timer_start(fdtd_dev *d) {
+    FDTD_TRY(hipEventRecord(d->ev0, d->stream));
+    return 0;
+}
+
+int fdtd_dev_timer_stop(fdtd_dev *d, double *elapsed_ms) {
+    FDTD_TRY(hipEventRecord(d->ev1, d->stream));
+    FDTD_TRY(hipEventSynchronize(d->ev1));
+    float ms = 0;
+    FDTD_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    *elapsed_ms = ms;
+    return 0;
+}
+
+int fdtd_dev_synchronize(fdtd_dev *d) {
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+}  // extern "C"

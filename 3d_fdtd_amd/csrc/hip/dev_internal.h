@@ -1,0 +1,129 @@
+// Internal declarations shared by the HIP translation units of the device layer.
+// Not part of the C-ABI: the public entry points are declared in include/fdtd3d_dev.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "fdtd3d_dev.h"
+
+namespace fdtd {
+
+// Component order inside a cell: Ex, Ey, Ez, Hx, Hy, Hz (timefield3d.h:19-25).
+enum { EX = 0, EY, EZ, HX, HY, HZ };
+
+constexpr int kMaxSrcPlanes = 8;
+constexpr int kBlock = 256;
+
+// Per-material coefficient record (floats), one for each conductivity scheme:
+//   [0..2] caE, [3..5] cbE, [6..8] cjE, [9..11] caH, [12..14] cbH, [15..17] cjH,
+//   [18..23] Mur s for E on faces x,y,z (two tangential comps each, order b=(a+1)%3, c=(a+2)%3),
+//   [24..29] Mur s for H, [30..31] pad.
+constexpr int kCoefStride = 32;
+// Raw per-material properties kept for sources and reports: ep[3], mu[3], sig[6] (+4 pad).
+constexpr int kPropStride = 16;
+
+struct SrcPlaneArg {
+    int k;
+    const float *J;  // 6 SoA planes of I*J floats (Jx,Jy,Jz,Mx,My,Mz)
+};
+
+struct SrcPlane {
+    int k;
+    float *J;  // device, 6 * P floats
+};
+
+struct ModeCache {
+    const double *host_key = nullptr;
+    int mi = 0, mj = 0;
+    double mdisc = 0;
+    std::vector<double> shadow;  // last uploaded content
+    double *dev = nullptr;       // 12*mi*mj doubles (interleaved complex)
+    unsigned long long last_use = 0;
+};
+
+struct TaperHist {
+    const void *key;
+    int H, L, T;
+    float *hist[2];              // forward, adjoint: [T][L][H][2][6]
+    int *edge_i;                 // [2][L] edge pixel per z row (right, left)
+    std::vector<int> edge_host;  // cached copy of edge_i
+};
+
+struct BlockHist {
+    const void *key;
+    int box[6];
+    int T;
+    float *fwd;     // [T][nz][ny][nx][3]
+    double *grad;   // [nz][ny][nx]
+};
+
+}  // namespace fdtd
+
+struct fdtd_dev {
+    int I, J, K;
+    long long P;  // cells per z plane
+    long long N;  // cells
+    long long Nal;  // stride between component arrays (N rounded up to 64)
+    double disc;
+    hipStream_t stream;
+
+    float *field_base;  // one allocation, 6 component arrays of N floats
+    float *F[6];
+
+    // materials
+    int nmat;
+    int mat_bytes;        // 1 or 2
+    void *mat;            // per-cell id
+    std::vector<float> props;  // nmat * kPropStride
+    float *d_props;
+    float *d_coef;        // [2 schemes][nmat][kCoefStride]
+    double coef_dt;       // dt the coefficient table was built for (NaN = invalid)
+    bool have_mats;
+
+    // Mur shells: 6 faces x 2 layers x 4 comps x face area
+    float *shell;
+    long long shell_off[6];
+
+    // sources
+    std::vector<fdtd::SrcPlane> src;
+
+    // reference single-frequency DFT accumulators (6 SoA arrays of N double2), lazily allocated
+    double2 *fourier;
+
+    // overlap reduction scratch
+    double2 *partials;
+    double2 *red_out;
+    int partial_cap;
+
+    std::vector<fdtd::ModeCache> modes;
+    unsigned long long use_clock;
+
+    hipEvent_t ev0, ev1;
+};
+
+namespace fdtd {
+
+void set_error(const std::string &msg);
+int fail(const char *where, hipError_t e);
+#define FDTD_TRY(expr)                                              \
+    do {                                                            \
+        hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return ::fdtd::fail(#expr, _e);       \
+    } while (0)
+
+int ensure_coefficients(fdtd_dev *d, double dt);
+const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, double mdisc, int *err);
+
+// Blocks over (plane chunks x z chunks) remapped so that each XCD walks a contiguous range
+// (workgroups are dispatched round-robin over the 8 XCDs of an MI355X).
+__device__ __forceinline__ int xcd_logical_block(int bid, int nb) {
+    int x = bid & 7, s = bid >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + s;
+}
+
+}  // namespace fdtd

@@ -1,0 +1,531 @@
+// Sources, monitors and adjoint histories on gfx950.
+//   mode sources           timefield3d.c:2977-3068
+//   single-frequency DFT   timefield3d.c:2851-2975   (FP64 accumulators)
+//   mode overlap           timefield3d.c:3176-3239   (FP64 two-stage deterministic reduction)
+//   taper edge store/grad  adjoint3d.c:259-373
+//   3-D block adjoint      adjoint3d.c:48-105
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "dev_internal.h"
+
+namespace fdtd {
+namespace {
+
+std::map<const void *, TaperHist> g_tapers;
+std::map<const void *, BlockHist> g_blocks;
+
+template <typename MatT>
+__device__ __forceinline__ int mat_id(const void *mat, long long c) {
+    return (int)static_cast<const MatT *>(mat)[c];
+}
+
+__device__ __forceinline__ int mode_index_1d(int n, int center, double ratio, int mn) {
+    double dist = (n - center) * ratio;
+    return (int)dist + mn / 2;
+}
+
+// Forward soft source: J = A ep Re(mode e^{-i phase}), M = -(2r-1) A mu Re(...), stored as float;
+// adjoint CW source: J = (Re Hy, -Re Hx, 0), M = (-(2r-1) Re Ey, (2r-1) Re Ex, 0).
+// Cells outside the mode window keep their previous value.
+template <typename MatT>
+__global__ void k_mode_source(float *plane, const void *mat, const float *props, int I, int J, long long P, int cz,
+                              int cx, int cy, const double *mode, int mi, int mj, double ratio, double cp, double sp,
+                              double amp, int msign, int adjoint) {
+#pragma clang fp contract(off)
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int j = (int)(p / I), i = (int)(p - (long long)j * I);
+    const int mx = mode_index_1d(i, cx, ratio, mi), my = mode_index_1d(j, cy, ratio, mj);
+    if (mx < 0 || mx >= mi || my < 0 || my >= mj) return;
+    const double *md = mode + 12 * (mx + (long long)mi * my);
+    if (!adjoint) {
+        const float *pr = props + (long long)mat_id<MatT>(mat, p + (long long)cz * P) * kPropStride;
+        for (int m = 0; m < 6; m++) {
+            double proj = md[2 * m] * cp + md[2 * m + 1] * sp;
+            double v;
+            if (m < 3) v = amp * (double)pr[m] * proj;
+            else v = (double)(-msign) * amp * (double)pr[3 + m % 3] * proj;
+            plane[m * P + p] = (float)v;
+        }
+    } else {
+        double rEx = md[0] * cp + md[1] * sp;
+        double rEy = md[2] * cp + md[3] * sp;
+        double rHx = md[6] * cp + md[7] * sp;
+        double rHy = md[8] * cp + md[9] * sp;
+        plane[0 * P + p] = (float)rHy;
+        plane[1 * P + p] = (float)(-rHx);
+        plane[2 * P + p] = 0.0f;
+        plane[3 * P + p] = (float)((double)(-1 * msign) * rEy);
+        plane[4 * P + p] = (float)((double)msign * rEx);
+        plane[5 * P + p] = 0.0f;
+    }
+}
+
+__global__ void k_plane_masked_copy(float *plane, const float *src, const unsigned char *mask, long long P) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    if (mask && !mask[p]) return;
+    for (int m = 0; m < 6; m++) plane[m * P + p] = src[m * P + p];
+}
+
+// Overlap stage 1: per-block FP64 partial sums of Hx mEy - Hy mEx + Ey mHx - Ex mHy.
+__global__ __launch_bounds__(kBlock) void k_overlap_partial(const float *Ex, const float *Ey, const float *Hx,
+                                                            const float *Hy, int I, int J, long long P, int cz,
+                                                            int cx, int cy, const double *mode, int mi, int mj,
+                                                            double ratio, int reverse, double2 *partials) {
+    __shared__ double sre[kBlock / 64], sim[kBlock / 64];
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double re = 0.0, im = 0.0;
+    if (p < P) {
+        const int j = (int)(p / I), i = (int)(p - (long long)j * I);
+        const int mx = mode_index_1d(i, cx, ratio, mi), my = mode_index_1d(j, cy, ratio, mj);
+        if (mx >= 0 && mx < mi && my >= 0 && my < mj) {
+            const double *md = mode + 12 * (mx + (long long)mi * my);
+            double mEx_r = md[0], mEx_i = md[1], mEy_r = md[2], mEy_i = md[3];
+            double mHx_r = md[6], mHx_i = md[7], mHy_r = md[8], mHy_i = md[9];
+            if (reverse == 1) {  // mH -> -conj(mH)
+                mHx_r = -mHx_r;
+                mHy_r = -mHy_r;
+            }
+            const long long c = p + (long long)cz * P;
+            const double ex = Ex[c], ey = Ey[c], hx = Hx[c], hy = Hy[c];
+            re = (hx * mEy_r - hy * mEx_r) + (ey * mHx_r - ex * mHy_r);
+            im = (hx * mEy_i - hy * mEx_i) + (ey * mHx_i - ex * mHy_i);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_down(re, off, 64);
+        im += __shfl_down(im, off, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sre[w] = re;
+        sim[w] = im;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0, s = 0;
+        for (int q = 0; q < kBlock / 64; q++) {
+            r += sre[q];
+            s += sim[q];
+        }
+        partials[blockIdx.x] = make_double2(r, s);
+    }
+}
+
+// Overlap stage 2: one block sums the partials in a fixed order and scales by -dx^2/2.
+__global__ __launch_bounds__(kBlock) void k_overlap_final(const double2 *partials, int n, double scale,
+                                                          double2 *out) {
+    __shared__ double sre[kBlock], sim[kBlock];
+    double r = 0, s = 0;
+    for (int q = threadIdx.x; q < n; q += kBlock) {
+        r += partials[q].x;
+        s += partials[q].y;
+    }
+    sre[threadIdx.x] = r;
+    sim[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sre[threadIdx.x] += sre[threadIdx.x + w];
+            sim[threadIdx.x] += sim[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = make_double2(scale * sre[0], scale * sim[0]);
+}
+
+// F += f * twiddle over a z range, optionally one y row (timefield3d.c:2851-2975).
+__global__ void k_dft(double2 *F, const float *base, long long Nal, long long N, int I, int J, long long P, int k0,
+                      int k1, int yslice, double twr, double twi) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c;
+    if (yslice >= 0) {
+        long long n = (long long)I * (k1 - k0);
+        if (t >= n) return;
+        int k = k0 + (int)(t / I), i = (int)(t % I);
+        c = i + (long long)I * (yslice + (long long)J * k);
+    } else {
+        long long n = P * (k1 - k0);
+        if (t >= n) return;
+        c = t + (long long)k0 * P;
+    }
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        double v = base[m * Nal + c];
+        double2 a = F[m * N + c];
+        a.x = a.x + v * twr;
+        a.y = a.y + v * twi;
+        F[m * N + c] = a;
+    }
+}
+
+// Gather the six components at the right (side 0) and left (side 1) edge pixels of every
+// taper row into hist[t][k][j][side][m] (adjoint3d.c:259-337).
+__global__ void k_taper_gather(float *hist, const float *base, long long Nal, const int *edge_i, int I, int J,
+                               int start, int L, int bottom, int H, int t) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * H * L) return;
+    int side = q & 1, jj = (q >> 1) % H, kk = (q >> 1) / H;
+    int i = edge_i[side * L + kk];
+    long long c = i + (long long)I * ((bottom + jj) + (long long)J * (start + kk));
+    float *o = hist + 6LL * (side + 2LL * (jj + (long long)H * (kk + (long long)L * t)));
+#pragma unroll
+    for (int m = 0; m < 6; m++) o[m] = base[m * Nal + c];
+}
+
+// g[side,j,k] = sum_t sum_{m<3} fwd*adj in FP64, t outer and m inner (adjoint3d.c:339-373).
+__global__ void k_taper_grad(const float *fwd, const float *adj, int H, int L, int T, double *g) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * H * L) return;
+    const long long stride = 12LL * H * L;
+    const float *f = fwd + 6LL * q;
+    const float *a = adj + 6LL * q;
+    double s = 0;
+    for (int t = 0; t < T; t++, f += stride, a += stride) {
+        for (int m = 0; m < 3; m++) s += (double)f[m] * (double)a[m];
+    }
+    g[q] = s;
+}
+
+__global__ void k_block_gather(float *fwd, const float *base, long long Nal, int I, int J, int x0, int nx, int y0,
+                               int ny, int z0, int nz, int t) {
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long n = (long long)nx * ny * nz;
+    if (q >= n) return;
+    int xi = (int)(q % nx), yi = (int)((q / nx) % ny), zi = (int)(q / ((long long)nx * ny));
+    long long c = (x0 + xi) + (long long)I * ((y0 + yi) + (long long)J * (z0 + zi));
+    float *o = fwd + 3 * (q + n * t);
+    for (int m = 0; m < 3; m++) o[m] = base[m * Nal + c];
+}
+
+__global__ void k_block_accumulate(double *grad, const float *fwd, const float *base, long long Nal, int I, int J,
+                                   int x0, int nx, int y0, int ny, int z0, int nz, int t) {
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long n = (long long)nx * ny * nz;
+    if (q >= n) return;
+    int xi = (int)(q % nx), yi = (int)((q / nx) % ny), zi = (int)(q / ((long long)nx * ny));
+    long long c = (x0 + xi) + (long long)I * ((y0 + yi) + (long long)J * (z0 + zi));
+    const float *f = fwd + 3 * (q + n * t);
+    double s = 0;
+    for (int m = 0; m < 3; m++) s += (double)f[m] * (double)base[m * Nal + c];
+    grad[q] += s;
+}
+
+fdtd::SrcPlane *plane_for(fdtd_dev *d, int k, int *err) {
+    *err = 0;
+    for (auto &sp : d->src)
+        if (sp.k == k) return &sp;
+    if ((int)d->src.size() >= kMaxSrcPlanes) {
+        set_error("sources on more than 8 distinct z planes are not supported by the device path");
+        *err = -3;
+        return nullptr;
+    }
+    SrcPlane sp;
+    sp.k = k;
+    hipError_t e = hipMalloc(&sp.J, (size_t)6 * d->P * sizeof(float));
+    if (e == hipSuccess) e = hipMemsetAsync(sp.J, 0, (size_t)6 * d->P * sizeof(float), d->stream);
+    if (e != hipSuccess) {
+        *err = fail("source plane", e);
+        return nullptr;
+    }
+    d->src.push_back(sp);
+    return &d->src.back();
+}
+
+}  // namespace
+}  // namespace fdtd
+
+using namespace fdtd;
+
+extern "C" {
+
+int fdtd_dev_mode_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
+                         double cos_phase, double sin_phase, double amplitude, int reverse, int adjoint) {
+    if (cz < 0 || cz >= d->K) {
+        set_error("source plane outside the grid");
+        return -2;
+    }
+    if (!d->have_mats) {
+        set_error("materials were never uploaded");
+        return -1;
+    }
+    int err;
+    SrcPlane *sp = plane_for(d, cz, &err);
+    if (!sp) return err;
+    const double *dm = mode_on_device(d, mode, mi, mj, mdisc, &err);
+    if (!dm) return err;
+    unsigned nb = (unsigned)((d->P + kBlock - 1) / kBlock);
+    int msign = -1 + 2 * reverse;
+    double ratio = d->disc / mdisc;
+    if (d->mat_bytes == 1)
+        hipLaunchKernelGGL(k_mode_source<uint8_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, d->mat, d->d_props,
+                           d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase, amplitude, msign,
+                           adjoint);
+    else
+        hipLaunchKernelGGL(k_mode_source<uint16_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, d->mat, d->d_props,
+                           d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase, amplitude, msign,
+                           adjoint);
+    FDTD_TRY(hipGetLastError());
+    return 0;
+}
+
+int fdtd_dev_set_source_plane(fdtd_dev *d, int k, const float *plane, const unsigned char *mask) {
+    if (k < 0 || k >= d->K) {
+        set_error("source plane outside the grid");
+        return -2;
+    }
+    int err;
+    SrcPlane *sp = plane_for(d, k, &err);
+    if (!sp) return err;
+    std::vector<float> soa((size_t)6 * d->P);
+    for (long long c = 0; c < d->P; c++)
+        for (int m = 0; m < 6; m++) soa[m * d->P + c] = plane[6 * c + m];
+    float *tmp = nullptr;
+    unsigned char *dmask = nullptr;
+    FDTD_TRY(hipMalloc(&tmp, soa.size() * sizeof(float)));
+    hipError_t e = hipMemcpy(tmp, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && mask) {
+        e = hipMalloc(&dmask, (size_t)d->P);
+        if (e == hipSuccess) e = hipMemcpy(dmask, mask, (size_t)d->P, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) {
+        unsigned nb = (unsigned)((d->P + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_plane_masked_copy, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, tmp, dmask, d->P);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(tmp);
+    (void)hipFree(dmask);
+    if (e != hipSuccess) return fail("fdtd_dev_set_source_plane", e);
+    return 0;
+}
+
+int fdtd_dev_mode_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
+                          int reverse, double out[2]) {
+    if (cz < 0 || cz >= d->K) {
+        set_error("overlap plane outside the grid");
+        return -2;
+    }
+    int err;
+    const double *dm = mode_on_device(d, mode, mi, mj, mdisc, &err);
+    if (!dm) return err;
+    int nb = (int)((d->P + kBlock - 1) / kBlock);
+    if (nb > d->partial_cap) {
+        (void)hipFree(d->partials);
+        d->partials = nullptr;
+        FDTD_TRY(hipMalloc(&d->partials, (size_t)nb * sizeof(double2)));
+        d->partial_cap = nb;
+    }
+    double ratio = d->disc / mdisc;
+    hipLaunchKernelGGL(k_overlap_partial, dim3(nb), dim3(kBlock), 0, d->stream, d->F[EX], d->F[EY], d->F[HX],
+                       d->F[HY], d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, reverse, d->partials);
+    FDTD_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(kBlock), 0, d->stream, d->partials, nb,
+                       -0.5 * d->disc * d->disc, d->red_out);
+    FDTD_TRY(hipGetLastError());
+    double2 r;
+    FDTD_TRY(hipMemcpyAsync(&r, d->red_out, sizeof r, hipMemcpyDeviceToHost, d->stream));
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    out[0] = r.x;
+    out[1] = r.y;
+    return 0;
+}
+
+int fdtd_dev_dft(fdtd_dev *d, double tw_re, double tw_im, int y_slice, int k0, int k1) {
+    if (!d->fourier) {
+        FDTD_TRY(hipMalloc(&d->fourier, (size_t)6 * d->N * sizeof(double2)));
+        FDTD_TRY(hipMemsetAsync(d->fourier, 0, (size_t)6 * d->N * sizeof(double2), d->stream));
+    }
+    if (y_slice >= d->J) {
+        set_error("y slice outside the grid");
+        return -2;
+    }
+    k0 = k0 < 0 ? 0 : k0;
+    k1 = k1 > d->K ? d->K : k1;
+    if (k1 <= k0) return 0;
+    long long n = y_slice >= 0 ? (long long)d->I * (k1 - k0) : d->P * (k1 - k0);
+    unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_dft, dim3(nb), dim3(kBlock), 0, d->stream, d->fourier, d->field_base, d->Nal, d->N, d->I,
+                       d->J, d->P, k0, k1, y_slice, tw_re, tw_im);
+    FDTD_TRY(hipGetLastError());
+    return 0;
+}
+
+int fdtd_dev_taper_store(fdtd_dev *d, const void *key, int which, int seg_num, int start, int end, const int *left,
+                         const int *right, int bottom, int top, int duration, int timestep) {
+    const int L = end - start, H = top - bottom;
+    if (L <= 0 || H <= 0 || duration <= 0 || seg_num <= 0) {
+        set_error("empty taper");
+        return -2;
+    }
+    if (timestep < 0 || timestep >= duration) {
+        set_error("taper timestep outside [0, duration)");
+        return -2;
+    }
+    auto it = g_tapers.find(key);
+    if (it != g_tapers.end() && (it->second.H != H || it->second.L != L || it->second.T != duration)) {
+        fdtd_dev_taper_release(key);
+        it = g_tapers.end();
+    }
+    if (it == g_tapers.end()) {
+        TaperHist th;
+        th.key = key;
+        th.H = H;
+        th.L = L;
+        th.T = duration;
+        size_t n = (size_t)12 * H * L * duration;
+        FDTD_TRY(hipMalloc(&th.hist[0], n * sizeof(float)));
+        FDTD_TRY(hipMalloc(&th.hist[1], n * sizeof(float)));
+        FDTD_TRY(hipMemset(th.hist[0], 0, n * sizeof(float)));
+        FDTD_TRY(hipMemset(th.hist[1], 0, n * sizeof(float)));
+        FDTD_TRY(hipMalloc(&th.edge_i, (size_t)2 * L * sizeof(int)));
+        it = g_tapers.emplace(key, th).first;
+    }
+    TaperHist &th = it->second;
+    // edge pixel per row: e[n] + round(int((e[n+1]-e[n])*(k-s0)/(s1-s0))) (adjoint3d.c:276,285)
+    std::vector<int> edge((size_t)2 * L);
+    for (int n = 0; n < seg_num; n++) {
+        int s0 = start + (n * L) / seg_num, s1 = start + ((n + 1) * L) / seg_num;
+        for (int k = s0; k < s1; k++) {
+            for (int side = 0; side < 2; side++) {
+                const int *e = side == 0 ? right : left;
+                int i = (int)(e[n] + std::round((double)(((e[n + 1] - e[n]) * (k - s0)) / (s1 - s0))));
+                if (i < 0 || i >= d->I) {
+                    set_error("taper edge pixel outside the grid");
+                    return -2;
+                }
+                edge[(size_t)side * L + (k - start)] = i;
+            }
+        }
+    }
+    if (edge != th.edge_host) {
+        th.edge_host = edge;
+        FDTD_TRY(hipMemcpyAsync(th.edge_i, th.edge_host.data(), edge.size() * sizeof(int), hipMemcpyHostToDevice,
+                                d->stream));
+    }
+    if (bottom < 0 || top > d->J || start < 0 || end > d->K) {
+        set_error("taper box outside the grid");
+        return -2;
+    }
+    unsigned nb = (unsigned)((2 * H * L + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_taper_gather, dim3(nb), dim3(kBlock), 0, d->stream, th.hist[which ? 1 : 0], d->field_base,
+                       d->Nal, th.edge_i, d->I, d->J, start, L, bottom, H, timestep);
+    FDTD_TRY(hipGetLastError());
+    return 0;
+}
+
+int fdtd_dev_taper_ep_gradient(const void *key, int height, int length, int duration, double *out) {
+    auto it = g_tapers.find(key);
+    if (it == g_tapers.end()) {
+        std::memset(out, 0, sizeof(double) * 2 * height * length);
+        return 0;
+    }
+    TaperHist &th = it->second;
+    if (th.H != height || th.L != length || th.T != duration) {
+        set_error("taper dimensions changed since the fields were stored");
+        return -2;
+    }
+    double *g = nullptr;
+    FDTD_TRY(hipDeviceSynchronize());
+    FDTD_TRY(hipMalloc(&g, sizeof(double) * 2 * height * length));
+    unsigned nb = (unsigned)((2 * height * length + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_taper_grad, dim3(nb), dim3(kBlock), 0, 0, th.hist[0], th.hist[1], height, length, duration,
+                       g);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, g, sizeof(double) * 2 * height * length, hipMemcpyDeviceToHost);
+    (void)hipFree(g);
+    if (e != hipSuccess) return fail("fdtd_dev_taper_ep_gradient", e);
+    return 0;
+}
+
+void fdtd_dev_taper_release(const void *key) {
+    auto it = g_tapers.find(key);
+    if (it == g_tapers.end()) return;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(it->second.hist[0]);
+    (void)hipFree(it->second.hist[1]);
+    (void)hipFree(it->second.edge_i);
+    g_tapers.erase(it);
+}
+
+int fdtd_dev_block_store(fdtd_dev *d, const void *key, const int box[6], int duration, int timestep) {
+    const int nx = box[1] - box[0], ny = box[3] - box[2], nz = box[5] - box[4];
+    if (nx <= 0 || ny <= 0 || nz <= 0 || box[0] < 0 || box[1] > d->I || box[2] < 0 || box[3] > d->J ||
+        box[4] < 0 || box[5] > d->K) {
+        set_error("adjoint block outside the grid");
+        return -2;
+    }
+    if (timestep < 0 || timestep >= duration) {
+        set_error("block timestep outside [0, t)");
+        return -2;
+    }
+    auto it = g_blocks.find(key);
+    if (it == g_blocks.end()) {
+        BlockHist bh;
+        bh.key = key;
+        std::memcpy(bh.box, box, sizeof bh.box);
+        bh.T = duration;
+        size_t n = (size_t)nx * ny * nz;
+        FDTD_TRY(hipMalloc(&bh.fwd, 3 * n * duration * sizeof(float)));
+        FDTD_TRY(hipMemset(bh.fwd, 0, 3 * n * duration * sizeof(float)));
+        FDTD_TRY(hipMalloc(&bh.grad, n * sizeof(double)));
+        FDTD_TRY(hipMemset(bh.grad, 0, n * sizeof(double)));
+        it = g_blocks.emplace(key, bh).first;
+    }
+    long long n = (long long)nx * ny * nz;
+    unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_block_gather, dim3(nb), dim3(kBlock), 0, d->stream, it->second.fwd, d->field_base, d->Nal,
+                       d->I, d->J, box[0], nx, box[2], ny, box[4], nz, timestep);
+    FDTD_TRY(hipGetLastError());
+    return 0;
+}
+
+int fdtd_dev_block_update_gradient(fdtd_dev *d, const void *key, const int box[6], int duration, int timestep,
+                                   double *gradient_host) {
+    auto it = g_blocks.find(key);
+    const int nx = box[1] - box[0], ny = box[3] - box[2], nz = box[5] - box[4];
+    long long n = (long long)nx * ny * nz;
+    if (it == g_blocks.end()) {
+        // no stored forward fields: the reference multiplies by zeros
+        return 0;
+    }
+    if (timestep < 0 || timestep >= duration) {
+        set_error("block timestep outside [0, t)");
+        return -2;
+    }
+    unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_block_accumulate, dim3(nb), dim3(kBlock), 0, d->stream, it->second.grad, it->second.fwd,
+                       d->field_base, d->Nal, d->I, d->J, box[0], nx, box[2], ny, box[4], nz, timestep);
+    FDTD_TRY(hipGetLastError());
+    if (gradient_host) {
+        FDTD_TRY(hipMemcpyAsync(gradient_host, it->second.grad, n * sizeof(double), hipMemcpyDeviceToHost,
+                                d->stream));
+        FDTD_TRY(hipStreamSynchronize(d->stream));
+    }
+    return 0;
+}
+
+int fdtd_dev_block_reset(const void *key, int which) {
+    auto it = g_blocks.find(key);
+    if (it == g_blocks.end()) return 0;
+    const int *box = it->second.box;
+    size_t n = (size_t)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
+    FDTD_TRY(hipDeviceSynchronize());
+    if (which == 0) FDTD_TRY(hipMemset(it->second.grad, 0, n * sizeof(double)));
+    else FDTD_TRY(hipMemset(it->second.fwd, 0, 3 * n * it->second.T * sizeof(float)));
+    return 0;
+}
+
+void fdtd_dev_block_release(const void *key) {
+    auto it = g_blocks.find(key);
+    if (it == g_blocks.end()) return;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(it->second.fwd);
+    (void)hipFree(it->second.grad);
+    g_blocks.erase(it);
+}
+
+}  // extern "C"

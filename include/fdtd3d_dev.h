@@ -1,0 +1,88 @@
+/*
+ * fdtd3d_dev.h -- the thin C-ABI between the C host layer and the HIP (gfx950) kernels.
+ *
+ * Plain C types only (no HIP or torch types): pointers, sizes, doubles.  Every call enqueues on
+ * the grid's own HIP stream; calls that return data to the host synchronise that stream.  All
+ * functions return 0 on success and a negative code on failure, with a message available from
+ * fdtd_dev_last_error().  Array arguments use the reference's AoS layout (timefield3d.h:15-27):
+ * cell c = i + I*(j + J*k), six components per cell for fields/sig/sources, three for ep/mu.
+ *
+ * Each entry point replaces the compute inside one reference function; the citation names it.
+ */
+#ifndef FDTD3D_DEV_H
+#define FDTD3D_DEV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdtd_dev fdtd_dev;
+
+const char *fdtd_dev_last_error(void);
+
+/* Device state for an I x J x K grid (timefield3d.c:6-41). */
+fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc);
+void fdtd_dev_destroy(fdtd_dev *d);                                           /* timefield3d.c:43  */
+
+/* Materials from host AoS float arrays: unique (ep[3], mu[3], sig[6]) tuples become a table and
+ * each cell stores a 1-byte id (2-byte when more than 256 tuples). */
+int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const float *sig);
+
+/* Fields / sources / shells / DFT accumulators between host AoS arrays and the device. */
+int fdtd_dev_set_fields(fdtd_dev *d, const double *fields);
+int fdtd_dev_get_fields(fdtd_dev *d, double *fields);                          /* timefield3d.c:123 */
+int fdtd_dev_set_sources(fdtd_dev *d, const float *sources);
+int fdtd_dev_get_sources(fdtd_dev *d, float *sources);
+int fdtd_dev_zero(fdtd_dev *d);                                                /* timefield3d.c:108 */
+int fdtd_dev_get_fourier(fdtd_dev *d, double *fourier_interleaved);
+int fdtd_dev_set_fourier(fdtd_dev *d, const double *fourier_interleaved);
+
+/* Time stepping.  scheme 0 = semi-implicit conductivity (timefield3d.c:1679-1837),
+ * 1 = explicit (timefield3d.c:1537-1677).  [k0, k1) bounds the z range like the reference's
+ * _zsection variants; interior cells are [1, N-2] in every axis. */
+int fdtd_dev_snapshot_shells(fdtd_dev *d);                                     /* timefield3d.c:246 */
+int fdtd_dev_set_shells(fdtd_dev *d, const double *old_fields);
+int fdtd_dev_update_E(fdtd_dev *d, double dt, int k0, int k1, int scheme);    /* timefield3d.c:1759 */
+int fdtd_dev_update_H(fdtd_dev *d, double dt, int k0, int k1, int scheme);    /* timefield3d.c:1679 */
+int fdtd_dev_mur_E(fdtd_dev *d, double dt, int k0, int k1);                   /* timefield3d.c:2499 */
+int fdtd_dev_mur_H(fdtd_dev *d, double dt, int k0, int k1);                   /* timefield3d.c:2275 */
+int fdtd_dev_step(fdtd_dev *d, double dt, int scheme);                        /* timefield3d.c:2787 */
+
+/* Soft sources on plane k = cz.  mode is interleaved complex double[6*mi*mj] (already
+ * power-normalised by the caller).  adjoint selects the CW adjoint form (timefield3d.c:3028). */
+int fdtd_dev_mode_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
+                         double cos_phase, double sin_phase, double amplitude, int reverse, int adjoint);
+/* Overwrite plane k of the sources with a host AoS plane float[6*I*J]; cells where mask == 0
+ * keep their previous value (mask may be NULL). */
+int fdtd_dev_set_source_plane(fdtd_dev *d, int k, const float *plane, const unsigned char *mask);
+
+/* Instantaneous overlap on plane cz (timefield3d.c:3176-3239, without the cumulative update). */
+int fdtd_dev_mode_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
+                          int reverse, double out[2]);
+
+/* Single-frequency running DFT with a host-computed twiddle; y_slice < 0 means the volume;
+ * [k0, k1) bounds z (timefield3d.c:2851-2975). */
+int fdtd_dev_dft(fdtd_dev *d, double tw_re, double tw_im, int y_slice, int k0, int k1);
+
+/* Taper edge histories, keyed by an opaque host pointer (adjoint3d.c:259-373). */
+int fdtd_dev_taper_store(fdtd_dev *d, const void *key, int which, int seg_num, int start, int end, const int *left,
+                         const int *right, int bottom, int top, int duration, int timestep);
+int fdtd_dev_taper_ep_gradient(const void *key, int height, int length, int duration, double *out);
+void fdtd_dev_taper_release(const void *key);
+
+/* 3-D block adjoint (adjoint3d.c:48-105), keyed by an opaque host pointer. */
+int fdtd_dev_block_store(fdtd_dev *d, const void *key, const int box[6], int duration, int timestep);
+int fdtd_dev_block_update_gradient(fdtd_dev *d, const void *key, const int box[6], int duration, int timestep,
+                                   double *gradient_host);
+int fdtd_dev_block_reset(const void *key, int which);
+void fdtd_dev_block_release(const void *key);
+
+/* Timing of the work enqueued on the grid's stream (HIP events on that stream). */
+int fdtd_dev_timer_start(fdtd_dev *d);
+int fdtd_dev_timer_stop(fdtd_dev *d, double *elapsed_ms);
+int fdtd_dev_synchronize(fdtd_dev *d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
