@@ -137,6 +137,17 @@ SIGNATURES = {
     "fdtd3d_ext_synchronize": ([TimeField], _i),
     "fdtd3d_ext_steps": ([TimeField, _d, _i], _i),
     "fdtd3d_ext_last_error": ([], C.c_char_p),
+    "fdtd3d_ext_kernel_timing": ([TimeField, _i], _i),
+    "fdtd3d_ext_kernel_times": ([TimeField, _P(_i), _P(_d), _P(_d)], _i),
+    "fdtd3d_ext_set_cpml": ([TimeField, _P(_i), _i, _d, _d, _d], _i),
+    "fdtd3d_ext_set_mur": ([TimeField], _i),
+    "fdtd3d_ext_run_clear": ([TimeField], _i),
+    "fdtd3d_ext_run_add_mode_source": ([TimeField, _i, _i, _i, Mode, _i, _i, _i, _P(_d), _P(_d)], _i),
+    "fdtd3d_ext_run_add_overlap": ([TimeField, _i, _i, _i, Mode, _i, _i, _P(_d)], _i),
+    "fdtd3d_ext_run_add_dft_plane": ([TimeField, _i, _i, _i, _P(_d)], _i),
+    "fdtd3d_ext_run": ([TimeField, _d, _i, _i, _d], _i),
+    "fdtd3d_ext_run_overlap": ([TimeField, _i, _P(Complex), _P(Complex)], _i),
+    "fdtd3d_ext_run_dft": ([TimeField, _i, _P(Complex)], _i),
 }
 
 _lib = None

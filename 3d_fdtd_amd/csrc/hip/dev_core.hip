@@ -183,6 +183,8 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->use_clock = 0;
     d->field_base = nullptr;
     d->shell = nullptr;
+    d->dense_src = nullptr;
+    d->run = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     // fields: 6 arrays, each N floats, each starting on a 256-byte boundary
     d->Nal = (d->N + 63) / 64 * 64;
@@ -218,6 +220,9 @@ void fdtd_dev_destroy(fdtd_dev *d) {
     (void)hipFree(d->d_props);
     (void)hipFree(d->d_coef);
     (void)hipFree(d->shell);
+    (void)hipFree(d->dense_src);
+    fdtd_dev_cpml_off(d);
+    run_release(d);
     (void)hipFree(d->fourier);
     (void)hipFree(d->partials);
     (void)hipFree(d->red_out);
@@ -333,13 +338,20 @@ int fdtd_dev_set_sources(fdtd_dev *d, const float *sources) {
         for (long long q = 0; q < 6 * d->P && !nz; q++) nz = p[q] != 0.0f;
         if (nz) planes.push_back(k);
     }
-    if ((int)planes.size() > kMaxSrcPlanes) {
-        set_error("sources on more than 8 distinct z planes are not supported by the device path");
-        return -3;
-    }
     FDTD_TRY(hipStreamSynchronize(d->stream));
     for (auto &p : d->src) (void)hipFree(p.J);
     d->src.clear();
+    (void)hipFree(d->dense_src);
+    d->dense_src = nullptr;
+    if ((int)planes.size() > kMaxSrcPlanes) {
+        // sources spread over many planes: keep the whole 6 x N volume (extra stream per cell)
+        std::vector<float> soa((size_t)6 * d->N);
+        for (long long c = 0; c < d->N; c++)
+            for (int m = 0; m < 6; m++) soa[m * d->N + c] = sources[6 * c + m];
+        FDTD_TRY(hipMalloc(&d->dense_src, soa.size() * sizeof(float)));
+        FDTD_TRY(hipMemcpy(d->dense_src, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
+        return 0;
+    }
     std::vector<float> soa((size_t)6 * d->P);
     for (int k : planes) {
         const float *p = sources + 6 * d->P * k;
@@ -347,6 +359,7 @@ int fdtd_dev_set_sources(fdtd_dev *d, const float *sources) {
             for (int m = 0; m < 6; m++) soa[m * d->P + c] = p[6 * c + m];
         SrcPlane sp;
         sp.k = k;
+        sp.stride = d->P;
         FDTD_TRY(hipMalloc(&sp.J, (size_t)6 * d->P * sizeof(float)));
         FDTD_TRY(hipMemcpy(sp.J, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
         d->src.push_back(sp);
@@ -356,8 +369,15 @@ int fdtd_dev_set_sources(fdtd_dev *d, const float *sources) {
 
 int fdtd_dev_get_sources(fdtd_dev *d, float *sources) {
     std::memset(sources, 0, (size_t)6 * d->N * sizeof(float));
-    std::vector<float> soa((size_t)6 * d->P);
     FDTD_TRY(hipStreamSynchronize(d->stream));
+    if (d->dense_src) {
+        std::vector<float> soa((size_t)6 * d->N);
+        FDTD_TRY(hipMemcpy(soa.data(), d->dense_src, soa.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (long long c = 0; c < d->N; c++)
+            for (int m = 0; m < 6; m++) sources[6 * c + m] = soa[m * d->N + c];
+        return 0;
+    }
+    std::vector<float> soa((size_t)6 * d->P);
     for (auto &sp : d->src) {
         FDTD_TRY(hipMemcpy(soa.data(), sp.J, soa.size() * sizeof(float), hipMemcpyDeviceToHost));
         float *p = sources + 6 * d->P * sp.k;
@@ -372,6 +392,8 @@ int fdtd_dev_zero(fdtd_dev *d) {
     long long shell_n = d->shell_off[5] + 8LL * d->I * d->J;
     FDTD_TRY(hipMemsetAsync(d->shell, 0, (size_t)shell_n * sizeof(float), d->stream));
     for (auto &p : d->src) FDTD_TRY(hipMemsetAsync(p.J, 0, (size_t)6 * d->P * sizeof(float), d->stream));
+    if (d->dense_src) FDTD_TRY(hipMemsetAsync(d->dense_src, 0, (size_t)6 * d->N * sizeof(float), d->stream));
+    if (int rc = cpml_zero(d)) return rc;
     if (d->fourier) FDTD_TRY(hipMemsetAsync(d->fourier, 0, (size_t)6 * d->N * sizeof(double2), d->stream));
     return 0;
 }

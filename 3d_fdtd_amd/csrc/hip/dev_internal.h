@@ -34,7 +34,8 @@ struct SrcPlaneArg {
 
 struct SrcPlane {
     int k;
-    float *J;  // device, 6 * P floats
+    float *J;          // device; component m of cell p at J[m * stride + p]
+    long long stride;  // P for a stand-alone plane, N inside the dense source volume
 };
 
 struct ModeCache {
@@ -52,6 +53,29 @@ struct TaperHist {
     float *hist[2];              // forward, adjoint: [T][L][H][2][6]
     int *edge_i;                 // [2][L] edge pixel per z row (right, left)
     std::vector<int> edge_host;  // cached copy of edge_i
+};
+
+// Convolutional PML (an extension; the reference has only Mur faces).  Along axis a the slab
+// index of coordinate n is n for n < nlo and nlo + n - (N-1-nhi) for n >= N-1-nhi (W = nlo +
+// nhi + 1 entries, or nlo when nhi == 0); psi arrays replace coordinate a by that index.
+struct Cpml {
+    bool on = false;
+    int nlo[3] = {0, 0, 0}, nhi[3] = {0, 0, 0}, W[3] = {0, 0, 0};
+    int order = 3;
+    double R0 = 1e-8, kappa_max = 1.0, alpha_max = 0.0;
+    float *tab = nullptr;  // per axis: bE, aE, kE, bH, aH, kH, each N_axis floats
+    long long tab_off[3] = {0, 0, 0};
+    double tab_dt = -1.0;
+    float *psi_base = nullptr;
+    float *psi[2][3][2] = {};  // [E/H][slab axis][q]: component (axis + 1 + q) % 3
+    long long psi_n[3] = {0, 0, 0};
+};
+
+// Sampled per-kernel timing of the E / H updates (HIP events on the grid's stream).
+struct KernelTiming {
+    int every = 0;
+    long long launches[2] = {0, 0};
+    std::vector<hipEvent_t> ev[2];  // start/stop pairs, E and H
 };
 
 struct BlockHist {
@@ -89,8 +113,10 @@ struct fdtd_dev {
     float *shell;
     long long shell_off[6];
 
-    // sources
+    // sources: up to kMaxSrcPlanes z planes; beyond that a dense 6 x N volume (dense_src)
     std::vector<fdtd::SrcPlane> src;
+    float *dense_src;
+    fdtd::SrcPlane dense_view;
 
     // reference single-frequency DFT accumulators (6 SoA arrays of N double2), lazily allocated
     double2 *fourier;
@@ -102,6 +128,10 @@ struct fdtd_dev {
 
     std::vector<fdtd::ModeCache> modes;
     unsigned long long use_clock;
+
+    fdtd::Cpml cpml;
+    void *run;  // device-resident run configuration (dev_run.hip), or null
+    fdtd::KernelTiming ktime;
 
     hipEvent_t ev0, ev1;
 };
@@ -117,6 +147,10 @@ int fail(const char *where, hipError_t e);
     } while (0)
 
 int ensure_coefficients(fdtd_dev *d, double dt);
+int cpml_zero(fdtd_dev *d);
+// The device source plane for z = k (created zeroed on first use), or null with *err set.
+SrcPlane *plane_for(fdtd_dev *d, int k, int *err);
+void run_release(fdtd_dev *d);
 const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, double mdisc, int *err);
 
 // Blocks over (plane chunks x z chunks) remapped so that each XCD walks a contiguous range

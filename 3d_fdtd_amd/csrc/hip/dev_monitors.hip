@@ -30,7 +30,8 @@ __device__ __forceinline__ int mode_index_1d(int n, int center, double ratio, in
 // adjoint CW source: J = (Re Hy, -Re Hx, 0), M = (-(2r-1) Re Ey, (2r-1) Re Ex, 0).
 // Cells outside the mode window keep their previous value.
 template <typename MatT>
-__global__ void k_mode_source(float *plane, const void *mat, const float *props, int I, int J, long long P, int cz,
+__global__ void k_mode_source(float *plane, long long cs, const void *mat, const float *props, int I, int J,
+                              long long P, int cz,
                               int cx, int cy, const double *mode, int mi, int mj, double ratio, double cp, double sp,
                               double amp, int msign, int adjoint) {
 #pragma clang fp contract(off)
@@ -47,27 +48,28 @@ __global__ void k_mode_source(float *plane, const void *mat, const float *props,
             double v;
             if (m < 3) v = amp * (double)pr[m] * proj;
             else v = (double)(-msign) * amp * (double)pr[3 + m % 3] * proj;
-            plane[m * P + p] = (float)v;
+            plane[m * cs + p] = (float)v;
         }
     } else {
         double rEx = md[0] * cp + md[1] * sp;
         double rEy = md[2] * cp + md[3] * sp;
         double rHx = md[6] * cp + md[7] * sp;
         double rHy = md[8] * cp + md[9] * sp;
-        plane[0 * P + p] = (float)rHy;
-        plane[1 * P + p] = (float)(-rHx);
-        plane[2 * P + p] = 0.0f;
-        plane[3 * P + p] = (float)((double)(-1 * msign) * rEy);
-        plane[4 * P + p] = (float)((double)msign * rEx);
-        plane[5 * P + p] = 0.0f;
+        plane[0 * cs + p] = (float)rHy;
+        plane[1 * cs + p] = (float)(-rHx);
+        plane[2 * cs + p] = 0.0f;
+        plane[3 * cs + p] = (float)((double)(-1 * msign) * rEy);
+        plane[4 * cs + p] = (float)((double)msign * rEx);
+        plane[5 * cs + p] = 0.0f;
     }
 }
 
-__global__ void k_plane_masked_copy(float *plane, const float *src, const unsigned char *mask, long long P) {
+__global__ void k_plane_masked_copy(float *plane, long long cs, const float *src, const unsigned char *mask,
+                                    long long P) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     if (mask && !mask[p]) return;
-    for (int m = 0; m < 6; m++) plane[m * P + p] = src[m * P + p];
+    for (int m = 0; m < 6; m++) plane[m * cs + p] = src[m * P + p];
 }
 
 // Overlap stage 1: per-block FP64 partial sums of Hx mEy - Hy mEx + Ey mHx - Ex mHy.
@@ -214,8 +216,16 @@ __global__ void k_block_accumulate(double *grad, const float *fwd, const float *
     grad[q] += s;
 }
 
-fdtd::SrcPlane *plane_for(fdtd_dev *d, int k, int *err) {
+}  // namespace
+
+SrcPlane *plane_for(fdtd_dev *d, int k, int *err) {
     *err = 0;
+    if (d->dense_src) {
+        d->dense_view.k = k;
+        d->dense_view.J = d->dense_src + (long long)k * d->P;
+        d->dense_view.stride = d->N;
+        return &d->dense_view;
+    }
     for (auto &sp : d->src)
         if (sp.k == k) return &sp;
     if ((int)d->src.size() >= kMaxSrcPlanes) {
@@ -225,6 +235,7 @@ fdtd::SrcPlane *plane_for(fdtd_dev *d, int k, int *err) {
     }
     SrcPlane sp;
     sp.k = k;
+    sp.stride = d->P;
     hipError_t e = hipMalloc(&sp.J, (size_t)6 * d->P * sizeof(float));
     if (e == hipSuccess) e = hipMemsetAsync(sp.J, 0, (size_t)6 * d->P * sizeof(float), d->stream);
     if (e != hipSuccess) {
@@ -235,7 +246,6 @@ fdtd::SrcPlane *plane_for(fdtd_dev *d, int k, int *err) {
     return &d->src.back();
 }
 
-}  // namespace
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -261,13 +271,13 @@ int fdtd_dev_mode_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode
     int msign = -1 + 2 * reverse;
     double ratio = d->disc / mdisc;
     if (d->mat_bytes == 1)
-        hipLaunchKernelGGL(k_mode_source<uint8_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, d->mat, d->d_props,
-                           d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase, amplitude, msign,
-                           adjoint);
+        hipLaunchKernelGGL(k_mode_source<uint8_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, sp->stride, d->mat,
+                           d->d_props, d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase,
+                           amplitude, msign, adjoint);
     else
-        hipLaunchKernelGGL(k_mode_source<uint16_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, d->mat, d->d_props,
-                           d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase, amplitude, msign,
-                           adjoint);
+        hipLaunchKernelGGL(k_mode_source<uint16_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, sp->stride, d->mat,
+                           d->d_props, d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase,
+                           amplitude, msign, adjoint);
     FDTD_TRY(hipGetLastError());
     return 0;
 }
@@ -293,7 +303,8 @@ int fdtd_dev_set_source_plane(fdtd_dev *d, int k, const float *plane, const unsi
     }
     if (e == hipSuccess) {
         unsigned nb = (unsigned)((d->P + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_plane_masked_copy, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, tmp, dmask, d->P);
+        hipLaunchKernelGGL(k_plane_masked_copy, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, sp->stride, tmp, dmask,
+                           d->P);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);

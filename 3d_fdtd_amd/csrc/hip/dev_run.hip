@@ -1,0 +1,420 @@
+// Device-resident runs: tabulated mode sources, FP64 overlap monitors with a multi-frequency
+// running DFT, FP64 multi-frequency DFT planes.  Extension of the reference's per-call API
+// (timefield3d.c:2977-3239): the same source and overlap formulas, but the per-step amplitude,
+// phase and twiddles come from tables, so a run advances many steps without host round trips.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dev_internal.h"
+
+namespace fdtd {
+namespace {
+
+struct RunSource {
+    int k;
+    float *PR, *PI;        // 6 x P profile planes: J_m(n) = c1(n) PR_m + c2(n) PI_m
+    unsigned char *win;    // P, 1 inside the mode window
+    std::vector<float2> coef;  // nsteps x (A cos(phase), A sin(phase)), passed as kernel arguments
+    int nsteps;
+};
+
+struct RunOverlap {
+    int k;
+    float *W;              // 8 x P: Re/Im of mEx, mEy, mHx, mHy (reverse applied), 0 outside window
+    int nfreq;
+    std::vector<double> omega;
+    double2 *acc;          // nfreq cumulative overlaps
+    double2 *inst;         // last instantaneous overlap
+    int freq_off;          // offset into the run's twiddle table
+};
+
+struct RunDft {
+    int k, mask, ncomp, nfreq;
+    std::vector<double> omega;
+    double2 *acc;          // ncomp x nfreq x P
+    int freq_off;
+};
+
+struct Run {
+    std::vector<RunSource> src;
+    std::vector<RunOverlap> ovl;
+    std::vector<RunDft> dft;
+    double2 *partials = nullptr;  // overlap stage-1 partial sums
+    int partial_cap = 0;
+    double2 *tw = nullptr;        // [step][total freq] twiddles for the current run() call
+    long long tw_cap = 0;
+};
+
+Run *run_of(fdtd_dev *d) {
+    if (!d->run) d->run = new Run();
+    return static_cast<Run *>(d->run);
+}
+
+__global__ void k_run_source(float *J, long long cs, const float *PR, const float *PI, const unsigned char *win,
+                             long long P, float2 c) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !win[p]) return;
+#pragma unroll
+    for (int m = 0; m < 6; m++) J[m * cs + p] = c.x * PR[m * P + p] + c.y * PI[m * P + p];
+}
+
+// Stage 1: per-block FP64 partial of Hx mEy - Hy mEx + Ey mHx - Ex mHy over plane k.
+__global__ __launch_bounds__(kBlock) void k_run_overlap_partial(const float *Ex, const float *Ey, const float *Hx,
+                                                                const float *Hy, long long P, long long koff,
+                                                                const float *W, double2 *partials) {
+    __shared__ double sre[kBlock / 64], sim[kBlock / 64];
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double re = 0.0, im = 0.0;
+    if (p < P) {
+        const long long c = p + koff;
+        const double ex = Ex[c], ey = Ey[c], hx = Hx[c], hy = Hy[c];
+        const double eEx_r = W[0 * P + p], eEx_i = W[1 * P + p], eEy_r = W[2 * P + p], eEy_i = W[3 * P + p];
+        const double hHx_r = W[4 * P + p], hHx_i = W[5 * P + p], hHy_r = W[6 * P + p], hHy_i = W[7 * P + p];
+        re = (hx * eEy_r - hy * eEx_r) + (ey * hHx_r - ex * hHy_r);
+        im = (hx * eEy_i - hy * eEx_i) + (ey * hHx_i - ex * hHy_i);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_down(re, off, 64);
+        im += __shfl_down(im, off, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sre[w] = re;
+        sim[w] = im;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0, s = 0;
+        for (int q = 0; q < kBlock / 64; q++) {
+            r += sre[q];
+            s += sim[q];
+        }
+        partials[blockIdx.x] = make_double2(r, s);
+    }
+}
+
+// Stage 2: fixed-order sum of the partials, scale by -dx^2/2, then acc_f += inst * tw_f.
+__global__ __launch_bounds__(kBlock) void k_run_overlap_final(const double2 *partials, int nb, double scale,
+                                                              double2 *inst, double2 *acc, const double2 *tw,
+                                                              int nfreq) {
+    __shared__ double sre[kBlock], sim[kBlock];
+    double r = 0, s = 0;
+    for (int q = threadIdx.x; q < nb; q += kBlock) {
+        r += partials[q].x;
+        s += partials[q].y;
+    }
+    sre[threadIdx.x] = r;
+    sim[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sre[threadIdx.x] += sre[threadIdx.x + w];
+            sim[threadIdx.x] += sim[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    const double vr = scale * sre[0], vi = scale * sim[0];
+    if (threadIdx.x == 0) *inst = make_double2(vr, vi);
+    for (int f = threadIdx.x; f < nfreq; f += kBlock) {
+        double2 t = tw[f], a = acc[f];
+        a.x += t.x * vr - t.y * vi;
+        a.y += t.x * vi + t.y * vr;
+        acc[f] = a;
+    }
+}
+
+__global__ void k_run_dft_plane(double2 *acc, const float *base, long long Nal, long long P, long long koff, int mask,
+                                int nfreq, const double2 *tw) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    int slot = 0;
+    for (int m = 0; m < 6; m++) {
+        if (!(mask >> m & 1)) continue;
+        const double v = base[m * Nal + koff + p];
+        for (int f = 0; f < nfreq; f++) {
+            double2 *a = acc + ((long long)slot * nfreq + f) * P + p;
+            double2 t = tw[f], x = *a;
+            x.x += v * t.x;
+            x.y += v * t.y;
+            *a = x;
+        }
+        slot++;
+    }
+}
+
+int mode_window(fdtd_dev *d, int n, int c, double ratio, int mn) {
+    double dist = (n - c) * ratio;
+    return (int)dist + mn / 2;
+}
+
+}  // namespace
+
+void run_release(fdtd_dev *d) {
+    if (!d->run) return;
+    Run *r = static_cast<Run *>(d->run);
+    (void)hipStreamSynchronize(d->stream);
+    for (auto &s : r->src) {
+        (void)hipFree(s.PR);
+        (void)hipFree(s.PI);
+        (void)hipFree(s.win);
+    }
+    for (auto &o : r->ovl) {
+        (void)hipFree(o.W);
+        (void)hipFree(o.acc);
+        (void)hipFree(o.inst);
+    }
+    for (auto &q : r->dft) (void)hipFree(q.acc);
+    (void)hipFree(r->partials);
+    (void)hipFree(r->tw);
+    delete r;
+    d->run = nullptr;
+}
+
+}  // namespace fdtd
+
+using namespace fdtd;
+
+extern "C" {
+
+int fdtd_dev_run_clear(fdtd_dev *d) {
+    run_release(d);
+    return 0;
+}
+
+int fdtd_dev_run_add_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
+                            int reverse, int adjoint, int nsteps, const double *amplitude, const double *phase) {
+    if (cz < 0 || cz >= d->K || nsteps <= 0) {
+        set_error("run source: plane outside the grid or empty schedule");
+        return -2;
+    }
+    if (!d->have_mats) {
+        set_error("materials were never uploaded");
+        return -1;
+    }
+    Run *r = run_of(d);
+    const long long P = d->P;
+    const double ratio = d->disc / mdisc;
+    const int msign = -1 + 2 * reverse;
+    // ep/mu of the plane cells from the material table
+    std::vector<uint16_t> ids((size_t)P);
+    {
+        std::vector<unsigned char> raw((size_t)P * d->mat_bytes);
+        FDTD_TRY(hipStreamSynchronize(d->stream));
+        FDTD_TRY(hipMemcpy(raw.data(), (const char *)d->mat + (size_t)cz * P * d->mat_bytes, raw.size(),
+                           hipMemcpyDeviceToHost));
+        for (long long p = 0; p < P; p++)
+            ids[p] = d->mat_bytes == 1 ? raw[p] : reinterpret_cast<const uint16_t *>(raw.data())[p];
+    }
+    std::vector<float> PR((size_t)6 * P, 0.0f), PI((size_t)6 * P, 0.0f);
+    std::vector<unsigned char> win((size_t)P, 0);
+    for (int j = 0; j < d->J; j++)
+        for (int i = 0; i < d->I; i++) {
+            int mx = mode_window(d, i, cx, ratio, mi), my = mode_window(d, j, cy, ratio, mj);
+            if (mx < 0 || mx >= mi || my < 0 || my >= mj) continue;
+            long long p = i + (long long)d->I * j;
+            win[p] = 1;
+            const double *md = mode + 12 * (mx + (long long)mi * my);
+            const float *pr = &d->props[(size_t)ids[p] * kPropStride];
+            if (!adjoint) {
+                for (int m = 0; m < 6; m++) {
+                    double f = m < 3 ? (double)pr[m] : (double)(-msign) * (double)pr[3 + m % 3];
+                    PR[m * P + p] = (float)(f * md[2 * m]);
+                    PI[m * P + p] = (float)(f * md[2 * m + 1]);
+                }
+            } else {
+                const double fr[6][2] = {{md[8], md[9]},
+                                         {-md[6], -md[7]},
+                                         {0, 0},
+                                         {(double)(-msign) * md[2], (double)(-msign) * md[3]},
+                                         {(double)msign * md[0], (double)msign * md[1]},
+                                         {0, 0}};
+                for (int m = 0; m < 6; m++) {
+                    PR[m * P + p] = (float)fr[m][0];
+                    PI[m * P + p] = (float)fr[m][1];
+                }
+            }
+        }
+    std::vector<float2> coef((size_t)nsteps);
+    for (int n = 0; n < nsteps; n++) {
+        double A = adjoint ? 1.0 : amplitude[n];
+        coef[n] = make_float2((float)(A * std::cos(phase[n])), (float)(A * std::sin(phase[n])));
+    }
+    int err;
+    if (!plane_for(d, cz, &err)) return err;
+    RunSource s;
+    s.k = cz;
+    s.nsteps = nsteps;
+    s.coef = std::move(coef);
+    FDTD_TRY(hipMalloc(&s.PR, PR.size() * sizeof(float)));
+    FDTD_TRY(hipMalloc(&s.PI, PI.size() * sizeof(float)));
+    FDTD_TRY(hipMalloc(&s.win, win.size()));
+    FDTD_TRY(hipMemcpy(s.PR, PR.data(), PR.size() * sizeof(float), hipMemcpyHostToDevice));
+    FDTD_TRY(hipMemcpy(s.PI, PI.data(), PI.size() * sizeof(float), hipMemcpyHostToDevice));
+    FDTD_TRY(hipMemcpy(s.win, win.data(), win.size(), hipMemcpyHostToDevice));
+    r->src.push_back(std::move(s));
+    return (int)r->src.size() - 1;
+}
+
+int fdtd_dev_run_add_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
+                             int reverse, int nfreq, const double *omega) {
+    if (cz < 0 || cz >= d->K || nfreq < 0) {
+        set_error("run overlap: plane outside the grid");
+        return -2;
+    }
+    Run *r = run_of(d);
+    const long long P = d->P;
+    const double ratio = d->disc / mdisc;
+    std::vector<float> W((size_t)8 * P, 0.0f);
+    for (int j = 0; j < d->J; j++)
+        for (int i = 0; i < d->I; i++) {
+            int mx = mode_window(d, i, cx, ratio, mi), my = mode_window(d, j, cy, ratio, mj);
+            if (mx < 0 || mx >= mi || my < 0 || my >= mj) continue;
+            long long p = i + (long long)d->I * j;
+            const double *md = mode + 12 * (mx + (long long)mi * my);
+            double v[8] = {md[0], md[1], md[2], md[3], md[6], md[7], md[8], md[9]};
+            if (reverse == 1) {  // mH -> -conj(mH)
+                v[4] = -v[4];
+                v[6] = -v[6];
+            }
+            for (int q = 0; q < 8; q++) W[q * P + p] = (float)v[q];
+        }
+    RunOverlap o;
+    o.k = cz;
+    o.nfreq = nfreq;
+    o.omega.assign(omega, omega + nfreq);
+    o.freq_off = 0;
+    FDTD_TRY(hipMalloc(&o.W, W.size() * sizeof(float)));
+    FDTD_TRY(hipMemcpy(o.W, W.data(), W.size() * sizeof(float), hipMemcpyHostToDevice));
+    FDTD_TRY(hipMalloc(&o.acc, (size_t)(nfreq > 0 ? nfreq : 1) * sizeof(double2)));
+    FDTD_TRY(hipMemset(o.acc, 0, (size_t)(nfreq > 0 ? nfreq : 1) * sizeof(double2)));
+    FDTD_TRY(hipMalloc(&o.inst, sizeof(double2)));
+    FDTD_TRY(hipMemset(o.inst, 0, sizeof(double2)));
+    r->ovl.push_back(std::move(o));
+    return (int)r->ovl.size() - 1;
+}
+
+int fdtd_dev_run_add_dft_plane(fdtd_dev *d, int k, int comp_mask, int nfreq, const double *omega) {
+    if (k < 0 || k >= d->K || nfreq <= 0 || (comp_mask & 63) == 0) {
+        set_error("run DFT plane: bad plane, component mask or frequency list");
+        return -2;
+    }
+    Run *r = run_of(d);
+    RunDft q;
+    q.k = k;
+    q.mask = comp_mask & 63;
+    q.ncomp = __builtin_popcount(q.mask);
+    q.nfreq = nfreq;
+    q.omega.assign(omega, omega + nfreq);
+    q.freq_off = 0;
+    size_t n = (size_t)q.ncomp * nfreq * d->P;
+    FDTD_TRY(hipMalloc(&q.acc, n * sizeof(double2)));
+    FDTD_TRY(hipMemset(q.acc, 0, n * sizeof(double2)));
+    r->dft.push_back(std::move(q));
+    return (int)r->dft.size() - 1;
+}
+
+int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
+    if (n <= 0) return 0;
+    Run *r = run_of(d);
+    // twiddle table for steps n0..n0+n-1 over every monitor frequency: e^{i omega (step + t_offset) dt}
+    int nf = 0;
+    for (auto &o : r->ovl) {
+        o.freq_off = nf;
+        nf += o.nfreq;
+    }
+    for (auto &q : r->dft) {
+        q.freq_off = nf;
+        nf += q.nfreq;
+    }
+    if (nf > 0) {
+        std::vector<double2> tw((size_t)n * nf);
+        for (int s = 0; s < n; s++) {
+            double t = ((double)(n0 + s) + t_offset) * dt;
+            int f = 0;
+            for (auto &o : r->ovl)
+                for (double w : o.omega) tw[(size_t)s * nf + f++] = make_double2(std::cos(w * t), std::sin(w * t));
+            for (auto &q : r->dft)
+                for (double w : q.omega) tw[(size_t)s * nf + f++] = make_double2(std::cos(w * t), std::sin(w * t));
+        }
+        if ((long long)tw.size() > r->tw_cap) {
+            FDTD_TRY(hipStreamSynchronize(d->stream));
+            (void)hipFree(r->tw);
+            r->tw = nullptr;
+            FDTD_TRY(hipMalloc(&r->tw, tw.size() * sizeof(double2)));
+            r->tw_cap = (long long)tw.size();
+        }
+        FDTD_TRY(hipMemcpyAsync(r->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, d->stream));
+        FDTD_TRY(hipStreamSynchronize(d->stream));
+    }
+    const unsigned nbp = (unsigned)((d->P + kBlock - 1) / kBlock);
+    if (!r->ovl.empty() && (int)nbp > r->partial_cap) {
+        (void)hipFree(r->partials);
+        r->partials = nullptr;
+        FDTD_TRY(hipMalloc(&r->partials, (size_t)nbp * sizeof(double2)));
+        r->partial_cap = (int)nbp;
+    }
+    // source planes resolved once per call
+    std::vector<SrcPlane *> planes;
+    for (auto &s : r->src) {
+        int err;
+        SrcPlane *sp = plane_for(d, s.k, &err);
+        if (!sp) return err;
+        planes.push_back(sp);
+    }
+    const double scale = -0.5 * d->disc * d->disc;
+    for (int s = 0; s < n; s++) {
+        const int step = n0 + s;
+        for (size_t q = 0; q < r->src.size(); q++) {
+            const RunSource &rs = r->src[q];
+            // outside its schedule a source is switched off (amplitude 0)
+            const float2 c = step >= 0 && step < rs.nsteps ? rs.coef[step] : make_float2(0.0f, 0.0f);
+            hipLaunchKernelGGL(k_run_source, dim3(nbp), dim3(kBlock), 0, d->stream, planes[q]->J, planes[q]->stride,
+                               rs.PR, rs.PI, rs.win, d->P, c);
+            FDTD_TRY(hipGetLastError());
+        }
+        int rc = fdtd_dev_step(d, dt, 0);
+        if (rc) return rc;
+        for (auto &o : r->ovl) {
+            hipLaunchKernelGGL(k_run_overlap_partial, dim3(nbp), dim3(kBlock), 0, d->stream, d->F[EX], d->F[EY],
+                               d->F[HX], d->F[HY], d->P, (long long)o.k * d->P, o.W, r->partials);
+            hipLaunchKernelGGL(k_run_overlap_final, dim3(1), dim3(kBlock), 0, d->stream, r->partials, (int)nbp, scale,
+                               o.inst, o.acc, r->tw + (size_t)s * nf + o.freq_off, o.nfreq);
+            FDTD_TRY(hipGetLastError());
+        }
+        for (auto &q : r->dft) {
+            hipLaunchKernelGGL(k_run_dft_plane, dim3(nbp), dim3(kBlock), 0, d->stream, q.acc, d->field_base, d->Nal,
+                               d->P, (long long)q.k * d->P, q.mask, q.nfreq, r->tw + (size_t)s * nf + q.freq_off);
+            FDTD_TRY(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+int fdtd_dev_run_overlap_result(fdtd_dev *d, int id, double *cumulative, double *last_inst) {
+    Run *r = run_of(d);
+    if (id < 0 || id >= (int)r->ovl.size()) {
+        set_error("no such overlap monitor");
+        return -2;
+    }
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    const RunOverlap &o = r->ovl[id];
+    if (cumulative && o.nfreq > 0)
+        FDTD_TRY(hipMemcpy(cumulative, o.acc, (size_t)o.nfreq * sizeof(double2), hipMemcpyDeviceToHost));
+    if (last_inst) FDTD_TRY(hipMemcpy(last_inst, o.inst, sizeof(double2), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fdtd_dev_run_dft_result(fdtd_dev *d, int id, double *out) {
+    Run *r = run_of(d);
+    if (id < 0 || id >= (int)r->dft.size()) {
+        set_error("no such DFT plane");
+        return -2;
+    }
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    const RunDft &q = r->dft[id];
+    FDTD_TRY(hipMemcpy(out, q.acc, (size_t)q.ncomp * q.nfreq * d->P * sizeof(double2), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

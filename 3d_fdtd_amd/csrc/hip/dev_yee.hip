@@ -1,6 +1,8 @@
-// Yee leapfrog on gfx950: split E / H z-marching kernels, boundary-shell snapshot and the
-// first-order Mur faces.  Reference semantics: timefield3d.c:246-346 (shell snapshot),
-// 1537-1837 (E/H updates), 2275-2721 (Mur faces), 2723-2849 (step drivers).
+// Yee leapfrog on gfx950: split E / H z-marching kernels (optionally with the convolutional PML),
+// boundary-shell snapshot and the first-order Mur faces.  Reference semantics:
+// timefield3d.c:246-346 (shell snapshot), 1537-1837 (E/H updates), 2275-2721 (Mur faces),
+// 2723-2849 (step drivers).  The PML is an extension with no reference counterpart; its
+// formulas are the ones stated in oracle/fdtd_oracle.c (Roden-Gedney CPML with CFS grading).
 //
 // Layout: six FP32 component arrays, cell c = i + I*(j + J*k).  A thread owns one (i, j) column of
 // a z chunk and marches k upward, keeping the k-1 (E update) or k+1 (H update) neighbours it
@@ -24,6 +26,15 @@ struct YeeArgs {
     int plane_blocks;    // blocks per z plane
     int nsrc;
     SrcPlaneArg src[kMaxSrcPlanes];
+    const float *dense;  // 6 x N source volume when sources span more than kMaxSrcPlanes planes
+    long long N;
+};
+
+struct PmlArgs {
+    const float *tab[3];  // per axis: bE, aE, kE, bH, aH, kH (N_axis floats each)
+    int nlo[3], nhi[3], W[3];
+    float *psiE[3][2];
+    float *psiH[3][2];
 };
 
 template <typename MatT>
@@ -31,16 +42,35 @@ __device__ __forceinline__ int mat_id(const void *mat, long long c) {
     return (int)static_cast<const MatT *>(mat)[c];
 }
 
-__device__ __forceinline__ const float *src_plane(const YeeArgs &a, int k) {
+// Source values of plane k (component m of plane cell p at J[m*stride + p]), or null.
+__device__ __forceinline__ const float *src_plane(const YeeArgs &a, int k, long long &stride) {
+    if (a.dense) {
+        stride = a.N;
+        return a.dense + (long long)k * a.P;
+    }
     const float *J = nullptr;
+    stride = a.P;
     for (int s = 0; s < a.nsrc; s++)
         if (a.src[s].k == k) J = a.src[s].J;
     return J;
 }
 
+__device__ __forceinline__ int slab_index(int n, int N, int nlo, int nhi) {
+    if (n < nlo) return n;
+    if (nhi > 0 && n >= N - 1 - nhi) return nlo + n - (N - 1 - nhi);
+    return -1;
+}
+
+// psi <- b psi + a d; returns the PML-corrected difference d/kappa + psi
+__device__ __forceinline__ float pml_term(float *psi, float b, float a, float kinv, float d) {
+    float v = b * *psi + a * d;
+    *psi = v;
+    return d * kinv + v;
+}
+
 // E <- caE*E + cbE*(backward-difference curl of H) - cjE*J   (timefield3d.c:1759-1837)
-template <typename MatT>
-__global__ __launch_bounds__(kBlock) void k_update_E(YeeArgs a) {
+template <typename MatT, bool PML>
+__global__ __launch_bounds__(kBlock) void k_update_E(YeeArgs a, PmlArgs pm) {
     const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int pb = lb % a.plane_blocks, kc = lb / a.plane_blocks;
     const long long p = (long long)pb * kBlock + threadIdx.x;
@@ -50,6 +80,14 @@ __global__ __launch_bounds__(kBlock) void k_update_E(YeeArgs a) {
     const int kb = a.k0 + kc * a.kchunk;
     const int ke = min(a.k1, kb + a.kchunk);
     if (kb >= ke) return;
+    int sx = -1, sy = -1;
+    float bx = 0, ax = 0, kx = 1, by = 0, ay = 0, ky = 1;
+    if (PML) {
+        sx = slab_index(i, a.I, pm.nlo[0], pm.nhi[0]);
+        sy = slab_index(j, a.J, pm.nlo[1], pm.nhi[1]);
+        bx = pm.tab[0][i], ax = pm.tab[0][a.I + i], kx = pm.tab[0][2 * a.I + i];
+        by = pm.tab[1][j], ay = pm.tab[1][a.J + j], ky = pm.tab[1][2 * a.J + j];
+    }
     long long c = p + (long long)kb * a.P;
     float hx_km = a.Hx[c - a.P], hy_km = a.Hy[c - a.P];
     for (int k = kb; k < ke; k++, c += a.P) {
@@ -57,17 +95,41 @@ __global__ __launch_bounds__(kBlock) void k_update_E(YeeArgs a) {
         const float hy_im = a.Hy[c - 1], hz_im = a.Hz[c - 1];
         const float hx_jm = a.Hx[c - a.I], hz_jm = a.Hz[c - a.I];
         const float *cf = a.coef + (long long)mat_id<MatT>(a.mat, c) * kCoefStride;
-        const float dx = (hz - hz_jm) - (hy - hy_km);
-        const float dy = (hx - hx_km) - (hz - hz_im);
-        const float dz = (hy - hy_im) - (hx - hx_jm);
-        float ex = cf[0] * a.Ex[c] + cf[3] * dx;
-        float ey = cf[1] * a.Ey[c] + cf[4] * dy;
-        float ez = cf[2] * a.Ez[c] + cf[5] * dz;
-        const float *J = src_plane(a, k);
+        // curl_x = dHz/dy - dHy/dz, curl_y = dHx/dz - dHz/dx, curl_z = dHy/dx - dHx/dy
+        float x1 = hz - hz_jm, x2 = hy - hy_km;
+        float y1 = hx - hx_km, y2 = hz - hz_im;
+        float z1 = hy - hy_im, z2 = hx - hx_jm;
+        if (PML) {
+            const int sz = slab_index(k, a.K, pm.nlo[2], pm.nhi[2]);
+            const float kz = pm.tab[2][2 * a.K + k];
+            float tx1 = x1 * ky, tx2 = x2 * kz, ty1 = y1 * kz, ty2 = y2 * kx, tz1 = z1 * kx, tz2 = z2 * ky;
+            if (sx >= 0) {
+                const long long q = sx + (long long)pm.W[0] * (j + (long long)a.J * k);
+                ty2 = pml_term(pm.psiE[0][0] + q, bx, ax, kx, y2);
+                tz1 = pml_term(pm.psiE[0][1] + q, bx, ax, kx, z1);
+            }
+            if (sy >= 0) {
+                const long long q = i + (long long)a.I * (sy + (long long)pm.W[1] * k);
+                tz2 = pml_term(pm.psiE[1][0] + q, by, ay, ky, z2);
+                tx1 = pml_term(pm.psiE[1][1] + q, by, ay, ky, x1);
+            }
+            if (sz >= 0) {
+                const float bz = pm.tab[2][k], az = pm.tab[2][a.K + k];
+                const long long q = p + a.P * sz;
+                tx2 = pml_term(pm.psiE[2][0] + q, bz, az, kz, x2);
+                ty1 = pml_term(pm.psiE[2][1] + q, bz, az, kz, y1);
+            }
+            x1 = tx1, x2 = tx2, y1 = ty1, y2 = ty2, z1 = tz1, z2 = tz2;
+        }
+        float ex = cf[0] * a.Ex[c] + cf[3] * (x1 - x2);
+        float ey = cf[1] * a.Ey[c] + cf[4] * (y1 - y2);
+        float ez = cf[2] * a.Ez[c] + cf[5] * (z1 - z2);
+        long long cs;
+        const float *J = src_plane(a, k, cs);
         if (J) {
             ex -= cf[6] * J[p];
-            ey -= cf[7] * J[a.P + p];
-            ez -= cf[8] * J[2 * a.P + p];
+            ey -= cf[7] * J[cs + p];
+            ez -= cf[8] * J[2 * cs + p];
         }
         a.Ex[c] = ex;
         a.Ey[c] = ey;
@@ -78,8 +140,8 @@ __global__ __launch_bounds__(kBlock) void k_update_E(YeeArgs a) {
 }
 
 // H <- caH*H - cbH*(forward-difference curl of E) - cjH*M   (timefield3d.c:1679-1757)
-template <typename MatT>
-__global__ __launch_bounds__(kBlock) void k_update_H(YeeArgs a) {
+template <typename MatT, bool PML>
+__global__ __launch_bounds__(kBlock) void k_update_H(YeeArgs a, PmlArgs pm) {
     const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int pb = lb % a.plane_blocks, kc = lb / a.plane_blocks;
     const long long p = (long long)pb * kBlock + threadIdx.x;
@@ -89,6 +151,14 @@ __global__ __launch_bounds__(kBlock) void k_update_H(YeeArgs a) {
     const int kb = a.k0 + kc * a.kchunk;
     const int ke = min(a.k1, kb + a.kchunk);
     if (kb >= ke) return;
+    int sx = -1, sy = -1;
+    float bx = 0, ax = 0, kx = 1, by = 0, ay = 0, ky = 1;
+    if (PML) {
+        sx = slab_index(i, a.I, pm.nlo[0], pm.nhi[0]);
+        sy = slab_index(j, a.J, pm.nlo[1], pm.nhi[1]);
+        bx = pm.tab[0][3 * a.I + i], ax = pm.tab[0][4 * a.I + i], kx = pm.tab[0][5 * a.I + i];
+        by = pm.tab[1][3 * a.J + j], ay = pm.tab[1][4 * a.J + j], ky = pm.tab[1][5 * a.J + j];
+    }
     long long c = p + (long long)kb * a.P;
     float ex = a.Ex[c], ey = a.Ey[c];
     for (int k = kb; k < ke; k++, c += a.P) {
@@ -97,17 +167,41 @@ __global__ __launch_bounds__(kBlock) void k_update_H(YeeArgs a) {
         const float ey_ip = a.Ey[c + 1], ez_ip = a.Ez[c + 1];
         const float ex_jp = a.Ex[c + a.I], ez_jp = a.Ez[c + a.I];
         const float *cf = a.coef + (long long)mat_id<MatT>(a.mat, c) * kCoefStride;
-        const float dx = (ez_jp - ez) - (ey_kp - ey);
-        const float dy = (ex_kp - ex) - (ez_ip - ez);
-        const float dz = (ey_ip - ey) - (ex_jp - ex);
-        float hx = cf[9] * a.Hx[c] - cf[12] * dx;
-        float hy = cf[10] * a.Hy[c] - cf[13] * dy;
-        float hz = cf[11] * a.Hz[c] - cf[14] * dz;
-        const float *J = src_plane(a, k);
+        // curl_x = dEz/dy - dEy/dz, curl_y = dEx/dz - dEz/dx, curl_z = dEy/dx - dEx/dy
+        float x1 = ez_jp - ez, x2 = ey_kp - ey;
+        float y1 = ex_kp - ex, y2 = ez_ip - ez;
+        float z1 = ey_ip - ey, z2 = ex_jp - ex;
+        if (PML) {
+            const int sz = slab_index(k, a.K, pm.nlo[2], pm.nhi[2]);
+            const float kz = pm.tab[2][5 * a.K + k];
+            float tx1 = x1 * ky, tx2 = x2 * kz, ty1 = y1 * kz, ty2 = y2 * kx, tz1 = z1 * kx, tz2 = z2 * ky;
+            if (sx >= 0) {
+                const long long q = sx + (long long)pm.W[0] * (j + (long long)a.J * k);
+                ty2 = pml_term(pm.psiH[0][0] + q, bx, ax, kx, y2);
+                tz1 = pml_term(pm.psiH[0][1] + q, bx, ax, kx, z1);
+            }
+            if (sy >= 0) {
+                const long long q = i + (long long)a.I * (sy + (long long)pm.W[1] * k);
+                tz2 = pml_term(pm.psiH[1][0] + q, by, ay, ky, z2);
+                tx1 = pml_term(pm.psiH[1][1] + q, by, ay, ky, x1);
+            }
+            if (sz >= 0) {
+                const float bz = pm.tab[2][3 * a.K + k], az = pm.tab[2][4 * a.K + k];
+                const long long q = p + a.P * sz;
+                tx2 = pml_term(pm.psiH[2][0] + q, bz, az, kz, x2);
+                ty1 = pml_term(pm.psiH[2][1] + q, bz, az, kz, y1);
+            }
+            x1 = tx1, x2 = tx2, y1 = ty1, y2 = ty2, z1 = tz1, z2 = tz2;
+        }
+        float hx = cf[9] * a.Hx[c] - cf[12] * (x1 - x2);
+        float hy = cf[10] * a.Hy[c] - cf[13] * (y1 - y2);
+        float hz = cf[11] * a.Hz[c] - cf[14] * (z1 - z2);
+        long long cs;
+        const float *J = src_plane(a, k, cs);
         if (J) {
-            hx -= cf[15] * J[3 * a.P + p];
-            hy -= cf[16] * J[4 * a.P + p];
-            hz -= cf[17] * J[5 * a.P + p];
+            hx -= cf[15] * J[3 * cs + p];
+            hy -= cf[16] * J[4 * cs + p];
+            hz -= cf[17] * J[5 * cs + p];
         }
         a.Hx[c] = hx;
         a.Hy[c] = hy;
@@ -210,9 +304,69 @@ FaceArgs face_args(fdtd_dev *d, int scheme) {
     return fa;
 }
 
-int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_field) {
+// Host restatement of the CPML grading (same formulas as oracle/fdtd_oracle.c:orc_cpml_profile),
+// evaluated in FP64 and rounded once to FP32.
+void cpml_profile(int N, int nlo, int nhi, const Cpml &c, double disc, double dt, float *out) {
+    for (int n = 0; n < N; n++) {
+        for (int half = 0; half < 2; half++) {
+            double x = n + 0.5 * half;
+            bool in_slab = half == 0 ? (n < nlo || (nhi > 0 && n >= N - nhi))
+                                     : (n < nlo || (nhi > 0 && n >= N - 1 - nhi));
+            double b = 0.0, a = 0.0, kinv = 1.0;
+            if (in_slab) {
+                int npml = n < nlo ? nlo : nhi;
+                double rho = n < nlo ? (nlo - x) / nlo : (x - (N - 1 - nhi)) / nhi;
+                rho = rho > 1.0 ? 1.0 : (rho < 0.0 ? 0.0 : rho);
+                double delta = npml * disc;
+                double smax = -(c.order + 1) * std::log(c.R0) / (2.0 * delta);
+                double rm = std::pow(rho, c.order);
+                double sigma = smax * rm;
+                double kappa = 1.0 + (c.kappa_max - 1.0) * rm;
+                double alpha = c.alpha_max * (1.0 - rho);
+                b = std::exp(-(sigma / kappa + alpha) * dt);
+                double den = sigma * kappa + kappa * kappa * alpha;
+                a = den > 0.0 ? sigma * (b - 1.0) / den : 0.0;
+                kinv = 1.0 / kappa;
+            }
+            out[(3 * half + 0) * N + n] = (float)b;
+            out[(3 * half + 1) * N + n] = (float)a;
+            out[(3 * half + 2) * N + n] = (float)kinv;
+        }
+    }
+}
+
+int ensure_cpml_tables(fdtd_dev *d, double dt) {
+    Cpml &c = d->cpml;
+    if (c.tab_dt == dt) return 0;
+    const int N[3] = {d->I, d->J, d->K};
+    std::vector<float> host((size_t)6 * (N[0] + N[1] + N[2]));
+    for (int a = 0; a < 3; a++) cpml_profile(N[a], c.nlo[a], c.nhi[a], c, d->disc, dt, host.data() + c.tab_off[a]);
+    FDTD_TRY(hipMemcpyAsync(c.tab, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    c.tab_dt = dt;
+    return 0;
+}
+
+PmlArgs pml_args(fdtd_dev *d) {
+    PmlArgs pm{};
+    const Cpml &c = d->cpml;
+    for (int a = 0; a < 3; a++) {
+        pm.tab[a] = c.tab ? c.tab + c.tab_off[a] : nullptr;
+        pm.nlo[a] = c.nlo[a];
+        pm.nhi[a] = c.nhi[a];
+        pm.W[a] = c.W[a];
+        for (int q = 0; q < 2; q++) {
+            pm.psiE[a][q] = c.psi[0][a][q];
+            pm.psiH[a][q] = c.psi[1][a][q];
+        }
+    }
+    return pm;
+}
+
+int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_field, bool pml) {
     int rc = ensure_coefficients(d, dt);
     if (rc) return rc;
+    if (pml && (rc = ensure_cpml_tables(d, dt))) return rc;
     if (scheme != 0 && scheme != 1) {
         set_error("scheme must be 0 (semi-implicit) or 1 (explicit)");
         return -2;
@@ -249,15 +403,34 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
         a.src[a.nsrc].J = sp.J;
         a.nsrc++;
     }
+    a.dense = d->dense_src;
+    a.N = d->N;
+    PmlArgs pm = pml_args(d);
     dim3 grid((unsigned)(a.plane_blocks * zchunks));
-    if (e_field) {
-        if (d->mat_bytes == 1) hipLaunchKernelGGL(k_update_E<uint8_t>, grid, dim3(kBlock), 0, d->stream, a);
-        else hipLaunchKernelGGL(k_update_E<uint16_t>, grid, dim3(kBlock), 0, d->stream, a);
-    } else {
-        if (d->mat_bytes == 1) hipLaunchKernelGGL(k_update_H<uint8_t>, grid, dim3(kBlock), 0, d->stream, a);
-        else hipLaunchKernelGGL(k_update_H<uint16_t>, grid, dim3(kBlock), 0, d->stream, a);
+    const bool u8 = d->mat_bytes == 1;
+    KernelTiming &kt = d->ktime;
+    const int which = e_field ? 0 : 1;
+    hipEvent_t ev_stop = nullptr;
+    if (kt.every > 0 && (kt.launches[which]++ % kt.every) == 0) {
+        hipEvent_t ev_start;
+        FDTD_TRY(hipEventCreate(&ev_start));
+        FDTD_TRY(hipEventCreate(&ev_stop));
+        FDTD_TRY(hipEventRecord(ev_start, d->stream));
+        kt.ev[which].push_back(ev_start);
+        kt.ev[which].push_back(ev_stop);
     }
+#define FDTD_LAUNCH(KER)                                                                                  \
+    do {                                                                                                  \
+        if (u8 && pml) hipLaunchKernelGGL((KER<uint8_t, true>), grid, dim3(kBlock), 0, d->stream, a, pm);   \
+        else if (u8) hipLaunchKernelGGL((KER<uint8_t, false>), grid, dim3(kBlock), 0, d->stream, a, pm);   \
+        else if (pml) hipLaunchKernelGGL((KER<uint16_t, true>), grid, dim3(kBlock), 0, d->stream, a, pm);  \
+        else hipLaunchKernelGGL((KER<uint16_t, false>), grid, dim3(kBlock), 0, d->stream, a, pm);          \
+    } while (0)
+    if (e_field) FDTD_LAUNCH(k_update_E);
+    else FDTD_LAUNCH(k_update_H);
+#undef FDTD_LAUNCH
     FDTD_TRY(hipGetLastError());
+    if (ev_stop) FDTD_TRY(hipEventRecord(ev_stop, d->stream));
     return 0;
 }
 
@@ -291,6 +464,14 @@ int launch_mur(fdtd_dev *d, double dt, int k0, int k1, int field) {
 }
 
 }  // namespace
+
+int cpml_zero(fdtd_dev *d) {
+    Cpml &c = d->cpml;
+    long long n = c.psi_n[0] + c.psi_n[1] + c.psi_n[2];
+    if (c.psi_base && n > 0) FDTD_TRY(hipMemsetAsync(c.psi_base, 0, (size_t)4 * n * sizeof(float), d->stream));
+    return 0;
+}
+
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -336,11 +517,27 @@ int fdtd_dev_set_shells(fdtd_dev *d, const double *old) {
 }
 
 int fdtd_dev_update_E(fdtd_dev *d, double dt, int k0, int k1, int scheme) {
-    return launch_update(d, dt, k0, k1, scheme, true);
+    return launch_update(d, dt, k0, k1, scheme, true, false);
 }
 
 int fdtd_dev_update_H(fdtd_dev *d, double dt, int k0, int k1, int scheme) {
-    return launch_update(d, dt, k0, k1, scheme, false);
+    return launch_update(d, dt, k0, k1, scheme, false, false);
+}
+
+int fdtd_dev_update_E_cpml(fdtd_dev *d, double dt) {
+    if (!d->cpml.on) {
+        set_error("CPML is not enabled on this grid");
+        return -2;
+    }
+    return launch_update(d, dt, 1, d->K - 1, 0, true, true);
+}
+
+int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt) {
+    if (!d->cpml.on) {
+        set_error("CPML is not enabled on this grid");
+        return -2;
+    }
+    return launch_update(d, dt, 1, d->K - 1, 0, false, true);
 }
 
 int fdtd_dev_mur_E(fdtd_dev *d, double dt, int k0, int k1) { return launch_mur(d, dt, k0, k1, 0); }
@@ -349,11 +546,102 @@ int fdtd_dev_mur_H(fdtd_dev *d, double dt, int k0, int k1) { return launch_mur(d
 
 int fdtd_dev_step(fdtd_dev *d, double dt, int scheme) {
     int rc;
+    if (d->cpml.on && scheme == 0) {
+        if ((rc = fdtd_dev_update_E_cpml(d, dt))) return rc;
+        return fdtd_dev_update_H_cpml(d, dt);
+    }
     if ((rc = fdtd_dev_snapshot_shells(d))) return rc;
     if ((rc = fdtd_dev_update_E(d, dt, 1, d->K - 1, scheme))) return rc;
     if ((rc = fdtd_dev_mur_E(d, dt, 0, d->K))) return rc;
     if ((rc = fdtd_dev_update_H(d, dt, 1, d->K - 1, scheme))) return rc;
     if ((rc = fdtd_dev_mur_H(d, dt, 0, d->K))) return rc;
+    return 0;
+}
+
+int fdtd_dev_cpml_on(fdtd_dev *d, const int npml[6], int order, double R0, double kappa_max, double alpha_max) {
+    const int N[3] = {d->I, d->J, d->K};
+    for (int a = 0; a < 3; a++) {
+        if (npml[2 * a] < 0 || npml[2 * a + 1] < 0 || npml[2 * a] + npml[2 * a + 1] + 2 > N[a]) {
+            set_error("PML thickness does not fit the grid");
+            return -2;
+        }
+    }
+    if (!(R0 > 0 && R0 < 1) || order < 0 || kappa_max < 1 || alpha_max < 0) {
+        set_error("invalid PML grading parameters");
+        return -2;
+    }
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    fdtd_dev_cpml_off(d);
+    Cpml &c = d->cpml;
+    c.order = order;
+    c.R0 = R0;
+    c.kappa_max = kappa_max;
+    c.alpha_max = alpha_max;
+    long long off = 0;
+    for (int a = 0; a < 3; a++) {
+        c.nlo[a] = npml[2 * a];
+        c.nhi[a] = npml[2 * a + 1];
+        c.W[a] = c.nlo[a] + (c.nhi[a] > 0 ? c.nhi[a] + 1 : 0);
+        c.tab_off[a] = off;
+        off += 6LL * N[a];
+        c.psi_n[a] = (long long)c.W[a] * N[(a + 1) % 3] * N[(a + 2) % 3];
+    }
+    FDTD_TRY(hipMalloc(&c.tab, (size_t)off * sizeof(float)));
+    long long total = c.psi_n[0] + c.psi_n[1] + c.psi_n[2];
+    if (total > 0) {
+        FDTD_TRY(hipMalloc(&c.psi_base, (size_t)4 * total * sizeof(float)));
+        FDTD_TRY(hipMemsetAsync(c.psi_base, 0, (size_t)4 * total * sizeof(float), d->stream));
+    }
+    float *p = c.psi_base;
+    for (int a = 0; a < 3; a++)
+        for (int f = 0; f < 2; f++)
+            for (int q = 0; q < 2; q++) {
+                c.psi[f][a][q] = c.psi_n[a] > 0 ? p : nullptr;
+                p += c.psi_n[a];
+            }
+    c.tab_dt = -1.0;
+    c.on = true;
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+int fdtd_dev_kernel_timing(fdtd_dev *d, int every) {
+    int cnt;
+    double e, h;
+    int rc = fdtd_dev_kernel_times(d, &cnt, &e, &h);  // drop any pending sample
+    if (rc) return rc;
+    d->ktime.every = every < 0 ? 0 : every;
+    d->ktime.launches[0] = d->ktime.launches[1] = 0;
+    return 0;
+}
+
+int fdtd_dev_kernel_times(fdtd_dev *d, int *count, double *e_ms, double *h_ms) {
+    KernelTiming &kt = d->ktime;
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    double sum[2] = {0, 0};
+    int n[2] = {0, 0};
+    for (int w = 0; w < 2; w++) {
+        for (size_t q = 0; q + 1 < kt.ev[w].size(); q += 2) {
+            float ms = 0;
+            FDTD_TRY(hipEventElapsedTime(&ms, kt.ev[w][q], kt.ev[w][q + 1]));
+            sum[w] += ms;
+            n[w]++;
+        }
+        for (hipEvent_t e : kt.ev[w]) (void)hipEventDestroy(e);
+        kt.ev[w].clear();
+    }
+    *count = n[0] < n[1] ? n[0] : n[1];
+    *e_ms = sum[0];
+    *h_ms = sum[1];
+    return 0;
+}
+
+int fdtd_dev_cpml_off(fdtd_dev *d) {
+    Cpml &c = d->cpml;
+    if (c.tab || c.psi_base) FDTD_TRY(hipStreamSynchronize(d->stream));
+    (void)hipFree(c.tab);
+    (void)hipFree(c.psi_base);
+    c = Cpml();
     return 0;
 }
 

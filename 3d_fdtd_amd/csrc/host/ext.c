@@ -50,7 +50,68 @@ int fdtd3d_ext_timer_stop(timefield3d psi, double *elapsed_ms) {
 
 int fdtd3d_ext_synchronize(timefield3d psi) { return fdtd_dev_synchronize(tf_dev(psi)); }
 
+int fdtd3d_ext_kernel_timing(timefield3d psi, int every) { return fdtd_dev_kernel_timing(tf_dev(psi), every); }
+
+int fdtd3d_ext_kernel_times(timefield3d psi, int *count, double *e_ms, double *h_ms) {
+    return fdtd_dev_kernel_times(tf_dev(psi), count, e_ms, h_ms);
+}
+
 int fdtd3d_ext_steps(timefield3d psi, double dt, int n) {
     for (int s = 0; s < n; s++) tf_step(psi, dt);
     return 0;
+}
+
+int fdtd3d_ext_set_cpml(timefield3d psi, const int npml[6], int order, double R0, double kappa_max,
+                        double alpha_max) {
+    return fdtd_dev_cpml_on(tf_dev(psi), npml, order, R0, kappa_max, alpha_max);
+}
+
+int fdtd3d_ext_set_mur(timefield3d psi) { return fdtd_dev_cpml_off(tf_dev(psi)); }
+
+int fdtd3d_ext_run_clear(timefield3d psi) { return fdtd_dev_run_clear(tf_dev(psi)); }
+
+/* Power-normalised copy of a mode's fields (interleaved complex), as mode_power_normalize
+ * would leave them, without touching the caller's mode. */
+static double *normalised_copy(mode m) {
+    mode c = mode_copy(m);
+    mode_power_normalize(c);
+    size_t n = (size_t)12 * m.i * m.j;
+    double *out = malloc(n * sizeof(double));
+    memcpy(out, c.fields.m, n * sizeof(double));
+    mode_free(c);
+    return out;
+}
+
+int fdtd3d_ext_run_add_mode_source(timefield3d psi, int cx, int cy, int cz, mode m, int reverse, int adjoint,
+                                   int nsteps, const double *amplitude, const double *phase) {
+    fdtd_dev *d = tf_dev(psi);
+    double *f = normalised_copy(m);
+    int rc = fdtd_dev_run_add_source(d, cx, cy, cz, f, m.i, m.j, m.disc, reverse, adjoint, nsteps, amplitude, phase);
+    free(f);
+    return rc;
+}
+
+int fdtd3d_ext_run_add_overlap(timefield3d psi, int cx, int cy, int cz, mode m, int reverse, int nfreq,
+                               const double *omega) {
+    fdtd_dev *d = tf_dev(psi);
+    double *f = normalised_copy(m);
+    int rc = fdtd_dev_run_add_overlap(d, cx, cy, cz, f, m.i, m.j, m.disc, reverse, nfreq, omega);
+    free(f);
+    return rc;
+}
+
+int fdtd3d_ext_run_add_dft_plane(timefield3d psi, int k, int comp_mask, int nfreq, const double *omega) {
+    return fdtd_dev_run_add_dft_plane(tf_dev(psi), k, comp_mask, nfreq, omega);
+}
+
+int fdtd3d_ext_run(timefield3d psi, double dt, int n0, int count, double t_offset) {
+    return fdtd_dev_run(tf_dev(psi), dt, n0, count, t_offset);
+}
+
+int fdtd3d_ext_run_overlap(timefield3d psi, int id, complex *cumulative, complex *last) {
+    return fdtd_dev_run_overlap_result(tf_dev(psi), id, (double *)cumulative, (double *)last);
+}
+
+int fdtd3d_ext_run_dft(timefield3d psi, int id, complex *out) {
+    return fdtd_dev_run_dft_result(tf_dev(psi), id, (double *)out);
 }

@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark: FP32 Yee cell-updates/s of the straight Si strip waveguide workload on one MI355X.
+
+Workload (propagation along x, cross-section y lateral x z vertical, 20 nm cells):
+  240 x 120 x 100 cells, SiO2 background (n = 1.44), Si core (n = 3.48) 500 nm wide x 220 nm
+  tall, 10-cell CPML on every face, a TE mode launched as a soft source at x = 20 with a
+  Gaussian-modulated carrier (lambda0 = 1.55 um, bandwidth 0.2 um), FP64 mode-overlap monitors
+  (launched / reflected at x = 25, transmitted at x = 220) each with a running DFT at 16
+  wavelengths 1.50-1.60 um, 4000 steps.  The grid is stored in the library's own axes
+  (reference z = propagation): I = 120 (lateral), J = 100 (vertical), K = 240 (propagation).
+  The launched mode profile is the library's mode_gaussian (an Hy Gaussian completed to E/Hz by
+  the discrete curl); it stands in for the eigen-solver mode, which the library does not ship.
+
+A "step" is one full time step of this workload (sources, E and H updates with CPML, monitors);
+--steps K times K consecutive steps of one run after --warmup W untimed steps.
+
+Output: one JSON line (see the keys below).  `roofline` covers the dominant kernel (the E or H
+update, whichever takes longer on average), its duration measured with HIP events on the
+grid's stream over the timed region (every 10th launch sampled).  `cpu_baseline` times the
+compiled reference's timefield3d_apply_timestep_omp_full (FP64, Mur boundary) on the same grid
+size on the host cores (oracle/_ref; the clean-room FP64 oracle when that is absent).
+"""
+import argparse
+import ctypes as C
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Yee cell-updates/s & % HBM roofline; verified taper designs/hour"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell for one split update kernel: 3 components read + written, the other
+# field's 3 components read (FP32)
+SPLIT_KERNEL_BYTES = 36
+STEP_BYTES = 48  # 6 components x (4 B read + 4 B written) per cell-update
+
+DISC = 0.02
+I, J, K = 120, 100, 240
+NPML = 10
+N_RUN = 4000
+
+
+def workload(fd, steps_total):
+    L = fd.lib()
+    dt = 0.99 * DISC / math.sqrt(3)
+    g = fd.Grid(I, J, K, DISC)
+    L.timefield3d_set_background(g.tf, 1.44 ** 2, 1.0)
+    ci, cj = I // 2, J // 2
+    L.timefield3d_set_prism_permittivity(g.tf, 3.48 ** 2, ci - 12, ci + 13, cj - 5, cj + 6, 0, K)
+    npml = (C.c_int * 6)(*([NPML] * 6))
+    fd.check(L.fdtd3d_ext_set_cpml(g.tf, npml, 3, 1e-8, 1.0, 0.05), "set_cpml")
+    lam0, dlam = 1.55, 0.2
+    w0 = 2 * math.pi / lam0
+    md = L.mode_gaussian(60, 50, DISC, fd.Complex(1.44 ** 2, 0), fd.Complex(1.0, 0), w0, 8.0, 0)
+    sig_t = lam0 * lam0 / (2 * math.pi * dlam)
+    t = np.arange(steps_total) * dt
+    t0 = 4 * sig_t
+    amp = np.exp(-(t - t0) ** 2 / (2 * sig_t ** 2))
+    phase = w0 * t
+    dptr = C.POINTER(C.c_double)
+    src = L.fdtd3d_ext_run_add_mode_source(g.tf, ci, cj, 20, md, 0, 0, steps_total, amp.ctypes.data_as(dptr),
+                                           phase.ctypes.data_as(dptr))
+    fd.check(0 if src >= 0 else src, "add source")
+    omegas = 2 * math.pi / np.linspace(1.50, 1.60, 16)
+    mons = []
+    for kz, rev in ((25, 0), (25, 1), (K - 20, 0)):
+        m = L.fdtd3d_ext_run_add_overlap(g.tf, ci, cj, kz, md, rev, len(omegas), omegas.ctypes.data_as(dptr))
+        fd.check(0 if m >= 0 else m, "add overlap")
+        mons.append(m)
+    return g, dt, md, mons, omegas
+
+
+def cpu_baseline(target_s=12.0):
+    """Reference FP64 step on the same grid size, host cores, bounded sample."""
+    threads = int(os.environ.get("FDTD3D_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    dt = 0.99 * DISC / math.sqrt(3)
+    from oracle import ref_py
+
+    if ref_py.available():
+        L = ref_py.lib()
+        tf = L.timefield3d_new(I, J, K, DISC)
+        L.timefield3d_set_background(tf, 1.44 ** 2, 1.0)
+        L.timefield3d_set_prism_permittivity(tf, 3.48 ** 2, I // 2 - 12, I // 2 + 13, J // 2 - 5, J // 2 + 6, 0, K)
+        A = ref_py.tf_arrays(tf)
+        A["fields"][:] = np.random.default_rng(0).uniform(-1, 1, A["fields"].size)
+
+        def step():
+            L.timefield3d_apply_timestep_omp_full(tf, dt)
+
+        kind = "reference"
+        cleanup = lambda: L.timefield3d_free(tf)  # noqa: E731
+    else:
+        from oracle import oracle_py as O
+
+        o = O.Grid(I, J, K, DISC)
+        o.f[:] = np.random.default_rng(0).uniform(-1, 1, o.f.size)
+
+        def step():
+            o.step(dt)
+
+        kind = "port"
+        cleanup = lambda: None  # noqa: E731
+    step()  # warm
+    t0 = time.perf_counter()
+    step()
+    est = time.perf_counter() - t0
+    n = max(2, min(2000, int(target_s / max(est, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        step()
+    el = time.perf_counter() - t0
+    cleanup()
+    rate = I * J * K * n / el
+    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"{n} steps of timefield3d_apply_timestep_omp_full (FP64, Mur) on the {I}x{J}x{K} grid, "
+                      f"{el:.1f} s, OMP_NUM_THREADS={threads}"}
+
+
+def read_traffic():
+    p = os.path.join(ROOT, "profiles", "yee_traffic_cfg2.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        return json.load(fh)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=N_RUN)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the path would shard by geometry or by x-slabs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    fd = importlib.import_module("3d_fdtd_amd")
+    L = fd.lib()
+    W, S = args.warmup, args.steps
+    g, dt, md, mons, omegas = workload(fd, W + S)
+    print(f"bench: warmup {W} steps", file=sys.stderr, flush=True)
+    fd.check(L.fdtd3d_ext_run(g.tf, dt, 0, W, 0.0), "warmup run")
+    fd.check(L.fdtd3d_ext_synchronize(g.tf), "sync")
+    fd.check(L.fdtd3d_ext_kernel_timing(g.tf, 10), "kernel timing")
+    print(f"bench: timing {S} steps", file=sys.stderr, flush=True)
+    fd.check(L.fdtd3d_ext_timer_start(g.tf), "timer")
+    t0 = time.perf_counter()
+    fd.check(L.fdtd3d_ext_run(g.tf, dt, W, S, 0.0), "timed run")
+    ms = C.c_double()
+    fd.check(L.fdtd3d_ext_timer_stop(g.tf, C.byref(ms)), "timer")
+    wall = time.perf_counter() - t0
+    cnt, e_ms, h_ms = C.c_int(), C.c_double(), C.c_double()
+    fd.check(L.fdtd3d_ext_kernel_times(g.tf, C.byref(cnt), C.byref(e_ms), C.byref(h_ms)), "kernel times")
+    fd.check(L.fdtd3d_ext_kernel_timing(g.tf, 0), "kernel timing off")
+    elapsed_s = max(ms.value / 1e3, wall)
+    cells = I * J * K
+    value = cells * S / elapsed_s
+    # transmission |t|^2 at 1.55 um as a sanity print
+    cum = (fd.Complex * len(omegas))()
+    launched = (fd.Complex * len(omegas))()
+    L.fdtd3d_ext_run_overlap(g.tf, mons[2], cum, None)
+    L.fdtd3d_ext_run_overlap(g.tf, mons[0], launched, None)
+    tr = [abs(complex(cum[q].re, cum[q].im) / complex(launched[q].re, launched[q].im)) ** 2
+          if (launched[q].re or launched[q].im) else float("nan") for q in range(len(omegas))]
+    print(f"bench: |t|^2 over 1.50-1.60 um: {min(tr):.4f} .. {max(tr):.4f}", file=sys.stderr, flush=True)
+    n = max(cnt.value, 1)
+    e_avg, h_avg = e_ms.value / n, h_ms.value / n
+    dom_name, dom_ms = ("yee_update_E", e_avg) if e_avg >= h_avg else ("yee_update_H", h_avg)
+    achieved = SPLIT_KERNEL_BYTES * cells / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    traffic = read_traffic()
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic.get(dom_name) if traffic else None,
+            "kernel": dom_name, "kernel_avg_us": round(dom_ms * 1e3, 2),
+            "step_kernels_avg_us": round((e_avg + h_avg) * 1e3, 2),
+            "step_frac_48B": round(STEP_BYTES * cells / ((e_avg + h_avg) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+            if e_avg + h_avg > 0 else None}
+    out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": S, "warmup": W,
+           "ms_per_step": elapsed_s * 1e3 / S, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "Si strip waveguide 240x120x100 (x-propagation, 20 nm cells), 10-cell CPML, "
+                                  "mode source, FP32 fields, 3 FP64 overlap monitors x 16 wavelengths",
+                      "grid_lib_axes": [I, J, K], "steps_per_run": N_RUN},
+           "roofline": roof}
+    if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    L.mode_free(md)
+    g.free()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -48,6 +48,32 @@ int fdtd_dev_mur_E(fdtd_dev *d, double dt, int k0, int k1);                   /*
 int fdtd_dev_mur_H(fdtd_dev *d, double dt, int k0, int k1);                   /* timefield3d.c:2275 */
 int fdtd_dev_step(fdtd_dev *d, double dt, int scheme);                        /* timefield3d.c:2787 */
 
+/* Boundary: convolutional PML (no reference counterpart; the reference has only Mur faces).
+ * npml[6] = cells on faces x-lo, x-hi, y-lo, y-hi, z-lo, z-hi; grading order, target reflection
+ * R0, kappa_max, alpha_max.  While enabled, fdtd_dev_step runs the PML update with PEC outer
+ * walls instead of the Mur faces.  fdtd_dev_cpml_off restores the reference boundary. */
+int fdtd_dev_cpml_on(fdtd_dev *d, const int npml[6], int order, double R0, double kappa_max, double alpha_max);
+int fdtd_dev_cpml_off(fdtd_dev *d);
+int fdtd_dev_update_E_cpml(fdtd_dev *d, double dt);
+int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt);
+
+/* Device-resident run (extension).  A run holds mode sources whose per-step amplitude and
+ * phase are tabulated up front, FP64 mode-overlap monitors with a running DFT at nfreq angular
+ * frequencies, and FP64 multi-frequency DFT planes.  fdtd_dev_run(n0, n) advances n steps; step
+ * n writes the sources with amplitude[n]/phase[n], runs fdtd_dev_step and then accumulates the
+ * monitors with twiddle e^{i omega (n + t_offset) dt}.  No host synchronisation happens inside. */
+int fdtd_dev_run_clear(fdtd_dev *d);
+int fdtd_dev_run_add_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
+                            int reverse, int adjoint, int nsteps, const double *amplitude, const double *phase);
+int fdtd_dev_run_add_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
+                             int reverse, int nfreq, const double *omega);
+int fdtd_dev_run_add_dft_plane(fdtd_dev *d, int k, int comp_mask, int nfreq, const double *omega);
+int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset);
+/* Monitor results: overlap -> cumulative[2*nfreq] (and the last instantaneous value);
+ * dft plane -> [comp][freq][J][I] complex interleaved. */
+int fdtd_dev_run_overlap_result(fdtd_dev *d, int id, double *cumulative, double *last_inst);
+int fdtd_dev_run_dft_result(fdtd_dev *d, int id, double *out);
+
 /* Soft sources on plane k = cz.  mode is interleaved complex double[6*mi*mj] (already
  * power-normalised by the caller).  adjoint selects the CW adjoint form (timefield3d.c:3028). */
 int fdtd_dev_mode_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
@@ -76,6 +102,12 @@ int fdtd_dev_block_update_gradient(fdtd_dev *d, const void *key, const int box[6
                                    double *gradient_host);
 int fdtd_dev_block_reset(const void *key, int which);
 void fdtd_dev_block_release(const void *key);
+
+/* Per-kernel timing of the Yee updates: while enabled, every `every`-th E and H launch is
+ * bracketed by HIP events on the grid's stream; kernel_times returns the sampled launch count
+ * and the summed durations (ms) of the E and H kernels, and resets the sample. */
+int fdtd_dev_kernel_timing(fdtd_dev *d, int every);
+int fdtd_dev_kernel_times(fdtd_dev *d, int *count, double *e_ms, double *h_ms);
 
 /* Timing of the work enqueued on the grid's stream (HIP events on that stream). */
 int fdtd_dev_timer_start(fdtd_dev *d);

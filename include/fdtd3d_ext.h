@@ -25,10 +25,45 @@ int fdtd3d_ext_upload_source_plane(timefield3d psi, int k);
 int fdtd3d_ext_timer_start(timefield3d psi);
 int fdtd3d_ext_timer_stop(timefield3d psi, double *elapsed_ms);
 int fdtd3d_ext_synchronize(timefield3d psi);
+/* Sample every `every`-th E and H update launch with HIP events (0 switches sampling off);
+ * kernel_times returns the number of sampled steps and the summed E / H kernel times (ms). */
+int fdtd3d_ext_kernel_timing(timefield3d psi, int every);
+int fdtd3d_ext_kernel_times(timefield3d psi, int *count, double *e_ms, double *h_ms);
 
 /* Run n reference steps (timefield3d_apply_timestep_omp_full semantics) without host work in
  * between; sources stay as last set. */
 int fdtd3d_ext_steps(timefield3d psi, double delta_t, int n);
+
+/* Boundary.  The reference terminates the grid with first-order Mur faces (the default).
+ * fdtd3d_ext_set_cpml switches every later step of psi -- including
+ * timefield3d_apply_timestep_omp_full -- to a convolutional PML (Roden-Gedney, CFS grading)
+ * of npml[6] cells on faces x-lo, x-hi, y-lo, y-hi, z-lo, z-hi in front of PEC walls:
+ * sigma = -(order+1) ln(R0) / (2 npml dx) rho^order, kappa = 1 + (kappa_max-1) rho^order,
+ * alpha = alpha_max (1 - rho).  fdtd3d_ext_set_mur restores the reference boundary. */
+int fdtd3d_ext_set_cpml(timefield3d psi, const int npml[6], int order, double R0, double kappa_max,
+                        double alpha_max);
+int fdtd3d_ext_set_mur(timefield3d psi);
+
+/* Device-resident runs: sources with tabulated amplitude/phase, FP64 overlap monitors with a
+ * running DFT at several angular frequencies, FP64 multi-frequency DFT planes.  Step n of
+ * fdtd3d_ext_run(psi, dt, n0, count, t_offset) writes each source as the reference's
+ * timefield3d_mode_source(_adjoint) would with amplitude[n] and phase[n] (a power-normalised copy
+ * of the mode is used; the caller's mode is not modified), advances one step, then adds
+ * e^{i omega (n + t_offset) dt} times the instantaneous overlap (or the field) to each monitor.
+ * Sources are off outside [0, nsteps).  The add functions return a non-negative id. */
+int fdtd3d_ext_run_clear(timefield3d psi);
+int fdtd3d_ext_run_add_mode_source(timefield3d psi, int center_x, int center_y, int center_z, mode source_mode,
+                                   int reverse, int adjoint, int nsteps, const double *amplitude,
+                                   const double *phase);
+int fdtd3d_ext_run_add_overlap(timefield3d psi, int center_x, int center_y, int center_z, mode overlap_mode,
+                               int reverse, int nfreq, const double *omega);
+/* comp_mask bit m selects field component m (Ex=bit0 ... Hz=bit5) on plane z = k. */
+int fdtd3d_ext_run_add_dft_plane(timefield3d psi, int k, int comp_mask, int nfreq, const double *omega);
+int fdtd3d_ext_run(timefield3d psi, double delta_t, int n0, int count, double t_offset);
+/* cumulative[nfreq] and the last instantaneous overlap (either pointer may be NULL). */
+int fdtd3d_ext_run_overlap(timefield3d psi, int id, complex *cumulative, complex *last);
+/* [selected comp][freq][j][i] complex values of a DFT plane. */
+int fdtd3d_ext_run_dft(timefield3d psi, int id, complex *out);
 
 const char *fdtd3d_ext_last_error(void);
 

@@ -624,7 +624,8 @@ int orc_taper_update_symmetric(int seg_num, int *left, int *right, double *left_
  * The psi recursion acts on the raw difference d (not divided by dx):
  *   psi <- b psi + a d,  curl term = (d / kappa + psi) / dx.
  * Membership: E position n is in the slab if n < nlo or n >= N-nhi; H position if n < nlo or
- * n >= N-1-nhi.  Outside, psi stays 0 and 1/kappa = 1, so the update reduces exactly to the
+ * n >= N-1-nhi (high-side rules only when nhi > 0).  Outside, psi stays 0 and 1/kappa = 1, so
+ * the update reduces exactly to the
  * reference interior update.  Boundary cells (index 0 and N-1) are not updated (PEC wall).
  */
 void orc_cpml_profile(int N, int nlo, int nhi, const orc_cpml_params *p, double disc, double dt,
@@ -635,8 +636,8 @@ void orc_cpml_profile(int N, int nlo, int nhi, const orc_cpml_params *p, double 
             double rho = 0.0;
             int npml = 0;
             int in_slab;
-            if (half == 0) in_slab = (n < nlo) || (n >= N - nhi);
-            else in_slab = (n < nlo) || (n >= N - 1 - nhi);
+            if (half == 0) in_slab = (n < nlo) || (nhi > 0 && n >= N - nhi);
+            else in_slab = (n < nlo) || (nhi > 0 && n >= N - 1 - nhi);
             if (in_slab) {
                 if (n < nlo) { rho = (nlo - x) / nlo; npml = nlo; }
                 else { rho = (x - (N - 1 - nhi)) / nhi; npml = nhi; }
@@ -662,8 +663,8 @@ void orc_cpml_profile(int N, int nlo, int nhi, const orc_cpml_params *p, double 
     }
 }
 
-static inline int in_slab_E(int n, int N, int nlo, int nhi) { return n < nlo || n >= N - nhi; }
-static inline int in_slab_H(int n, int N, int nlo, int nhi) { return n < nlo || n >= N - 1 - nhi; }
+static inline int in_slab_E(int n, int N, int nlo, int nhi) { return n < nlo || (nhi > 0 && n >= N - nhi); }
+static inline int in_slab_H(int n, int N, int nlo, int nhi) { return n < nlo || (nhi > 0 && n >= N - 1 - nhi); }
 
 void orc_update_E_cpml(const orc_grid *g, const orc_cpml *cp, double dt) {
     const int N[3] = {g->I, g->J, g->K};

@@ -1,0 +1,152 @@
+"""GPU tests of the extensions (no reference counterpart): CPML boundary and device-resident runs
+with tabulated sources, FP64 overlap monitors and multi-frequency DFT planes.  The FP64 oracle
+carries the same formulas (its CPML is pinned by the CPU tests in test_oracle_invariants.py);
+gate: relative L2 <= 1e-4 for FP32 fields, same bound for FP64 monitors fed by FP32 fields."""
+import ctypes as C
+import importlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+fd = importlib.import_module("3d_fdtd_amd")
+L = fd.lib
+REL = 1e-4
+DP = C.POINTER(C.c_double)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(got, want):
+    return float(np.linalg.norm(np.asarray(got) - np.asarray(want)) / max(np.linalg.norm(want), 1e-300))
+
+
+@pytest.mark.parametrize("npml", [(4, 3, 5, 2, 3, 4), (0, 0, 6, 6, 0, 5)])
+def test_cpml_step_vs_oracle(npml):
+    rng = np.random.default_rng(sum(npml))
+    I, J, K, disc = 22, 18, 26, 0.02
+    dt = 0.99 * disc / np.sqrt(3)
+    say("cpml case", npml)
+    g = fd.Grid(I, J, K, disc)
+    o = O.Grid(I, J, K, disc)
+    ep = rng.uniform(1, 12, 3 * I * J * K).astype(np.float32)
+    sig = np.zeros(6 * I * J * K, np.float32)
+    sig[0::6] = rng.uniform(0, 2, I * J * K).astype(np.float32)
+    f0 = rng.uniform(-1, 1, 6 * I * J * K).astype(np.float32)
+    for a, v in (("ep", ep), ("sig", sig), ("fields", f0)):
+        g.a[a][:] = v
+    o.ep[:] = ep
+    o.sig[:] = sig
+    o.f[:] = f0
+    g.push()
+    fd.check(L().fdtd3d_ext_set_cpml(g.tf, (C.c_int * 6)(*npml), 3, 1e-6, 2.0, 0.1), "set_cpml")
+    cp = O.Cpml(o, npml, dt, order=3, R0=1e-6, kappa_max=2.0, alpha_max=0.1)
+    for _ in range(40):
+        L().timefield3d_apply_timestep_omp_full(g.tf, dt)
+        O.step_cpml(o, cp, dt)
+    err = rel_l2(g.fields(), o.f)
+    say("cpml rel L2", err)
+    assert err <= REL
+    # zero_fields clears the PML memory too: a second identical run reproduces the first
+    first = g.fields().copy()
+    L().timefield3d_zero_fields(g.tf)
+    g.a["fields"][:] = f0
+    g.push()
+    for _ in range(40):
+        L().timefield3d_apply_timestep_omp_full(g.tf, dt)
+    assert np.array_equal(g.fields(), first)
+    g.free()
+
+
+def test_run_api_matches_reference_monitors(golden_dir):
+    """Device-resident run (tabulated source, overlap monitors with DFT) vs the reference's
+    per-step source/overlap calls recorded in fixture F4 (single frequency)."""
+    z = np.load(os.path.join(golden_dir, "F4_mode_source_overlap.npz"))
+    I, J, K = (int(v) for v in z["dims"])
+    dt, om = float(z["dt"]), float(z["omega"])
+    g = fd.Grid(I, J, K, float(z["disc"]))
+    g.a["ep"][:] = z["ep"]
+    g.a["mu"][:] = z["mu"]
+    g.push()
+    m1 = fd.mode_from_array(30, 30, float(z["disc"]), z["mode1"])
+    m2 = fd.mode_from_array(24, 24, float(z["disc"]), z["mode2"])
+    n = int(z["fwd_steps"])
+    steps = np.arange(n)
+    amp = np.exp(-(steps - 60) ** 2 / (2 * 25.0 ** 2))
+    phase = steps * dt * om
+    sid = L().fdtd3d_ext_run_add_mode_source(g.tf, I // 2, J // 2, 10, m1, 0, 0, n, amp.ctypes.data_as(DP),
+                                             phase.ctypes.data_as(DP))
+    assert sid >= 0
+    oms = np.array([om])
+    ids = [L().fdtd3d_ext_run_add_overlap(g.tf, I // 2, J // 2, kz, mm, rev, 1, oms.ctypes.data_as(DP))
+           for kz, rev, mm in ((15, 0, m1), (15, 1, m1), (45, 0, m2))]
+    assert min(ids) >= 0
+    # two calls, to exercise the step offset
+    fd.check(L().fdtd3d_ext_run(g.tf, dt, 0, 50, 0.0), "run")
+    fd.check(L().fdtd3d_ext_run(g.tf, dt, 50, n - 50, 0.0), "run")
+    want = z["fwd_cum"]
+    for q, mid in enumerate(ids):
+        cum, last = fd.Complex(), fd.Complex()
+        fd.check(L().fdtd3d_ext_run_overlap(g.tf, mid, C.byref(cum), C.byref(last)), "result")
+        err = abs(complex(cum.re, cum.im) - complex(*want[q])) / np.abs(want).max()
+        say("run monitor", q, complex(cum.re, cum.im), complex(*want[q]), err)
+        assert err <= REL
+        inst_err = abs(complex(last.re, last.im) - complex(*z["fwd_inst"][n - 1, q])) / np.abs(z["fwd_inst"][:, q]).max()
+        assert inst_err <= REL
+    # the caller's modes are untouched by the run API
+    assert np.array_equal(fd.mode_fields(m1), z["mode1"])
+    g.free()
+
+
+def test_run_dft_plane_vs_oracle(golden_dir):
+    """Multi-frequency FP64 DFT plane vs the oracle's reference-semantics DFT on a plane view."""
+    z = np.load(os.path.join(golden_dir, "F4_mode_source_overlap.npz"))
+    I, J, K = (int(v) for v in z["dims"])
+    dt, om = float(z["dt"]), float(z["omega"])
+    n = 80
+    g = fd.Grid(I, J, K, float(z["disc"]))
+    g.a["ep"][:] = z["ep"]
+    g.a["mu"][:] = z["mu"]
+    g.push()
+    m1 = fd.mode_from_array(30, 30, float(z["disc"]), z["mode1"])
+    steps = np.arange(n)
+    amp = np.exp(-(steps - 40) ** 2 / (2 * 15.0 ** 2))
+    phase = steps * dt * om
+    assert L().fdtd3d_ext_run_add_mode_source(g.tf, I // 2, J // 2, 10, m1, 0, 0, n, amp.ctypes.data_as(DP),
+                                              phase.ctypes.data_as(DP)) >= 0
+    oms = np.array([om, 0.8 * om])
+    pid = L().fdtd3d_ext_run_add_dft_plane(g.tf, 30, 0b010011, len(oms), oms.ctypes.data_as(DP))
+    assert pid >= 0
+    fd.check(L().fdtd3d_ext_run(g.tf, dt, 0, n, 0.0), "run")
+    out = (fd.Complex * (3 * len(oms) * I * J))()
+    fd.check(L().fdtd3d_ext_run_dft(g.tf, pid, out), "dft result")
+    got = np.ctypeslib.as_array(C.cast(out, DP), shape=(3, len(oms), J, I, 2))
+    # oracle
+    o = O.Grid(I, J, K, float(z["disc"]))
+    o.ep[:] = z["ep"]
+    o.mu[:] = z["mu"]
+    mo = z["mode1"].copy()
+    P = I * J
+    bufs = [np.zeros(12 * P) for _ in oms]
+    for s in range(n):
+        O.lib().orc_mode_source(o.ptr, I // 2, J // 2, 10, 30, 30, o.disc, O.dptr(mo), phase[s], amp[s], 0)
+        o.step(dt)
+        view_f = o.f[6 * P * 30: 6 * P * 31]
+        vs = O.OrcGrid(I, J, 1, o.disc, O.fptr(o.ep), O.fptr(o.mu), O.fptr(o.sig), O.dptr(view_f), O.dptr(o.old),
+                       O.fptr(o.src))
+        for q, w in enumerate(oms):
+            O.lib().orc_dft_volume(C.byref(vs), s * dt * w, O.dptr(bufs[q]))
+    want = np.stack([b.reshape(J, I, 6, 2) for b in bufs])  # [f, j, i, m, 2]
+    want = np.stack([want[:, :, :, m, :] for m in (0, 1, 4)])  # [comp, f, j, i, 2]
+    err = rel_l2(got, want)
+    say("run DFT plane rel L2", err)
+    assert err <= REL
+    g.free()
