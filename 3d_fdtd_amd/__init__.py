@@ -120,6 +120,8 @@ SIGNATURES = {
     "adjoint3d_update_gradient": ([Block, TimeField, _i], None),
     "adjoint3d_reset_gradient": ([Block], None),
     "adjoint3d_reset_forward_fields": ([Block], None),
+    "adjoint3d_adjust_permittivity": ([Block, TimeField, _d, _d, _d, _d], None),
+    "adjoint3d_taper_calculate_gradients": ([Taper, Block, _d], None),
     "adjoint3d_taper_new": ([_d, _d, _i, _i, _i, _P(_i), _P(_i), _i, _i, _i], Taper),
     "adjoint3d_taper_insert": ([TimeField, Taper], None),
     "adjoint3d_taper_store_forward_fields": ([Taper, TimeField, _i], None),
@@ -138,7 +140,8 @@ SIGNATURES = {
     "fdtd3d_ext_steps": ([TimeField, _d, _i], _i),
     "fdtd3d_ext_last_error": ([], C.c_char_p),
     "fdtd3d_ext_kernel_timing": ([TimeField, _i], _i),
-    "fdtd3d_ext_kernel_times": ([TimeField, _P(_i), _P(_d), _P(_d)], _i),
+    "fdtd3d_ext_kernel_times": ([TimeField, _P(_i), _P(_d)], _i),
+    "fdtd3d_ext_set_step_kernel": ([TimeField, _i, _i], _i),
     "fdtd3d_ext_set_cpml": ([TimeField, _P(_i), _i, _d, _d, _d], _i),
     "fdtd3d_ext_set_mur": ([TimeField], _i),
     "fdtd3d_ext_run_clear": ([TimeField], _i),

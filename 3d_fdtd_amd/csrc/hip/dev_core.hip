@@ -185,6 +185,11 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->shell = nullptr;
     d->dense_src = nullptr;
     d->run = nullptr;
+    d->alt_base = nullptr;
+    d->alt_valid = false;
+    d->n_run_src = 0;
+    d->fused_rows = 4;
+    d->fused_kchunk = 16;
     hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     // fields: 6 arrays, each N floats, each starting on a 256-byte boundary
     d->Nal = (d->N + 63) / 64 * 64;
@@ -216,6 +221,7 @@ void fdtd_dev_destroy(fdtd_dev *d) {
     if (!d) return;
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     (void)hipFree(d->field_base);
+    (void)hipFree(d->alt_base);
     (void)hipFree(d->mat);
     (void)hipFree(d->d_props);
     (void)hipFree(d->d_coef);
@@ -301,6 +307,7 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
 }
 
 int fdtd_dev_set_fields(fdtd_dev *d, const double *fields) {
+    d->alt_valid = false;
     double *tmp = nullptr;
     FDTD_TRY(hipMalloc(&tmp, (size_t)6 * d->N * sizeof(double)));
     hipError_t e = hipMemcpyAsync(tmp, fields, (size_t)6 * d->N * sizeof(double), hipMemcpyHostToDevice, d->stream);
@@ -389,6 +396,10 @@ int fdtd_dev_get_sources(fdtd_dev *d, float *sources) {
 
 int fdtd_dev_zero(fdtd_dev *d) {
     FDTD_TRY(hipMemsetAsync(d->field_base, 0, (size_t)6 * d->Nal * sizeof(float), d->stream));
+    if (d->alt_base) {
+        FDTD_TRY(hipMemsetAsync(d->alt_base, 0, (size_t)6 * d->Nal * sizeof(float), d->stream));
+        d->alt_valid = true;
+    }
     long long shell_n = d->shell_off[5] + 8LL * d->I * d->J;
     FDTD_TRY(hipMemsetAsync(d->shell, 0, (size_t)shell_n * sizeof(float), d->stream));
     for (auto &p : d->src) FDTD_TRY(hipMemsetAsync(p.J, 0, (size_t)6 * d->P * sizeof(float), d->stream));
@@ -421,6 +432,20 @@ int fdtd_dev_set_fourier(fdtd_dev *d, const double *in) {
         for (int m = 0; m < 6; m++) soa[m * d->N + c] = make_double2(in[2 * (6 * c + m)], in[2 * (6 * c + m) + 1]);
     FDTD_TRY(hipStreamSynchronize(d->stream));
     FDTD_TRY(hipMemcpy(d->fourier, soa.data(), soa.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk) {
+    if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8) {
+        set_error("fused step rows per lane must be 0 (split kernels), 1, 2, 4 or 8");
+        return -2;
+    }
+    if (rows != 0 && kchunk < 1) {
+        set_error("fused step z chunk must be positive");
+        return -2;
+    }
+    d->fused_rows = rows;
+    d->fused_kchunk = kchunk;
     return 0;
 }
 

@@ -32,6 +32,17 @@ struct SrcPlaneArg {
     const float *J;  // 6 SoA planes of I*J floats (Jx,Jy,Jz,Mx,My,Mz)
 };
 
+// A tabulated mode source evaluated inside the Yee kernels during a device-resident run:
+// inside the mode window J_m = c1 PR_m + c2 PI_m (PR/PI: 6 x P planes), outside it the stored
+// source plane applies.
+constexpr int kMaxRunSrc = 2;
+struct RunSrcArg {
+    int k;
+    const float *PR, *PI;
+    const unsigned char *win;
+    float c1, c2;
+};
+
 struct SrcPlane {
     int k;
     float *J;          // device; component m of cell p at J[m * stride + p]
@@ -67,6 +78,7 @@ struct Cpml {
     long long tab_off[3] = {0, 0, 0};
     double tab_dt = -1.0;
     float *psi_base = nullptr;
+    float *psi_alt = nullptr;  // second psi buffer for the fused (ping-pong) step
     float *psi[2][3][2] = {};  // [E/H][slab axis][q]: component (axis + 1 + q) % 3
     long long psi_n[3] = {0, 0, 0};
 };
@@ -74,9 +86,12 @@ struct Cpml {
 // Sampled per-kernel timing of the E / H updates (HIP events on the grid's stream).
 struct KernelTiming {
     int every = 0;
-    long long launches[2] = {0, 0};
-    std::vector<hipEvent_t> ev[2];  // start/stop pairs, E and H
+    long long launches[3] = {0, 0, 0};
+    std::vector<hipEvent_t> ev[3];  // start/stop pairs: split E, split H, fused step
 };
+
+// Start a sampled timing bracket for kernel slot `which` (returns the stop event or null).
+hipEvent_t timing_begin(fdtd_dev *d, int which);
 
 struct BlockHist {
     const void *key;
@@ -98,6 +113,11 @@ struct fdtd_dev {
 
     float *field_base;  // one allocation, 6 component arrays of N floats
     float *F[6];
+    // second field buffer for the fused (ping-pong) step; alt_valid = its wall cells match
+    float *alt_base;
+    bool alt_valid;
+    int fused_rows;     // rows per lane of the fused step (0 = split E/H kernels)
+    int fused_kchunk;   // z planes per wavefront of the fused step
 
     // materials
     int nmat;
@@ -131,6 +151,8 @@ struct fdtd_dev {
 
     fdtd::Cpml cpml;
     void *run;  // device-resident run configuration (dev_run.hip), or null
+    int n_run_src;                              // run sources evaluated by the next step
+    fdtd::RunSrcArg run_src[fdtd::kMaxRunSrc];
     fdtd::KernelTiming ktime;
 
     hipEvent_t ev0, ev1;
@@ -148,6 +170,10 @@ int fail(const char *where, hipError_t e);
 
 int ensure_coefficients(fdtd_dev *d, double dt);
 int cpml_zero(fdtd_dev *d);
+int cpml_tables(fdtd_dev *d, double dt);
+int fused_step(fdtd_dev *d, double dt, int ty_rows, int kc_len);
+bool fused_supported(const fdtd_dev *d);
+int fused_sync_alt(fdtd_dev *d);
 // The device source plane for z = k (created zeroed on first use), or null with *err set.
 SrcPlane *plane_for(fdtd_dev *d, int k, int *err);
 void run_release(fdtd_dev *d);

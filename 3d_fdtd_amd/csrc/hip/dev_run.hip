@@ -36,12 +36,24 @@ struct RunDft {
     int freq_off;
 };
 
+// Per-monitor device descriptor for the merged overlap kernels.
+struct MonDev {
+    long long koff;   // plane offset k * P
+    const float *W;   // 8 x P weights
+    double2 *inst;
+    double2 *acc;
+    int nfreq;
+    int twoff;        // offset of this monitor's frequencies in a twiddle row
+};
+
 struct Run {
     std::vector<RunSource> src;
     std::vector<RunOverlap> ovl;
     std::vector<RunDft> dft;
-    double2 *partials = nullptr;  // overlap stage-1 partial sums
+    double2 *partials = nullptr;  // overlap stage-1 partial sums, [monitor][block]
     int partial_cap = 0;
+    MonDev *mons = nullptr;       // device copy of the monitor descriptors
+    int mon_cap = 0;
     double2 *tw = nullptr;        // [step][total freq] twiddles for the current run() call
     long long tw_cap = 0;
 };
@@ -59,16 +71,19 @@ __global__ void k_run_source(float *J, long long cs, const float *PR, const floa
     for (int m = 0; m < 6; m++) J[m * cs + p] = c.x * PR[m * P + p] + c.y * PI[m * P + p];
 }
 
-// Stage 1: per-block FP64 partial of Hx mEy - Hy mEx + Ey mHx - Ex mHy over plane k.
-__global__ __launch_bounds__(kBlock) void k_run_overlap_partial(const float *Ex, const float *Ey, const float *Hx,
-                                                                const float *Hy, long long P, long long koff,
-                                                                const float *W, double2 *partials) {
+// All overlap monitors in one launch, blockIdx.y selects the monitor: per-block FP64 partial of
+// Hx mEy - Hy mEx + Ey mHx - Ex mHy over the monitor plane (timefield3d.c:3195-3224).
+__global__ __launch_bounds__(kBlock) void k_run_overlap_partial_multi(const float *base, long long Nal, long long P,
+                                                                      const MonDev *mons, double2 *partials,
+                                                                      int nbp) {
     __shared__ double sre[kBlock / 64], sim[kBlock / 64];
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const MonDev M = mons[blockIdx.y];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double re = 0.0, im = 0.0;
     if (p < P) {
-        const long long c = p + koff;
-        const double ex = Ex[c], ey = Ey[c], hx = Hx[c], hy = Hy[c];
+        const long long c = p + M.koff;
+        const float *W = M.W;
+        const double ex = base[c], ey = base[Nal + c], hx = base[3 * Nal + c], hy = base[4 * Nal + c];
         const double eEx_r = W[0 * P + p], eEx_i = W[1 * P + p], eEy_r = W[2 * P + p], eEy_i = W[3 * P + p];
         const double hHx_r = W[4 * P + p], hHx_i = W[5 * P + p], hHy_r = W[6 * P + p], hHy_i = W[7 * P + p];
         re = (hx * eEy_r - hy * eEx_r) + (ey * hHx_r - ex * hHy_r);
@@ -90,19 +105,20 @@ __global__ __launch_bounds__(kBlock) void k_run_overlap_partial(const float *Ex,
             r += sre[q];
             s += sim[q];
         }
-        partials[blockIdx.x] = make_double2(r, s);
+        partials[(long long)blockIdx.y * nbp + blockIdx.x] = make_double2(r, s);
     }
 }
 
-// Stage 2: fixed-order sum of the partials, scale by -dx^2/2, then acc_f += inst * tw_f.
-__global__ __launch_bounds__(kBlock) void k_run_overlap_final(const double2 *partials, int nb, double scale,
-                                                              double2 *inst, double2 *acc, const double2 *tw,
-                                                              int nfreq) {
+// One block per monitor: fixed-order sum of its partials, scale, DFT accumulation.
+__global__ __launch_bounds__(kBlock) void k_run_overlap_final_multi(const double2 *partials, int nbp, double scale,
+                                                                    const MonDev *mons, const double2 *tw_row) {
     __shared__ double sre[kBlock], sim[kBlock];
+    const MonDev M = mons[blockIdx.x];
+    const double2 *pp = partials + (long long)blockIdx.x * nbp;
     double r = 0, s = 0;
-    for (int q = threadIdx.x; q < nb; q += kBlock) {
-        r += partials[q].x;
-        s += partials[q].y;
+    for (int q = threadIdx.x; q < nbp; q += kBlock) {
+        r += pp[q].x;
+        s += pp[q].y;
     }
     sre[threadIdx.x] = r;
     sim[threadIdx.x] = s;
@@ -115,12 +131,12 @@ __global__ __launch_bounds__(kBlock) void k_run_overlap_final(const double2 *par
         __syncthreads();
     }
     const double vr = scale * sre[0], vi = scale * sim[0];
-    if (threadIdx.x == 0) *inst = make_double2(vr, vi);
-    for (int f = threadIdx.x; f < nfreq; f += kBlock) {
-        double2 t = tw[f], a = acc[f];
+    if (threadIdx.x == 0) *M.inst = make_double2(vr, vi);
+    for (int f = threadIdx.x; f < M.nfreq; f += kBlock) {
+        double2 t = tw_row[M.twoff + f], a = M.acc[f];
         a.x += t.x * vr - t.y * vi;
         a.y += t.x * vi + t.y * vr;
-        acc[f] = a;
+        M.acc[f] = a;
     }
 }
 
@@ -166,6 +182,7 @@ void run_release(fdtd_dev *d) {
     }
     for (auto &q : r->dft) (void)hipFree(q.acc);
     (void)hipFree(r->partials);
+    (void)hipFree(r->mons);
     (void)hipFree(r->tw);
     delete r;
     d->run = nullptr;
@@ -348,13 +365,32 @@ int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
         FDTD_TRY(hipStreamSynchronize(d->stream));
     }
     const unsigned nbp = (unsigned)((d->P + kBlock - 1) / kBlock);
-    if (!r->ovl.empty() && (int)nbp > r->partial_cap) {
+    const int nmon = (int)r->ovl.size();
+    if (nmon > 0 && (long long)nbp * nmon > r->partial_cap) {
+        FDTD_TRY(hipStreamSynchronize(d->stream));
         (void)hipFree(r->partials);
         r->partials = nullptr;
-        FDTD_TRY(hipMalloc(&r->partials, (size_t)nbp * sizeof(double2)));
-        r->partial_cap = (int)nbp;
+        FDTD_TRY(hipMalloc(&r->partials, (size_t)nbp * nmon * sizeof(double2)));
+        r->partial_cap = (int)(nbp * nmon);
     }
-    // source planes resolved once per call
+    if (nmon > 0) {
+        std::vector<MonDev> md((size_t)nmon);
+        for (int m = 0; m < nmon; m++) {
+            const RunOverlap &o = r->ovl[m];
+            md[m] = MonDev{(long long)o.k * d->P, o.W, o.inst, o.acc, o.nfreq, o.freq_off};
+        }
+        if (nmon > r->mon_cap) {
+            FDTD_TRY(hipStreamSynchronize(d->stream));
+            (void)hipFree(r->mons);
+            r->mons = nullptr;
+            FDTD_TRY(hipMalloc(&r->mons, (size_t)nmon * sizeof(MonDev)));
+            r->mon_cap = nmon;
+        }
+        FDTD_TRY(hipMemcpyAsync(r->mons, md.data(), md.size() * sizeof(MonDev), hipMemcpyHostToDevice, d->stream));
+        FDTD_TRY(hipStreamSynchronize(d->stream));
+    }
+    // source planes resolved once per call; the first kMaxRunSrc sources are evaluated inside
+    // the Yee kernels, any further ones are written into their plane by a small kernel
     std::vector<SrcPlane *> planes;
     for (auto &s : r->src) {
         int err;
@@ -362,24 +398,36 @@ int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
         if (!sp) return err;
         planes.push_back(sp);
     }
+    auto coef_at = [](const RunSource &rs, int step) {
+        // outside its schedule a source is switched off (amplitude 0)
+        return step >= 0 && step < rs.nsteps ? rs.coef[step] : make_float2(0.0f, 0.0f);
+    };
     const double scale = -0.5 * d->disc * d->disc;
+    const int ninl = (int)r->src.size() < kMaxRunSrc ? (int)r->src.size() : kMaxRunSrc;
     for (int s = 0; s < n; s++) {
         const int step = n0 + s;
-        for (size_t q = 0; q < r->src.size(); q++) {
+        d->n_run_src = ninl;
+        for (int q = 0; q < ninl; q++) {
             const RunSource &rs = r->src[q];
-            // outside its schedule a source is switched off (amplitude 0)
-            const float2 c = step >= 0 && step < rs.nsteps ? rs.coef[step] : make_float2(0.0f, 0.0f);
+            const float2 c = coef_at(rs, step);
+            d->run_src[q] = RunSrcArg{rs.k, rs.PR, rs.PI, rs.win, c.x, c.y};
+        }
+        for (size_t q = ninl; q < r->src.size(); q++) {
+            const RunSource &rs = r->src[q];
             hipLaunchKernelGGL(k_run_source, dim3(nbp), dim3(kBlock), 0, d->stream, planes[q]->J, planes[q]->stride,
-                               rs.PR, rs.PI, rs.win, d->P, c);
+                               rs.PR, rs.PI, rs.win, d->P, coef_at(rs, step));
             FDTD_TRY(hipGetLastError());
         }
         int rc = fdtd_dev_step(d, dt, 0);
+        d->n_run_src = 0;
         if (rc) return rc;
-        for (auto &o : r->ovl) {
-            hipLaunchKernelGGL(k_run_overlap_partial, dim3(nbp), dim3(kBlock), 0, d->stream, d->F[EX], d->F[EY],
-                               d->F[HX], d->F[HY], d->P, (long long)o.k * d->P, o.W, r->partials);
-            hipLaunchKernelGGL(k_run_overlap_final, dim3(1), dim3(kBlock), 0, d->stream, r->partials, (int)nbp, scale,
-                               o.inst, o.acc, r->tw + (size_t)s * nf + o.freq_off, o.nfreq);
+        if (nmon > 0) {
+            hipLaunchKernelGGL(k_run_overlap_partial_multi, dim3(nbp, nmon), dim3(kBlock), 0, d->stream,
+                               (const float *)d->field_base, d->Nal, d->P, (const MonDev *)r->mons, r->partials,
+                               (int)nbp);
+            hipLaunchKernelGGL(k_run_overlap_final_multi, dim3(nmon), dim3(kBlock), 0, d->stream,
+                               (const double2 *)r->partials, (int)nbp, scale, (const MonDev *)r->mons,
+                               (const double2 *)(r->tw + (size_t)s * nf));
             FDTD_TRY(hipGetLastError());
         }
         for (auto &q : r->dft) {
@@ -387,6 +435,14 @@ int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
                                d->P, (long long)q.k * d->P, q.mask, q.nfreq, r->tw + (size_t)s * nf + q.freq_off);
             FDTD_TRY(hipGetLastError());
         }
+    }
+    // leave each inline source's plane holding the values of the last step, as a per-step
+    // write would have
+    for (int q = 0; q < ninl; q++) {
+        const RunSource &rs = r->src[q];
+        hipLaunchKernelGGL(k_run_source, dim3(nbp), dim3(kBlock), 0, d->stream, planes[q]->J, planes[q]->stride,
+                           rs.PR, rs.PI, rs.win, d->P, coef_at(rs, n0 + n - 1));
+        FDTD_TRY(hipGetLastError());
     }
     return 0;
 }

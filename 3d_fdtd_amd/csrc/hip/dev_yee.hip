@@ -28,7 +28,13 @@ struct YeeArgs {
     SrcPlaneArg src[kMaxSrcPlanes];
     const float *dense;  // 6 x N source volume when sources span more than kMaxSrcPlanes planes
     long long N;
+    int nrs;
+    RunSrcArg rs[kMaxRunSrc];
 };
+
+// Source components m0..m0+2 of plane cell p at plane k: tabulated run source inside its mode
+// window, else the stored source plane.  Returns false when plane k carries no source.
+__device__ __forceinline__ bool source3(const YeeArgs &a, int k, long long p, int m0, float v[3]);
 
 struct PmlArgs {
     const float *tab[3];  // per axis: bE, aE, kE, bH, aH, kH (N_axis floats each)
@@ -53,6 +59,21 @@ __device__ __forceinline__ const float *src_plane(const YeeArgs &a, int k, long 
     for (int s = 0; s < a.nsrc; s++)
         if (a.src[s].k == k) J = a.src[s].J;
     return J;
+}
+
+__device__ __forceinline__ bool source3(const YeeArgs &a, int k, long long p, int m0, float v[3]) {
+    for (int q = 0; q < a.nrs; q++) {
+        const RunSrcArg &r = a.rs[q];
+        if (r.k == k && r.win[p]) {
+            for (int m = 0; m < 3; m++) v[m] = r.c1 * r.PR[(m0 + m) * a.P + p] + r.c2 * r.PI[(m0 + m) * a.P + p];
+            return true;
+        }
+    }
+    long long cs;
+    const float *J = src_plane(a, k, cs);
+    if (!J) return false;
+    for (int m = 0; m < 3; m++) v[m] = J[(m0 + m) * cs + p];
+    return true;
 }
 
 __device__ __forceinline__ int slab_index(int n, int N, int nlo, int nhi) {
@@ -124,12 +145,11 @@ __global__ __launch_bounds__(kBlock) void k_update_E(YeeArgs a, PmlArgs pm) {
         float ex = cf[0] * a.Ex[c] + cf[3] * (x1 - x2);
         float ey = cf[1] * a.Ey[c] + cf[4] * (y1 - y2);
         float ez = cf[2] * a.Ez[c] + cf[5] * (z1 - z2);
-        long long cs;
-        const float *J = src_plane(a, k, cs);
-        if (J) {
-            ex -= cf[6] * J[p];
-            ey -= cf[7] * J[cs + p];
-            ez -= cf[8] * J[2 * cs + p];
+        float sv[3];
+        if (source3(a, k, p, 0, sv)) {
+            ex -= cf[6] * sv[0];
+            ey -= cf[7] * sv[1];
+            ez -= cf[8] * sv[2];
         }
         a.Ex[c] = ex;
         a.Ey[c] = ey;
@@ -196,12 +216,11 @@ __global__ __launch_bounds__(kBlock) void k_update_H(YeeArgs a, PmlArgs pm) {
         float hx = cf[9] * a.Hx[c] - cf[12] * (x1 - x2);
         float hy = cf[10] * a.Hy[c] - cf[13] * (y1 - y2);
         float hz = cf[11] * a.Hz[c] - cf[14] * (z1 - z2);
-        long long cs;
-        const float *J = src_plane(a, k, cs);
-        if (J) {
-            hx -= cf[15] * J[3 * cs + p];
-            hy -= cf[16] * J[4 * cs + p];
-            hz -= cf[17] * J[5 * cs + p];
+        float sv[3];
+        if (source3(a, k, p, 3, sv)) {
+            hx -= cf[15] * sv[0];
+            hy -= cf[16] * sv[1];
+            hz -= cf[17] * sv[2];
         }
         a.Hx[c] = hx;
         a.Hy[c] = hy;
@@ -335,7 +354,20 @@ void cpml_profile(int N, int nlo, int nhi, const Cpml &c, double disc, double dt
     }
 }
 
-int ensure_cpml_tables(fdtd_dev *d, double dt) {
+}  // namespace
+
+hipEvent_t timing_begin(fdtd_dev *d, int which) {
+    KernelTiming &kt = d->ktime;
+    if (kt.every <= 0 || (kt.launches[which]++ % kt.every) != 0) return nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    if (hipEventCreate(&ev_start) != hipSuccess || hipEventCreate(&ev_stop) != hipSuccess) return nullptr;
+    if (hipEventRecord(ev_start, d->stream) != hipSuccess) return nullptr;
+    kt.ev[which].push_back(ev_start);
+    kt.ev[which].push_back(ev_stop);
+    return ev_stop;
+}
+
+int cpml_tables(fdtd_dev *d, double dt) {
     Cpml &c = d->cpml;
     if (c.tab_dt == dt) return 0;
     const int N[3] = {d->I, d->J, d->K};
@@ -346,6 +378,8 @@ int ensure_cpml_tables(fdtd_dev *d, double dt) {
     c.tab_dt = dt;
     return 0;
 }
+
+namespace {
 
 PmlArgs pml_args(fdtd_dev *d) {
     PmlArgs pm{};
@@ -366,7 +400,7 @@ PmlArgs pml_args(fdtd_dev *d) {
 int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_field, bool pml) {
     int rc = ensure_coefficients(d, dt);
     if (rc) return rc;
-    if (pml && (rc = ensure_cpml_tables(d, dt))) return rc;
+    if (pml && (rc = cpml_tables(d, dt))) return rc;
     if (scheme != 0 && scheme != 1) {
         set_error("scheme must be 0 (semi-implicit) or 1 (explicit)");
         return -2;
@@ -405,20 +439,12 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
     }
     a.dense = d->dense_src;
     a.N = d->N;
+    a.nrs = d->n_run_src;
+    for (int q = 0; q < d->n_run_src; q++) a.rs[q] = d->run_src[q];
     PmlArgs pm = pml_args(d);
     dim3 grid((unsigned)(a.plane_blocks * zchunks));
     const bool u8 = d->mat_bytes == 1;
-    KernelTiming &kt = d->ktime;
-    const int which = e_field ? 0 : 1;
-    hipEvent_t ev_stop = nullptr;
-    if (kt.every > 0 && (kt.launches[which]++ % kt.every) == 0) {
-        hipEvent_t ev_start;
-        FDTD_TRY(hipEventCreate(&ev_start));
-        FDTD_TRY(hipEventCreate(&ev_stop));
-        FDTD_TRY(hipEventRecord(ev_start, d->stream));
-        kt.ev[which].push_back(ev_start);
-        kt.ev[which].push_back(ev_stop);
-    }
+    hipEvent_t ev_stop = timing_begin(d, e_field ? 0 : 1);
 #define FDTD_LAUNCH(KER)                                                                                  \
     do {                                                                                                  \
         if (u8 && pml) hipLaunchKernelGGL((KER<uint8_t, true>), grid, dim3(kBlock), 0, d->stream, a, pm);   \
@@ -437,6 +463,7 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
 int launch_mur(fdtd_dev *d, double dt, int k0, int k1, int field) {
     int rc = ensure_coefficients(d, dt);
     if (rc) return rc;
+    d->alt_valid = false;  // wall cells change: the fused step's second buffer must be resynced
     k0 = k0 < 0 ? 0 : k0;
     k1 = k1 > d->K ? d->K : k1;
     FaceArgs fa = face_args(d, 0);
@@ -469,6 +496,7 @@ int cpml_zero(fdtd_dev *d) {
     Cpml &c = d->cpml;
     long long n = c.psi_n[0] + c.psi_n[1] + c.psi_n[2];
     if (c.psi_base && n > 0) FDTD_TRY(hipMemsetAsync(c.psi_base, 0, (size_t)4 * n * sizeof(float), d->stream));
+    if (c.psi_alt && n > 0) FDTD_TRY(hipMemsetAsync(c.psi_alt, 0, (size_t)4 * n * sizeof(float), d->stream));
     return 0;
 }
 
@@ -547,6 +575,7 @@ int fdtd_dev_mur_H(fdtd_dev *d, double dt, int k0, int k1) { return launch_mur(d
 int fdtd_dev_step(fdtd_dev *d, double dt, int scheme) {
     int rc;
     if (d->cpml.on && scheme == 0) {
+        if (d->fused_rows > 0 && fused_supported(d)) return fused_step(d, dt, d->fused_rows, d->fused_kchunk);
         if ((rc = fdtd_dev_update_E_cpml(d, dt))) return rc;
         return fdtd_dev_update_H_cpml(d, dt);
     }
@@ -606,33 +635,32 @@ int fdtd_dev_cpml_on(fdtd_dev *d, const int npml[6], int order, double R0, doubl
 }
 
 int fdtd_dev_kernel_timing(fdtd_dev *d, int every) {
-    int cnt;
-    double e, h;
-    int rc = fdtd_dev_kernel_times(d, &cnt, &e, &h);  // drop any pending sample
+    int cnt[3];
+    double ms[3];
+    int rc = fdtd_dev_kernel_times(d, cnt, ms);  // drop any pending sample
     if (rc) return rc;
     d->ktime.every = every < 0 ? 0 : every;
-    d->ktime.launches[0] = d->ktime.launches[1] = 0;
+    for (int w = 0; w < 3; w++) d->ktime.launches[w] = 0;
     return 0;
 }
 
-int fdtd_dev_kernel_times(fdtd_dev *d, int *count, double *e_ms, double *h_ms) {
+int fdtd_dev_kernel_times(fdtd_dev *d, int counts[3], double ms_out[3]) {
     KernelTiming &kt = d->ktime;
     FDTD_TRY(hipStreamSynchronize(d->stream));
-    double sum[2] = {0, 0};
-    int n[2] = {0, 0};
-    for (int w = 0; w < 2; w++) {
+    for (int w = 0; w < 3; w++) {
+        double sum = 0;
+        int n = 0;
         for (size_t q = 0; q + 1 < kt.ev[w].size(); q += 2) {
             float ms = 0;
             FDTD_TRY(hipEventElapsedTime(&ms, kt.ev[w][q], kt.ev[w][q + 1]));
-            sum[w] += ms;
-            n[w]++;
+            sum += ms;
+            n++;
         }
         for (hipEvent_t e : kt.ev[w]) (void)hipEventDestroy(e);
         kt.ev[w].clear();
+        counts[w] = n;
+        ms_out[w] = sum;
     }
-    *count = n[0] < n[1] ? n[0] : n[1];
-    *e_ms = sum[0];
-    *h_ms = sum[1];
     return 0;
 }
 
@@ -641,6 +669,7 @@ int fdtd_dev_cpml_off(fdtd_dev *d) {
     if (c.tab || c.psi_base) FDTD_TRY(hipStreamSynchronize(d->stream));
     (void)hipFree(c.tab);
     (void)hipFree(c.psi_base);
+    (void)hipFree(c.psi_alt);
     c = Cpml();
     return 0;
 }

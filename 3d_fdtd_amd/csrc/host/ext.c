@@ -52,8 +52,12 @@ int fdtd3d_ext_synchronize(timefield3d psi) { return fdtd_dev_synchronize(tf_dev
 
 int fdtd3d_ext_kernel_timing(timefield3d psi, int every) { return fdtd_dev_kernel_timing(tf_dev(psi), every); }
 
-int fdtd3d_ext_kernel_times(timefield3d psi, int *count, double *e_ms, double *h_ms) {
-    return fdtd_dev_kernel_times(tf_dev(psi), count, e_ms, h_ms);
+int fdtd3d_ext_kernel_times(timefield3d psi, int counts[3], double ms[3]) {
+    return fdtd_dev_kernel_times(tf_dev(psi), counts, ms);
+}
+
+int fdtd3d_ext_set_step_kernel(timefield3d psi, int rows, int kchunk) {
+    return fdtd_dev_set_step_kernel(tf_dev(psi), rows, kchunk);
 }
 
 int fdtd3d_ext_steps(timefield3d psi, double dt, int n) {

@@ -56,6 +56,10 @@ int fdtd_dev_cpml_on(fdtd_dev *d, const int npml[6], int order, double R0, doubl
 int fdtd_dev_cpml_off(fdtd_dev *d);
 int fdtd_dev_update_E_cpml(fdtd_dev *d, double dt);
 int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt);
+/* Kernel used by fdtd_dev_step under the PML boundary: rows = 0 selects the split E / H pair,
+ * rows = 2, 4 or 8 the fused single-kernel step with that many y rows per lane and kchunk z
+ * planes per wavefront (default 4 and 16). */
+int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk);
 
 /* Device-resident run (extension).  A run holds mode sources whose per-step amplitude and
  * phase are tabulated up front, FP64 mode-overlap monitors with a running DFT at nfreq angular
@@ -103,11 +107,12 @@ int fdtd_dev_block_update_gradient(fdtd_dev *d, const void *key, const int box[6
 int fdtd_dev_block_reset(const void *key, int which);
 void fdtd_dev_block_release(const void *key);
 
-/* Per-kernel timing of the Yee updates: while enabled, every `every`-th E and H launch is
- * bracketed by HIP events on the grid's stream; kernel_times returns the sampled launch count
- * and the summed durations (ms) of the E and H kernels, and resets the sample. */
+/* Per-kernel timing of the Yee updates: while enabled, every `every`-th launch of each update
+ * kernel is bracketed by HIP events on the grid's stream; kernel_times returns, for the split E
+ * kernel, the split H kernel and the fused step kernel (slots 0, 1, 2), the number of sampled
+ * launches and their summed durations in ms, and resets the sample. */
 int fdtd_dev_kernel_timing(fdtd_dev *d, int every);
-int fdtd_dev_kernel_times(fdtd_dev *d, int *count, double *e_ms, double *h_ms);
+int fdtd_dev_kernel_times(fdtd_dev *d, int counts[3], double ms[3]);
 
 /* Timing of the work enqueued on the grid's stream (HIP events on that stream). */
 int fdtd_dev_timer_start(fdtd_dev *d);

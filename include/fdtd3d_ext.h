@@ -25,10 +25,14 @@ int fdtd3d_ext_upload_source_plane(timefield3d psi, int k);
 int fdtd3d_ext_timer_start(timefield3d psi);
 int fdtd3d_ext_timer_stop(timefield3d psi, double *elapsed_ms);
 int fdtd3d_ext_synchronize(timefield3d psi);
-/* Sample every `every`-th E and H update launch with HIP events (0 switches sampling off);
- * kernel_times returns the number of sampled steps and the summed E / H kernel times (ms). */
+/* Sample every `every`-th launch of each Yee update kernel with HIP events (0 switches
+ * sampling off); kernel_times returns per slot (0 split E, 1 split H, 2 fused step) the number
+ * of sampled launches and their summed durations in ms. */
 int fdtd3d_ext_kernel_timing(timefield3d psi, int every);
-int fdtd3d_ext_kernel_times(timefield3d psi, int *count, double *e_ms, double *h_ms);
+int fdtd3d_ext_kernel_times(timefield3d psi, int counts[3], double ms[3]);
+/* Under the PML boundary: rows = 0 selects the split E/H kernels, rows = 2, 4 or 8 the fused
+ * one-kernel step (rows per lane, kchunk z planes per wavefront; default 4, 16). */
+int fdtd3d_ext_set_step_kernel(timefield3d psi, int rows, int kchunk);
 
 /* Run n reference steps (timefield3d_apply_timestep_omp_full semantics) without host work in
  * between; sources stay as last set. */

@@ -44,6 +44,12 @@ class Taper(C.Structure):
                 ("right_grad", C.POINTER(C.c_double))]
 
 
+class Block(C.Structure):
+    _fields_ = [("t", C.c_int), ("x_start", C.c_int), ("x_end", C.c_int), ("y_start", C.c_int),
+                ("y_end", C.c_int), ("z_start", C.c_int), ("z_end", C.c_int),
+                ("forward_fields", C.POINTER(C.c_double)), ("gradient", C.POINTER(C.c_double))]
+
+
 def available():
     return os.path.exists(REF_LIB)
 
@@ -85,6 +91,11 @@ def lib():
         "complex_exp": ([Complex], Complex),
         "complex_angle": ([Complex], d),
         "complex_sqrt": ([Complex], Complex),
+        "ajoint3d_new": ([i, i, i, i, i, i, i], Block),
+        "adjoint3d_store_fwd_fields": ([Block, TimeField, i], None),
+        "adjoint3d_update_gradient": ([Block, TimeField, i], None),
+        "adjoint3d_adjust_permittivity": ([Block, TimeField, d, d, d, d], None),
+        "adjoint3d_taper_calculate_gradients": ([Taper, Block, d], None),
         "adjoint3d_taper_new": ([d, d, i, i, i, C.POINTER(i), C.POINTER(i), i, i, i], Taper),
         "adjoint3d_taper_insert": ([TimeField, Taper], None),
         "adjoint3d_taper_store_forward_fields": ([Taper, TimeField, i], None),

@@ -254,6 +254,60 @@ def f6(L, manifest):
     L.timefield3d_free(tf2)
 
 
+def f7(L, manifest):
+    """F7: the explicit-conductivity serial step (timefield3d_apply_timestep), and the 3-D block
+    adjoint (store, reverse-time gradient accumulation, permittivity adjustment, taper projection)."""
+    rng = np.random.default_rng(707)
+    I, J, K = 14, 12, 16
+    tf, A = new_tf(L, I, J, K)
+    A["ep"][:] = rng.uniform(1.0, 12.1, A["ep"].size).astype(np.float32)
+    A["mu"][:] = rng.uniform(1.0, 2.0, A["mu"].size).astype(np.float32)
+    A["sig"][:] = rng.uniform(0.0, 3.0, A["sig"].size).astype(np.float32)
+    A["sources"][:] = rng.uniform(-1.0, 1.0, A["sources"].size).astype(np.float32)
+    f0 = rng.uniform(-1.0, 1.0, A["fields"].size).astype(np.float32)
+    A["fields"][:] = f0
+    ex_in = {k: A[k].copy() for k in ("ep", "mu", "sig", "sources")}
+    for _ in range(20):
+        L.timefield3d_apply_timestep(tf, DT)
+    ex_out = A["fields"].copy()
+    L.timefield3d_free(tf)
+    # block adjoint
+    BI, BJ, BK, T = 20, 16, 18, 6
+    box = (4, 12, 3, 9, 5, 14)
+    tf, A = new_tf(L, BI, BJ, BK)
+    blk = L.ajoint3d_new(*box, T)
+    fwd = pattern_fields(T, A["fields"].size, 7, 13)
+    adj = pattern_fields(T, A["fields"].size, 11, 5)
+    for t in range(T):
+        A["fields"][:] = fwd[t]
+        L.adjoint3d_store_fwd_fields(blk, tf, t)
+    for t in range(T - 1, -1, -1):
+        A["fields"][:] = adj[t]
+        L.adjoint3d_update_gradient(blk, tf, t)
+    nb = (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4])
+    grad = np.ctypeslib.as_array(blk.gradient, shape=(nb,)).copy()
+    L.timefield3d_set_background(tf, 2.0, 1.0)
+    L.adjoint3d_adjust_permittivity(blk, tf, 4.0, 0.05, 1.5, 12.0)
+    ep_after = A["ep"].copy()
+    # projection of the block gradient onto a taper inside the box
+    left = np.array([6, 6, 5, 5], np.int32)
+    right = np.array([9, 9, 10, 10], np.int32)
+    tp = L.adjoint3d_taper_new(12.0, 1.0, 3, 5, 14, left.ctypes.data_as(C.POINTER(C.c_int)),
+                               right.ctypes.data_as(C.POINTER(C.c_int)), 3, 9, 1)
+    L.adjoint3d_taper_calculate_gradients(tp, blk, 10.0)
+    tlg = np.ctypeslib.as_array(tp.left_grad, shape=(4,)).copy()
+    trg = np.ctypeslib.as_array(tp.right_grad, shape=(4,)).copy()
+    save("F7_explicit_block.npz", dims=np.array([I, J, K]), dt=DT, disc=DISC, steps=20, fields0=f0, fields=ex_out,
+         block_dims=np.array([BI, BJ, BK]), box=np.array(box), T=T, grad=grad, ep_after=ep_after,
+         taper_left=left, taper_right=right, taper_left_grad=tlg, taper_right_grad=trg, **ex_in)
+    manifest["F7_explicit_block"] = {
+        "explicit": "14x12x16 random eps/mu/sigma/sources/fields, 20 x timefield3d_apply_timestep",
+        "block": "20x16x18 grid, box (4,12,3,9,5,14), T=6, fields pattern_fields(7,13) forward, (11,5) reverse; "
+                 "adjust_permittivity(omega 4, improvement 0.05, [1.5, 12]) on background 2; taper seg 3 "
+                 "z 5..14 y 3..9 left [6,6,5,5] right [9,9,10,10] calculate_gradients(ep_diff 10)"}
+    L.timefield3d_free(tf)
+
+
 def f9(L, manifest):
     """F9: complex_exp (Taylor series) on the imaginary axis and off it, complex_angle, complex_sqrt."""
     rng = np.random.default_rng(909)
@@ -279,6 +333,7 @@ def main():
     f4(L, manifest)
     f5(L, manifest)
     f6(L, manifest)
+    f7(L, manifest)
     f9(L, manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, sort_keys=True)

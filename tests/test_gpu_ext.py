@@ -66,6 +66,53 @@ def test_cpml_step_vs_oracle(npml):
     g.free()
 
 
+@pytest.mark.parametrize("rows,kchunk", [(2, 3), (4, 16), (8, 5)])
+def test_fused_step_vs_split_and_oracle(rows, kchunk):
+    """Fused one-kernel step (ping-pong buffers) vs the split E/H kernels and the FP64 oracle, on
+    an odd-sized grid (partial strips, tiles and chunks) with conductivity, a plane source and
+    PML on some faces; switches kernels mid-run to exercise the buffer resynchronisation."""
+    rng = np.random.default_rng(rows * 100 + kchunk)
+    I, J, K, disc = 70, 23, 29, 0.02
+    dt = 0.99 * disc / np.sqrt(3)
+    npml = (5, 4, 0, 3, 6, 0)
+    say("fused case rows", rows, "kchunk", kchunk)
+    ep = rng.uniform(1, 12, 3 * I * J * K).astype(np.float32)
+    sig = np.zeros(6 * I * J * K, np.float32)
+    sig[1::6] = rng.uniform(0, 2, I * J * K).astype(np.float32)
+    src = np.zeros(6 * I * J * K, np.float32)
+    src.reshape(K, J, I, 6)[11] = rng.uniform(-1, 1, (J, I, 6)).astype(np.float32)
+    f0 = rng.uniform(-1, 1, 6 * I * J * K).astype(np.float32)
+    grids = []
+    for mode in ("fused", "split", "mixed"):
+        g = fd.Grid(I, J, K, disc)
+        for a_, v in (("ep", ep), ("sig", sig), ("sources", src), ("fields", f0)):
+            g.a[a_][:] = v
+        g.push()
+        fd.check(L().fdtd3d_ext_set_cpml(g.tf, (C.c_int * 6)(*npml), 3, 1e-6, 1.5, 0.1), "set_cpml")
+        fd.check(L().fdtd3d_ext_set_step_kernel(g.tf, 0 if mode == "split" else rows, kchunk), "kernel")
+        for s in range(30):
+            if mode == "mixed" and s in (10, 20):
+                fd.check(L().fdtd3d_ext_set_step_kernel(g.tf, 0 if s == 10 else rows, kchunk), "kernel")
+            L().timefield3d_apply_timestep_omp_full(g.tf, dt)
+        grids.append(g.fields().copy())
+        g.free()
+    o = O.Grid(I, J, K, disc)
+    o.ep[:] = ep
+    o.sig[:] = sig
+    o.src[:] = src
+    o.f[:] = f0
+    cp = O.Cpml(o, npml, dt, order=3, R0=1e-6, kappa_max=1.5, alpha_max=0.1)
+    for _ in range(30):
+        O.step_cpml(o, cp, dt)
+    for name, got in zip(("fused", "split", "mixed"), grids):
+        err = rel_l2(got, o.f)
+        say(name, "rel L2 vs oracle", err)
+        assert err <= REL
+    d = rel_l2(grids[0], grids[1])
+    say("fused vs split", d)
+    assert d <= 1e-6
+
+
 def test_run_api_matches_reference_monitors(golden_dir):
     """Device-resident run (tabulated source, overlap monitors with DFT) vs the reference's
     per-step source/overlap calls recorded in fixture F4 (single frequency)."""

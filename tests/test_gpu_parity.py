@@ -63,8 +63,8 @@ def test_step_random_vs_reference(golden_dir, name):
     g.free()
 
 
-def test_explicit_and_split_entry_points_match_oracle():
-    """apply_timestep (explicit sigma) and the zsection/abc entry points, vs the oracle's pieces."""
+def test_zsection_entry_points_match_oracle():
+    """The _zsection E/H updates and Mur faces driven slab by slab, vs the oracle's full step."""
     rng = np.random.default_rng(7)
     I, J, K, disc = 18, 14, 20, 0.02
     dt = 0.99 * disc / np.sqrt(3)
@@ -78,14 +78,15 @@ def test_explicit_and_split_entry_points_match_oracle():
     g.a["fields"][:] = f0
     o.f[:] = f0
     g.push()
-    # reference-mode step assembled from the zsection entry points over two slabs
+    # reference-mode step assembled from the zsection entry points over two slabs, with the
+    # one-plane widening of the reference's OpenMP driver (timefield3d.c:2794-2808)
     for _ in range(10):
         L().timefield3d_update_old_edge_fields(g.tf)
-        for lo, hi in ((0, 10), (9, 21)):
+        for lo, hi in ((0, 11), (9, K)):
             L().timefield3d_apply_E_timestep_zsection(g.tf, dt, lo, hi)
         L().timefield3d_apply_E_first_order_abc_zsection(g.tf, dt, 0, 10)
         L().timefield3d_apply_E_first_order_abc_zsection(g.tf, dt, 10, K)
-        for lo, hi in ((0, 10), (9, 21)):
+        for lo, hi in ((0, 11), (9, K)):
             L().timefield3d_apply_H_timestep_zsection(g.tf, dt, lo, hi)
         L().timefield3d_apply_H_first_order_abc_zsection(g.tf, dt, 0, 10)
         L().timefield3d_apply_H_first_order_abc_zsection(g.tf, dt, 10, K)
@@ -93,6 +94,57 @@ def test_explicit_and_split_entry_points_match_oracle():
     err = rel_l2(g.fields(), o.f)
     say("zsection assembly rel L2", err)
     assert err <= REL_L2
+    g.free()
+
+
+def test_explicit_step_vs_reference(golden_dir):
+    """F7: timefield3d_apply_timestep (explicit conductivity, timefield3d.c:1537-1677)."""
+    z = load(golden_dir, "F7_explicit_block.npz")
+    I, J, K = (int(v) for v in z["dims"])
+    g = grid_from(z, I, J, K)
+    g.a["sig"][:] = z["sig"]
+    g.a["sources"][:] = z["sources"]
+    g.a["fields"][:] = z["fields0"]
+    g.push()
+    for _ in range(int(z["steps"])):
+        L().timefield3d_apply_timestep(g.tf, float(z["dt"]))
+    err = rel_l2(g.fields(), z["fields"])
+    say("explicit step rel L2", err)
+    assert err <= REL_L2
+    g.free()
+
+
+def test_block_adjoint_vs_reference(golden_dir):
+    """F7: 3-D block adjoint store / reverse accumulation / permittivity step / taper projection.
+    The stored values are exact FP32 numbers, so the FP64 device accumulation is bit-exact."""
+    z = load(golden_dir, "F7_explicit_block.npz")
+    BI, BJ, BK = (int(v) for v in z["block_dims"])
+    box, T = [int(v) for v in z["box"]], int(z["T"])
+    g = fd.Grid(BI, BJ, BK, 0.02)
+    blk = L().ajoint3d_new(*box, T)
+    fwd = pattern_fields(T, g.a["fields"].size, 7, 13)
+    adj = pattern_fields(T, g.a["fields"].size, 11, 5)
+    for t in range(T):
+        g.a["fields"][:] = fwd[t]
+        g.push()
+        L().adjoint3d_store_fwd_fields(blk, g.tf, t)
+    for t in range(T - 1, -1, -1):
+        g.a["fields"][:] = adj[t]
+        g.push()
+        L().adjoint3d_update_gradient(blk, g.tf, t)
+    nb = (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4])
+    grad = np.ctypeslib.as_array(blk.gradient, shape=(nb,))
+    assert np.array_equal(grad, z["grad"])
+    L().timefield3d_set_background(g.tf, 2.0, 1.0)
+    L().adjoint3d_adjust_permittivity(blk, g.tf, 4.0, 0.05, 1.5, 12.0)
+    assert np.array_equal(g.a["ep"], z["ep_after"])
+    left = np.ascontiguousarray(z["taper_left"].astype(np.int32))
+    right = np.ascontiguousarray(z["taper_right"].astype(np.int32))
+    ip = C.POINTER(C.c_int)
+    tp = L().adjoint3d_taper_new(12.0, 1.0, 3, 5, 14, left.ctypes.data_as(ip), right.ctypes.data_as(ip), 3, 9, 1)
+    L().adjoint3d_taper_calculate_gradients(tp, blk, 10.0)
+    assert np.array_equal(np.ctypeslib.as_array(tp.left_grad, shape=(4,)), z["taper_left_grad"])
+    assert np.array_equal(np.ctypeslib.as_array(tp.right_grad, shape=(4,)), z["taper_right_grad"])
     g.free()
 
 
