@@ -188,8 +188,10 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->alt_base = nullptr;
     d->alt_valid = false;
     d->n_run_src = 0;
+    d->n_run_mon = 0;
     d->fused_rows = 0;  // split E/H kernels: measured faster than the fused step on MI355X
     d->fused_kchunk = 8;
+    d->split_kchunk = 0;
     hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     // fields: 6 arrays, each N floats, each starting on a 256-byte boundary
     d->Nal = (d->N + 63) / 64 * 64;
@@ -436,8 +438,9 @@ int fdtd_dev_set_fourier(fdtd_dev *d, const double *in) {
 }
 
 int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk) {
-    if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8) {
-        set_error("fused step rows per lane must be 0 (split kernels), 1, 2, 4 or 8");
+    if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16 && rows != 32) {
+        set_error("step kernel rows must be 0 (split kernels), 1, 2, 4, 8 (fused register strips) or 16, 32 "
+                  "(fused LDS tiles of 8 / 16 rows)");
         return -2;
     }
     if (rows != 0 && kchunk < 1) {
@@ -445,7 +448,11 @@ int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk) {
         return -2;
     }
     d->fused_rows = rows;
-    d->fused_kchunk = kchunk;
+    if (rows == 0) {
+        d->split_kchunk = kchunk > 0 ? kchunk : 0;
+    } else {
+        d->fused_kchunk = kchunk;
+    }
     return 0;
 }
 

@@ -150,7 +150,8 @@ __global__ __launch_bounds__(256) void k_step_fused(FusedArgs a) {
         }
     }
 
-    // software pipeline: the inputs of plane k+1 are in flight while plane k is computed
+    // inputs of the current plane (loaded at the top of each iteration: the many resident
+    // wavefronts hide the latency better than a register-hungry prefetch of plane k+1)
     float nhx[TY + 2], nhy[TY + 2], nhz[TY + 2], nex[TY + 2], ney[TY + 2], nez[TY + 2];
     int nmt[TY + 2];
     auto load_plane = [&](int kk) {
@@ -169,9 +170,9 @@ __global__ __launch_bounds__(256) void k_step_fused(FusedArgs a) {
             }
         }
     };
-    load_plane(k0);
     float eox[TY + 2], eoy[TY + 2], eoz[TY + 2];
     for (int k = k0; k <= k1; k++) {
+        load_plane(k);
         const int base = k * a.P + ic;
         const bool plane_int = k <= a.K - 2;
         const bool plane_own = k < k1;
@@ -188,7 +189,6 @@ __global__ __launch_bounds__(256) void k_step_fused(FusedArgs a) {
                 mcur[r] = nmt[r];
             }
         }
-        if (k < k1) load_plane(k + 1);
         const float *Jp = k == a.sk[0] ? a.sJ[0] : (k == a.sk[1] ? a.sJ[1] : nullptr);
         int sz = -1;
         float bzE = 0, azE = 0, kzE = 1;
@@ -333,6 +333,229 @@ __global__ __launch_bounds__(256) void k_step_fused(FusedArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// LDS-tiled fused step.  A 256-thread block owns a tile of kTileOwn x-columns by TJ y-rows and
+// marches a z chunk.  Per plane k: the block gathers H_old(k) over the tile plus its i-1 / j-1
+// halo into LDS (one coalesced pass, every value read once), computes E_new(k) over the tile plus
+// one halo column and row (the halo is what the H update needs from the neighbours), keeps
+// E_new(k) in LDS and finishes H_new(k-1) from E_new(k-1), E_new(k) and H_old(k-1).  Lanes are
+// columns; wave w handles rows w, w+4, ... .  Halo E values are recomputed from the same old
+// inputs with the same arithmetic as their owner's, and fields / PML memories are ping-ponged, so
+// halo work never races with an owner's store.
+constexpr int kTileW = 64;    // E columns of a tile (lanes)
+constexpr int kTileOwn = 63;  // owned columns (lane 63 is the i+1 halo)
+
+template <int TJ>
+struct TileGeom {
+    static constexpr int HR = TJ + 2;           // H rows j0-1 .. j0+TJ
+    static constexpr int HC = kTileW + 1;       // H columns i0-1 .. i0+63
+    static constexpr int NH = 3 * HR * HC;      // gathered H floats per plane
+    static constexpr int NHL = (NH + 255) / 256;
+    static constexpr int ER = TJ + 1;           // E rows j0 .. j0+TJ
+    static constexpr int NE = (ER + 3) / 4;     // E rows per thread
+};
+
+template <typename MatT, int TJ, bool PML>
+__global__ __launch_bounds__(256) void k_step_tiled(FusedArgs a) {
+    using G = TileGeom<TJ>;
+    // H_old of planes k-2, k-1, k (ring of three, [comp][row][col]); E_new of planes k-1, k
+    __shared__ float Hs[3][G::NH];
+    __shared__ float Es[2][3][G::ER][kTileW];
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int tx = lb % a.nwx, rest = lb / a.nwx;
+    const int ty = rest % a.nty, kc = rest / a.nty;
+    if (kc >= a.nkc) return;  // uniform over the block
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = 1 + kTileOwn * tx, j0 = 1 + TJ * ty;
+    const int kb = 1 + a.kc_len * kc, ke = min(kb + a.kc_len, a.K - 1);  // owned planes [kb, ke)
+    const int i = i0 + lane;
+    const int ic = i <= a.I - 1 ? i : a.I - 1;
+    const bool col_int = i <= a.I - 2;
+    const bool col_own = lane < kTileOwn && col_int;
+    const int lane_ip = lane < kTileW - 1 ? lane + 1 : lane;  // i+1 neighbour (lane 63 never owns H)
+    const int nal = a.nal;
+    const float *__restrict__ S = a.src;
+    float *__restrict__ D = a.dst;
+
+    // plane-relative source offsets of this thread's share of the H gather
+    int hoff[G::NHL];
+#pragma unroll
+    for (int t = 0; t < G::NHL; t++) {
+        const int q0 = threadIdx.x + 256 * t;
+        const int q = q0 < G::NH ? q0 : G::NH - 1;
+        const int comp = q / (G::HR * G::HC), rem = q - comp * (G::HR * G::HC);
+        const int rr = rem / G::HC, cc = rem - rr * G::HC;
+        const int gi = min(i0 - 1 + cc, a.I - 1), gj = min(j0 - 1 + rr, a.J - 1);
+        hoff[t] = (3 + comp) * nal + gi + a.I * gj;
+    }
+    auto gather = [&](int kk, float *dst) {
+        const int pk = kk * a.P;
+        float hv[G::NHL];
+#pragma unroll
+        for (int t = 0; t < G::NHL; t++) hv[t] = S[hoff[t] + pk];
+#pragma unroll
+        for (int t = 0; t < G::NHL; t++)
+            if (threadIdx.x + 256 * t < G::NH) dst[threadIdx.x + 256 * t] = hv[t];
+    };
+    int sx = -1;
+    float bxE = 0, axE = 0, kxE = 1, bxH = 0, axH = 0, kxH = 1;
+    if (PML) {
+        const float *t = a.tab + a.toff[0];
+        sx = slab_index(ic, a.I, a.nlo[0], a.nhi[0]);
+        bxE = t[ic], axE = t[a.I + ic], kxE = t[2 * a.I + ic];
+        bxH = t[3 * a.I + ic], axH = t[4 * a.I + ic], kxH = t[5 * a.I + ic];
+    }
+    int bm = (kb - 1) % 3;  // ring slot of plane k-1
+    gather(kb - 1, Hs[bm]);
+    for (int k = kb; k <= ke; k++) {
+        const int bk = bm == 2 ? 0 : bm + 1;  // ring slot of plane k
+        const int pk = k * a.P;
+        gather(k, Hs[bk]);
+        __syncthreads();
+        const float *Hk = Hs[bk];
+        const float *Hm = Hs[bm];
+        // ---- E_new(k) over the tile + halo ----
+        const bool plane_int = k <= a.K - 2, plane_own = k < ke;
+        const float *Jp = k == a.sk[0] ? a.sJ[0] : (k == a.sk[1] ? a.sJ[1] : nullptr);
+        int sz = -1;
+        float bzE = 0, azE = 0, kzE = 1;
+        if (PML) {
+            const float *t = a.tab + a.toff[2];
+            sz = slab_index(k, a.K, a.nlo[2], a.nhi[2]);
+            bzE = t[k], azE = t[a.K + k], kzE = t[2 * a.K + k];
+        }
+        float (*Ek)[G::ER][kTileW] = Es[k & 1];
+#pragma unroll 1
+        for (int r = w; r < G::ER; r += 4) {
+            const int jj = j0 + r, j = jj <= a.J - 1 ? jj : a.J - 1;
+            const bool row_int = jj <= a.J - 2;
+            const int c = pk + ic + a.I * j;
+            const float eox = S[c], eoy = S[nal + c], eoz = S[2 * nal + c];
+            const int m = mat_id<MatT>(a.mat, c);
+            const int h0 = (r + 1) * G::HC + lane + 1;  // centre of row r in the gather layout
+            const float hx = Hk[h0], hy = Hk[G::HR * G::HC + h0], hz = Hk[2 * G::HR * G::HC + h0];
+            const float hy_im = Hk[G::HR * G::HC + h0 - 1], hz_im = Hk[2 * G::HR * G::HC + h0 - 1];
+            const float hx_jm = Hk[h0 - G::HC], hz_jm = Hk[2 * G::HR * G::HC + h0 - G::HC];
+            const float hx_km = Hm[h0], hy_km = Hm[G::HR * G::HC + h0];
+            float x1 = hz - hz_jm, x2 = hy - hy_km;
+            float y1 = hx - hx_km, y2 = hz - hz_im;
+            float z1 = hy - hy_im, z2 = hx - hx_jm;
+            const bool upd = col_int && row_int && plane_int;
+            const bool own = col_own && r < TJ && row_int && plane_own && plane_int;
+            if (PML) {
+                const float *t = a.tab + a.toff[1];
+                const int sy = slab_index(j, a.J, a.nlo[1], a.nhi[1]);
+                const float byE = t[j], ayE = t[a.J + j], kyE = t[2 * a.J + j];
+                float tx1 = x1 * kyE, tx2 = x2 * kzE, ty1 = y1 * kzE, ty2 = y2 * kxE, tz1 = z1 * kxE, tz2 = z2 * kyE;
+                if (upd && sx >= 0) {
+                    const int q = a.poff[0] + sx + a.W[0] * (j + a.J * k);
+                    ty2 = pml_rw(a, q, bxE, axE, kxE, y2, own);
+                    tz1 = pml_rw(a, q + a.pn[0], bxE, axE, kxE, z1, own);
+                }
+                if (upd && sy >= 0) {
+                    const int q = a.poff[1] + ic + a.I * (sy + a.W[1] * k);
+                    tz2 = pml_rw(a, q, byE, ayE, kyE, z2, own);
+                    tx1 = pml_rw(a, q + a.pn[1], byE, ayE, kyE, x1, own);
+                }
+                if (upd && sz >= 0) {
+                    const int q = a.poff[2] + ic + a.I * j + a.P * sz;
+                    tx2 = pml_rw(a, q, bzE, azE, kzE, x2, own);
+                    ty1 = pml_rw(a, q + a.pn[2], bzE, azE, kzE, y1, own);
+                }
+                x1 = tx1, x2 = tx2, y1 = ty1, y2 = ty2, z1 = tz1, z2 = tz2;
+            }
+            float ex = eox, ey = eoy, ez = eoz;
+            if (upd) {
+                const float *cf = a.coef + m * kCoefStride;
+                ex = cf[0] * eox + cf[3] * (x1 - x2);
+                ey = cf[1] * eoy + cf[4] * (y1 - y2);
+                ez = cf[2] * eoz + cf[5] * (z1 - z2);
+                float sv[3];
+                if (fsource3(a, Jp, k, c - pk, 0, sv)) {
+                    ex -= cf[6] * sv[0];
+                    ey -= cf[7] * sv[1];
+                    ez -= cf[8] * sv[2];
+                }
+            }
+            Ek[0][r][lane] = ex;
+            Ek[1][r][lane] = ey;
+            Ek[2][r][lane] = ez;
+            if (own) {
+                D[c] = ex;
+                D[nal + c] = ey;
+                D[2 * nal + c] = ez;
+            }
+        }
+        __syncthreads();
+        // ---- H_new(k-1) over the owned cells ----
+        if (k > kb) {
+            const int kh = k - 1, ph = kh * a.P;
+            const float *Mp = kh == a.sk[0] ? a.sJ[0] : (kh == a.sk[1] ? a.sJ[1] : nullptr);
+            float (*Ep)[G::ER][kTileW] = Es[kh & 1];
+            int szh = -1;
+            float bzH = 0, azH = 0, kzH = 1;
+            if (PML) {
+                const float *t = a.tab + a.toff[2];
+                szh = slab_index(kh, a.K, a.nlo[2], a.nhi[2]);
+                bzH = t[3 * a.K + kh], azH = t[4 * a.K + kh], kzH = t[5 * a.K + kh];
+            }
+#pragma unroll 1
+            for (int r = w; r < TJ; r += 4) {
+                const int jj = j0 + r;
+                if (!col_own || jj > a.J - 2) continue;
+                const int j = jj;
+                const int c = ph + ic + a.I * j;
+                const int m = mat_id<MatT>(a.mat, c);
+                const int h0 = (r + 1) * G::HC + lane + 1;
+                const float hxo = Hm[h0], hyo = Hm[G::HR * G::HC + h0], hzo = Hm[2 * G::HR * G::HC + h0];
+                const float ex = Ep[0][r][lane], ey = Ep[1][r][lane], ez = Ep[2][r][lane];
+                const float ey_ip = Ep[1][r][lane_ip], ez_ip = Ep[2][r][lane_ip];
+                const float ex_jp = Ep[0][r + 1][lane], ez_jp = Ep[2][r + 1][lane];
+                const float ex_kp = Ek[0][r][lane], ey_kp = Ek[1][r][lane];
+                float x1 = ez_jp - ez, x2 = ey_kp - ey;
+                float y1 = ex_kp - ex, y2 = ez_ip - ez;
+                float z1 = ey_ip - ey, z2 = ex_jp - ex;
+                if (PML) {
+                    const float *t = a.tab + a.toff[1];
+                    const int sy = slab_index(j, a.J, a.nlo[1], a.nhi[1]);
+                    const float byH = t[3 * a.J + j], ayH = t[4 * a.J + j], kyH = t[5 * a.J + j];
+                    float tx1 = x1 * kyH, tx2 = x2 * kzH, ty1 = y1 * kzH, ty2 = y2 * kxH, tz1 = z1 * kxH,
+                          tz2 = z2 * kyH;
+                    if (sx >= 0) {
+                        const int q = a.poff[0] + 2 * a.pn[0] + sx + a.W[0] * (j + a.J * kh);
+                        ty2 = pml_rw(a, q, bxH, axH, kxH, y2, true);
+                        tz1 = pml_rw(a, q + a.pn[0], bxH, axH, kxH, z1, true);
+                    }
+                    if (sy >= 0) {
+                        const int q = a.poff[1] + 2 * a.pn[1] + ic + a.I * (sy + a.W[1] * kh);
+                        tz2 = pml_rw(a, q, byH, ayH, kyH, z2, true);
+                        tx1 = pml_rw(a, q + a.pn[1], byH, ayH, kyH, x1, true);
+                    }
+                    if (szh >= 0) {
+                        const int q = a.poff[2] + 2 * a.pn[2] + ic + a.I * j + a.P * szh;
+                        tx2 = pml_rw(a, q, bzH, azH, kzH, x2, true);
+                        ty1 = pml_rw(a, q + a.pn[2], bzH, azH, kzH, y1, true);
+                    }
+                    x1 = tx1, x2 = tx2, y1 = ty1, y2 = ty2, z1 = tz1, z2 = tz2;
+                }
+                const float *cf = a.coef + m * kCoefStride;
+                float hxn = cf[9] * hxo - cf[12] * (x1 - x2);
+                float hyn = cf[10] * hyo - cf[13] * (y1 - y2);
+                float hzn = cf[11] * hzo - cf[14] * (z1 - z2);
+                float sv[3];
+                if (fsource3(a, Mp, kh, c - ph, 3, sv)) {
+                    hxn -= cf[15] * sv[0];
+                    hyn -= cf[16] * sv[1];
+                    hzn -= cf[17] * sv[2];
+                }
+                D[3 * nal + c] = hxn;
+                D[4 * nal + c] = hyn;
+                D[5 * nal + c] = hzn;
+            }
+        }
+        bm = bk;
+    }
+}
 }  // namespace
 
 // Copy the current fields into the alternate buffer (boundary walls and all), so that both
@@ -378,8 +601,11 @@ int fused_step(fdtd_dev *d, double dt, int ty_rows, int kc_len) {
     a.J = d->J;
     a.K = d->K;
     a.P = (int)d->P;
-    a.nwx = (d->I - 2 + kStrip - 1) / kStrip;
-    a.nty = (d->J - 2 + ty_rows - 1) / ty_rows;
+    // rows 16 / 32: LDS-tiled kernel with tiles of 63 x 8 / 63 x 16 cells; else register strips
+    const bool tiled = ty_rows == 16 || ty_rows == 32;
+    const int tile_rows = ty_rows / 2;
+    a.nwx = tiled ? (d->I - 2 + kTileOwn - 1) / kTileOwn : (d->I - 2 + kStrip - 1) / kStrip;
+    a.nty = tiled ? (d->J - 2 + tile_rows - 1) / tile_rows : (d->J - 2 + ty_rows - 1) / ty_rows;
     a.kc_len = kc_len;
     a.nkc = (d->K - 2 + kc_len - 1) / kc_len;
     a.sk[0] = a.sk[1] = -1;
@@ -406,29 +632,45 @@ int fused_step(fdtd_dev *d, double dt, int ty_rows, int kc_len) {
         a.psrc = cp.psi_base;
         a.pdst = cp.psi_alt;
     }
-    const long long waves = (long long)a.nwx * a.nty * a.nkc;
-    dim3 grid((unsigned)((waves + 3) / 4));
+    const long long units = (long long)a.nwx * a.nty * a.nkc;  // wavefronts (strips) or blocks (tiles)
+    dim3 grid((unsigned)(tiled ? units : (units + 3) / 4));
     const bool u8 = d->mat_bytes == 1;
-    hipEvent_t ev_stop = timing_begin(d, 2);
-#define FDTD_FUSED(TYV)                                                                                       \
-    do {                                                                                                      \
-        if (u8 && pml) hipLaunchKernelGGL((k_step_fused<uint8_t, TYV, true>), grid, dim3(256), 0, d->stream, a); \
-        else if (u8) hipLaunchKernelGGL((k_step_fused<uint8_t, TYV, false>), grid, dim3(256), 0, d->stream, a); \
-        else if (pml) hipLaunchKernelGGL((k_step_fused<uint16_t, TYV, true>), grid, dim3(256), 0, d->stream, a); \
-        else hipLaunchKernelGGL((k_step_fused<uint16_t, TYV, false>), grid, dim3(256), 0, d->stream, a);       \
+    hipEvent_t t0, t1;
+    timing_events(d, 2, &t0, &t1);
+#define FDTD_FUSED_ONE(MT, TYV, PM) \
+    hipExtLaunchKernelGGL((k_step_fused<MT, TYV, PM>), grid, dim3(256), 0, d->stream, t0, t1, 0, a)
+#define FDTD_FUSED(TYV)                                    \
+    do {                                                   \
+        if (u8 && pml) FDTD_FUSED_ONE(uint8_t, TYV, true);  \
+        else if (u8) FDTD_FUSED_ONE(uint8_t, TYV, false);   \
+        else if (pml) FDTD_FUSED_ONE(uint16_t, TYV, true);  \
+        else FDTD_FUSED_ONE(uint16_t, TYV, false);          \
+    } while (0)
+#define FDTD_TILED_ONE(MT, TJV, PM) \
+    hipExtLaunchKernelGGL((k_step_tiled<MT, TJV, PM>), grid, dim3(256), 0, d->stream, t0, t1, 0, a)
+#define FDTD_TILED(TJV)                                    \
+    do {                                                   \
+        if (u8 && pml) FDTD_TILED_ONE(uint8_t, TJV, true);  \
+        else if (u8) FDTD_TILED_ONE(uint8_t, TJV, false);   \
+        else if (pml) FDTD_TILED_ONE(uint16_t, TJV, true);  \
+        else FDTD_TILED_ONE(uint16_t, TJV, false);          \
     } while (0)
     switch (ty_rows) {
         case 1: FDTD_FUSED(1); break;
         case 2: FDTD_FUSED(2); break;
         case 4: FDTD_FUSED(4); break;
         case 8: FDTD_FUSED(8); break;
+        case 16: FDTD_TILED(8); break;
+        case 32: FDTD_TILED(16); break;
         default:
-            set_error("fused step: rows per lane must be 1, 2, 4 or 8");
+            set_error("fused step: rows must be 1, 2, 4, 8 (register strips) or 16, 32 (LDS tiles)");
             return -2;
     }
 #undef FDTD_FUSED
+#undef FDTD_FUSED_ONE
+#undef FDTD_TILED
+#undef FDTD_TILED_ONE
     FDTD_TRY(hipGetLastError());
-    if (ev_stop) FDTD_TRY(hipEventRecord(ev_stop, d->stream));
     // swap the ping-pong buffers
     float *t = d->field_base;
     d->field_base = d->alt_base;

@@ -2,6 +2,7 @@
 // Not part of the C-ABI: the public entry points are declared in include/fdtd3d_dev.h.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -41,6 +42,17 @@ struct RunSrcArg {
     const float *PR, *PI;
     const unsigned char *win;
     float c1, c2;
+};
+
+// An overlap monitor evaluated inside the H update of a device-resident run: each block of the
+// monitor plane stores its FP64 partial of Hx mEy - Hy mEx + Ey mHx - Ex mHy (W holds Re/Im of
+// mEx, mEy, mHx, mHy as 8 x P floats) into part[block] for the step; the running DFT over the
+// steps of a run call is formed afterwards from those partials in a fixed order.
+constexpr int kMaxRunMon = 4;
+struct RunMonArg {
+    int k;
+    const float *W;
+    double2 *part;  // this step's [plane blocks] partials
 };
 
 struct SrcPlane {
@@ -83,15 +95,17 @@ struct Cpml {
     long long psi_n[3] = {0, 0, 0};
 };
 
-// Sampled per-kernel timing of the E / H updates (HIP events on the grid's stream).
+// Sampled per-kernel timing of the E / H updates.  A sampled launch goes through
+// hipExtLaunchKernelGGL with a start/stop event pair, which the dispatch itself records, so the
+// interval is the kernel's own execution (no marker packets around it).
 struct KernelTiming {
     int every = 0;
     long long launches[3] = {0, 0, 0};
     std::vector<hipEvent_t> ev[3];  // start/stop pairs: split E, split H, fused step
 };
 
-// Start a sampled timing bracket for kernel slot `which` (returns the stop event or null).
-hipEvent_t timing_begin(fdtd_dev *d, int which);
+// Events for the next launch of kernel slot `which`: a fresh pair on a sampled launch, else null.
+void timing_events(fdtd_dev *d, int which, hipEvent_t *start, hipEvent_t *stop);
 
 struct BlockHist {
     const void *key;
@@ -118,6 +132,7 @@ struct fdtd_dev {
     bool alt_valid;
     int fused_rows;     // rows per lane of the fused step (0 = split E/H kernels)
     int fused_kchunk;   // z planes per wavefront of the fused step
+    int split_kchunk;   // z planes per thread of the split E/H kernels (0 = chosen per launch)
 
     // materials
     int nmat;
@@ -153,6 +168,8 @@ struct fdtd_dev {
     void *run;  // device-resident run configuration (dev_run.hip), or null
     int n_run_src;                              // run sources evaluated by the next step
     fdtd::RunSrcArg run_src[fdtd::kMaxRunSrc];
+    int n_run_mon;                              // overlap monitors folded into the next H update
+    fdtd::RunMonArg run_mon[fdtd::kMaxRunMon];
     fdtd::KernelTiming ktime;
 
     hipEvent_t ev0, ev1;
@@ -173,6 +190,8 @@ int cpml_zero(fdtd_dev *d);
 int cpml_tables(fdtd_dev *d, double dt);
 int fused_step(fdtd_dev *d, double dt, int ty_rows, int kc_len);
 bool fused_supported(const fdtd_dev *d);
+// True when fdtd_dev_step runs the split E/H kernels (the path that can fold run monitors).
+bool step_uses_split(const fdtd_dev *d);
 int fused_sync_alt(fdtd_dev *d);
 // The device source plane for z = k (created zeroed on first use), or null with *err set.
 SrcPlane *plane_for(fdtd_dev *d, int k, int *err);

@@ -27,6 +27,7 @@ struct RunOverlap {
     double2 *acc;          // nfreq cumulative overlaps
     double2 *inst;         // last instantaneous overlap
     int freq_off;          // offset into the run's twiddle table
+    bool ring_free;        // plane inside [1, K-2] and window clear of the i/j boundary ring
 };
 
 struct RunDft {
@@ -56,6 +57,11 @@ struct Run {
     int mon_cap = 0;
     double2 *tw = nullptr;        // [step][total freq] twiddles for the current run() call
     long long tw_cap = 0;
+    // monitors evaluated inside the H update: per-step block partials [monitor][step][block]
+    // and per-step overlaps [monitor][step] of the current run() call
+    double2 *mpart = nullptr;
+    double2 *minst = nullptr;
+    long long mstep_cap = 0;
 };
 
 Run *run_of(fdtd_dev *d) {
@@ -140,6 +146,57 @@ __global__ __launch_bounds__(kBlock) void k_run_overlap_final_multi(const double
     }
 }
 
+// Overlap per step from the block partials the H update stored: inst[m][s] = scale * sum_b
+// part[m][s][b], blocks summed in index order.
+__global__ void k_run_mon_steps(const double2 *part, double2 *inst, int n, int nbp, long long cap, double scale) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
+    if (s >= n) return;
+    const double2 *pp = part + ((long long)m * cap + s) * nbp;
+    double r = 0, q = 0;
+    for (int b = 0; b < nbp; b++) {
+        r += pp[b].x;
+        q += pp[b].y;
+    }
+    inst[(long long)m * cap + s] = make_double2(scale * r, scale * q);
+}
+
+// Running DFT of a run call's per-step overlaps, one block per monitor: acc[f] += sum_s tw[s][f]
+// inst[s].  Thread t sums a fixed contiguous range of steps, the block combines the 256 partial
+// sums in a fixed tree, so the result does not depend on scheduling.  Also leaves the last
+// step's overlap in *inst_last.
+__global__ __launch_bounds__(kBlock) void k_run_mon_dft(const MonDev *mons, const double2 *inst, int n, long long cap,
+                                                        const double2 *tw, int nf) {
+    __shared__ double sre[kBlock], sim[kBlock];
+    const MonDev M = mons[blockIdx.x];
+    const double2 *x = inst + (long long)blockIdx.x * cap;
+    const int per = (n + kBlock - 1) / kBlock;
+    const int s0 = threadIdx.x * per, s1 = min(n, s0 + per);
+    for (int f = 0; f < M.nfreq; f++) {
+        double r = 0, q = 0;
+        for (int s = s0; s < s1; s++) {
+            const double2 t = tw[(long long)s * nf + M.twoff + f], v = x[s];
+            r += t.x * v.x - t.y * v.y;
+            q += t.x * v.y + t.y * v.x;
+        }
+        sre[threadIdx.x] = r;
+        sim[threadIdx.x] = q;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                sre[threadIdx.x] += sre[threadIdx.x + w];
+                sim[threadIdx.x] += sim[threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double2 a = M.acc[f];
+            M.acc[f] = make_double2(a.x + sre[0], a.y + sim[0]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *M.inst = x[n - 1];
+}
+
 __global__ void k_run_dft_plane(double2 *acc, const float *base, long long Nal, long long P, long long koff, int mask,
                                 int nfreq, const double2 *tw) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -184,6 +241,8 @@ void run_release(fdtd_dev *d) {
     (void)hipFree(r->partials);
     (void)hipFree(r->mons);
     (void)hipFree(r->tw);
+    (void)hipFree(r->mpart);
+    (void)hipFree(r->minst);
     delete r;
     d->run = nullptr;
 }
@@ -301,12 +360,25 @@ int fdtd_dev_run_add_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *
     o.nfreq = nfreq;
     o.omega.assign(omega, omega + nfreq);
     o.freq_off = 0;
+    // the H update only sees cells with 1 <= i <= I-2, 1 <= j <= J-2: the monitor can ride on it
+    // when its weights vanish on the boundary ring and its plane is an updated one
+    o.ring_free = cz >= 1 && cz <= d->K - 2;
+    for (int j = 0; j < d->J && o.ring_free; j++)
+        for (int i = 0; i < d->I; i++) {
+            if (i != 0 && i != d->I - 1 && j != 0 && j != d->J - 1) continue;
+            for (int q = 0; q < 8; q++)
+                if (W[q * P + i + (long long)d->I * j] != 0.0f) o.ring_free = false;
+        }
+    const size_t nf1 = (size_t)(nfreq > 0 ? nfreq : 1);
+    o.W = nullptr;
+    o.acc = o.inst = nullptr;
     FDTD_TRY(hipMalloc(&o.W, W.size() * sizeof(float)));
     FDTD_TRY(hipMemcpy(o.W, W.data(), W.size() * sizeof(float), hipMemcpyHostToDevice));
-    FDTD_TRY(hipMalloc(&o.acc, (size_t)(nfreq > 0 ? nfreq : 1) * sizeof(double2)));
-    FDTD_TRY(hipMemset(o.acc, 0, (size_t)(nfreq > 0 ? nfreq : 1) * sizeof(double2)));
+    FDTD_TRY(hipMalloc(&o.acc, nf1 * sizeof(double2)));
+    FDTD_TRY(hipMemset(o.acc, 0, nf1 * sizeof(double2)));
     FDTD_TRY(hipMalloc(&o.inst, sizeof(double2)));
     FDTD_TRY(hipMemset(o.inst, 0, sizeof(double2)));
+
     r->ovl.push_back(std::move(o));
     return (int)r->ovl.size() - 1;
 }
@@ -331,8 +403,16 @@ int fdtd_dev_run_add_dft_plane(fdtd_dev *d, int k, int comp_mask, int nfreq, con
     return (int)r->dft.size() - 1;
 }
 
-int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
-    if (n <= 0) return 0;
+}  // extern "C"
+
+namespace fdtd {
+namespace {
+
+// Steps n0 .. n0+n-1 of a run (n at most kRunChunk, which bounds the twiddle table and the
+// per-step monitor partials).
+constexpr int kRunChunk = 4096;
+
+int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     Run *r = run_of(d);
     // twiddle table for steps n0..n0+n-1 over every monitor frequency: e^{i omega (step + t_offset) dt}
     int nf = 0;
@@ -404,6 +484,18 @@ int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     };
     const double scale = -0.5 * d->disc * d->disc;
     const int ninl = (int)r->src.size() < kMaxRunSrc ? (int)r->src.size() : kMaxRunSrc;
+    // overlap monitors accumulate inside the H update when every one of them can
+    bool fold = nmon > 0 && nmon <= kMaxRunMon && step_uses_split(d);
+    for (auto &o : r->ovl) fold = fold && o.ring_free;
+    if (fold && n > r->mstep_cap) {
+        FDTD_TRY(hipStreamSynchronize(d->stream));
+        (void)hipFree(r->mpart);
+        (void)hipFree(r->minst);
+        r->mpart = r->minst = nullptr;
+        FDTD_TRY(hipMalloc(&r->mpart, (size_t)kMaxRunMon * n * nbp * sizeof(double2)));
+        FDTD_TRY(hipMalloc(&r->minst, (size_t)kMaxRunMon * n * sizeof(double2)));
+        r->mstep_cap = n;
+    }
     for (int s = 0; s < n; s++) {
         const int step = n0 + s;
         d->n_run_src = ninl;
@@ -418,10 +510,22 @@ int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
                                rs.PR, rs.PI, rs.win, d->P, coef_at(rs, step));
             FDTD_TRY(hipGetLastError());
         }
+        if (fold) {
+            d->n_run_mon = nmon;
+            for (int m = 0; m < nmon; m++)
+                d->run_mon[m] = RunMonArg{r->ovl[m].k, r->ovl[m].W,
+                                          r->mpart + ((size_t)m * r->mstep_cap + s) * nbp};
+        }
         int rc = fdtd_dev_step(d, dt, 0);
         d->n_run_src = 0;
+        const bool consumed = d->n_run_mon == 0;
+        d->n_run_mon = 0;
         if (rc) return rc;
-        if (nmon > 0) {
+        if (!consumed) {
+            set_error("run monitors were not consumed by the H update");
+            return -3;
+        }
+        if (nmon > 0 && !fold) {
             hipLaunchKernelGGL(k_run_overlap_partial_multi, dim3(nbp, nmon), dim3(kBlock), 0, d->stream,
                                (const float *)d->field_base, d->Nal, d->P, (const MonDev *)r->mons, r->partials,
                                (int)nbp);
@@ -436,6 +540,13 @@ int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
             FDTD_TRY(hipGetLastError());
         }
     }
+    if (fold) {
+        hipLaunchKernelGGL(k_run_mon_steps, dim3((n + kBlock - 1) / kBlock, nmon), dim3(kBlock), 0, d->stream,
+                           (const double2 *)r->mpart, r->minst, n, (int)nbp, r->mstep_cap, scale);
+        hipLaunchKernelGGL(k_run_mon_dft, dim3(nmon), dim3(kBlock), 0, d->stream, (const MonDev *)r->mons,
+                           (const double2 *)r->minst, n, r->mstep_cap, (const double2 *)r->tw, nf);
+        FDTD_TRY(hipGetLastError());
+    }
     // leave each inline source's plane holding the values of the last step, as a per-step
     // write would have
     for (int q = 0; q < ninl; q++) {
@@ -443,6 +554,19 @@ int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
         hipLaunchKernelGGL(k_run_source, dim3(nbp), dim3(kBlock), 0, d->stream, planes[q]->J, planes[q]->stride,
                            rs.PR, rs.PI, rs.win, d->P, coef_at(rs, n0 + n - 1));
         FDTD_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace fdtd
+
+extern "C" {
+
+int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
+    for (int s = 0; s < n; s += kRunChunk) {
+        const int rc = run_chunk(d, dt, n0 + s, n - s < kRunChunk ? n - s : kRunChunk, t_offset);
+        if (rc) return rc;
     }
     return 0;
 }

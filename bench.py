@@ -124,6 +124,21 @@ def cpu_baseline(target_s=12.0):
                       f"{el:.1f} s, OMP_NUM_THREADS={threads}"}
 
 
+def calibrate_copy(mib, reps=8):
+    """A plain read+write elementwise kernel of known size (torch.neg over `mib` MiB of FP32):
+    what rocprofv3's FETCH_SIZE / WRITE_SIZE read for it scales those counters to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "kernel": "torch.neg (elementwise FP32)", "launches": reps,
+                      "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4}), flush=True)
+
+
 def read_traffic():
     p = os.path.join(ROOT, "profiles", "yee_traffic_cfg2.json")
     if not os.path.exists(p):
@@ -139,8 +154,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
-    ap.add_argument("--kchunk", type=int, default=None, help="fused step z planes per wavefront")
+    ap.add_argument("--kchunk", type=int, default=None,
+                    help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the path would shard by geometry or by x-slabs, see DESIGN.md)", file=sys.stderr)
@@ -150,8 +171,9 @@ def main():
     W, S = args.warmup, args.steps
     g, dt, md, mons, omegas = workload(fd, W + S)
     if args.rows is not None or args.kchunk is not None:
-        fd.check(L.fdtd3d_ext_set_step_kernel(g.tf, 4 if args.rows is None else args.rows,
-                                              16 if args.kchunk is None else args.kchunk), "step kernel")
+        rows = 0 if args.rows is None else args.rows
+        kchunk = args.kchunk if args.kchunk is not None else (0 if rows == 0 else 8)
+        fd.check(L.fdtd3d_ext_set_step_kernel(g.tf, rows, kchunk), "step kernel")
     print(f"bench: warmup {W} steps", file=sys.stderr, flush=True)
     fd.check(L.fdtd3d_ext_run(g.tf, dt, 0, W, 0.0), "warmup run")
     fd.check(L.fdtd3d_ext_synchronize(g.tf), "sync")

@@ -56,9 +56,10 @@ int fdtd_dev_cpml_on(fdtd_dev *d, const int npml[6], int order, double R0, doubl
 int fdtd_dev_cpml_off(fdtd_dev *d);
 int fdtd_dev_update_E_cpml(fdtd_dev *d, double dt);
 int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt);
-/* Kernel used by fdtd_dev_step under the PML boundary: rows = 0 selects the split E / H pair,
- * rows = 2, 4 or 8 the fused single-kernel step with that many y rows per lane and kchunk z
- * planes per wavefront.  Default: the split pair (rows = 0). */
+/* Kernel used by fdtd_dev_step under the PML boundary: rows = 0 selects the split E / H pair
+ * (kchunk > 0 fixes their z planes per thread, 0 lets each launch choose), rows = 1, 2, 4 or 8
+ * the fused single-kernel step with that many y rows per lane and kchunk z planes per
+ * wavefront.  Default: the split pair with automatic chunking. */
 int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk);
 
 /* Device-resident run (extension).  A run holds mode sources whose per-step amplitude and

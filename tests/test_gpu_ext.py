@@ -66,7 +66,7 @@ def test_cpml_step_vs_oracle(npml):
     g.free()
 
 
-@pytest.mark.parametrize("rows,kchunk", [(2, 3), (4, 16), (8, 5)])
+@pytest.mark.parametrize("rows,kchunk", [(2, 3), (4, 16), (8, 5), (16, 3), (16, 7), (32, 4)])
 def test_fused_step_vs_split_and_oracle(rows, kchunk):
     """Fused one-kernel step (ping-pong buffers) vs the split E/H kernels and the FP64 oracle, on
     an odd-sized grid (partial strips, tiles and chunks) with conductivity, a plane source and
@@ -113,9 +113,13 @@ def test_fused_step_vs_split_and_oracle(rows, kchunk):
     assert d <= 1e-6
 
 
-def test_run_api_matches_reference_monitors(golden_dir):
+@pytest.mark.parametrize("ring_monitor", [False, True])
+def test_run_api_matches_reference_monitors(golden_dir, ring_monitor):
     """Device-resident run (tabulated source, overlap monitors with DFT) vs the reference's
-    per-step source/overlap calls recorded in fixture F4 (single frequency)."""
+    per-step source/overlap calls recorded in fixture F4 (single frequency).  The three F4
+    monitors are accumulated inside the H update; an extra monitor whose window reaches the
+    boundary ring switches the run to the separate monitor kernels."""
+    say("run API vs F4, ring monitor", ring_monitor)
     z = np.load(os.path.join(golden_dir, "F4_mode_source_overlap.npz"))
     I, J, K = (int(v) for v in z["dims"])
     dt, om = float(z["dt"]), float(z["omega"])
@@ -136,6 +140,8 @@ def test_run_api_matches_reference_monitors(golden_dir):
     ids = [L().fdtd3d_ext_run_add_overlap(g.tf, I // 2, J // 2, kz, mm, rev, 1, oms.ctypes.data_as(DP))
            for kz, rev, mm in ((15, 0, m1), (15, 1, m1), (45, 0, m2))]
     assert min(ids) >= 0
+    if ring_monitor:
+        assert L().fdtd3d_ext_run_add_overlap(g.tf, 2, J // 2, 30, m1, 0, 1, oms.ctypes.data_as(DP)) >= 0
     # two calls, to exercise the step offset
     fd.check(L().fdtd3d_ext_run(g.tf, dt, 0, 50, 0.0), "run")
     fd.check(L().fdtd3d_ext_run(g.tf, dt, 50, n - 50, 0.0), "run")
@@ -151,6 +157,132 @@ def test_run_api_matches_reference_monitors(golden_dir):
     # the caller's modes are untouched by the run API
     assert np.array_equal(fd.mode_fields(m1), z["mode1"])
     g.free()
+
+
+def test_run_monitor_fold_matches_separate_kernels(golden_dir):
+    """CPML grid, 16 frequencies, two run calls: monitors accumulated inside the H update agree
+    with the separate monitor kernels (FP64 sums in another order: relative 1e-10)."""
+    z = np.load(os.path.join(golden_dir, "F4_mode_source_overlap.npz"))
+    I, J, K = (int(v) for v in z["dims"])
+    dt, om = float(z["dt"]), float(z["omega"])
+    n = 90
+    steps = np.arange(n)
+    amp = np.exp(-(steps - 40) ** 2 / (2 * 15.0 ** 2))
+    phase = steps * dt * om
+    oms = om * np.linspace(0.9, 1.1, 16)
+    results = []
+    for ring_monitor in (False, True):
+        say("fold vs separate, ring monitor", ring_monitor)
+        g = fd.Grid(I, J, K, float(z["disc"]))
+        g.a["ep"][:] = z["ep"]
+        g.a["mu"][:] = z["mu"]
+        g.push()
+        npml = (C.c_int * 6)(6, 6, 6, 6, 6, 6)
+        fd.check(L().fdtd3d_ext_set_cpml(g.tf, npml, 3, 1e-6, 1.0, 0.05), "cpml")
+        m1 = fd.mode_from_array(30, 30, float(z["disc"]), z["mode1"])
+        m2 = fd.mode_from_array(24, 24, float(z["disc"]), z["mode2"])
+        assert L().fdtd3d_ext_run_add_mode_source(g.tf, I // 2, J // 2, 10, m1, 0, 0, n, amp.ctypes.data_as(DP),
+                                                  phase.ctypes.data_as(DP)) >= 0
+        ids = [L().fdtd3d_ext_run_add_overlap(g.tf, I // 2, J // 2, kz, mm, rev, len(oms), oms.ctypes.data_as(DP))
+               for kz, rev, mm in ((15, 0, m1), (15, 1, m1), (45, 0, m2))]
+        assert min(ids) >= 0
+        if ring_monitor:
+            assert L().fdtd3d_ext_run_add_overlap(g.tf, 1, J // 2, 30, m1, 0, 2, oms.ctypes.data_as(DP)) >= 0
+        fd.check(L().fdtd3d_ext_run(g.tf, dt, 0, 37, 0.0), "run")
+        fd.check(L().fdtd3d_ext_run(g.tf, dt, 37, n - 37, 0.0), "run")
+        out = []
+        for mid in ids:
+            cum = (fd.Complex * len(oms))()
+            last = fd.Complex()
+            fd.check(L().fdtd3d_ext_run_overlap(g.tf, mid, cum, C.byref(last)), "result")
+            out.append([complex(c.re, c.im) for c in cum] + [complex(last.re, last.im)])
+        results.append(np.array(out))
+        g.free()
+    fold, sep = results
+    err = np.abs(fold - sep).max() / np.abs(sep).max()
+    say("fold vs separate max rel", err)
+    assert np.abs(sep).max() > 0
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("rows", [8, 16])
+def test_run_api_fused_matches_split(golden_dir, rows):
+    """Device-resident run under the PML with a fused step kernel (run sources evaluated inside
+    the fused kernel, monitors by the separate kernels) vs the split kernels (monitors inside the
+    H update): overlaps agree to relative 1e-5."""
+    z = np.load(os.path.join(golden_dir, "F4_mode_source_overlap.npz"))
+    I, J, K = (int(v) for v in z["dims"])
+    dt, om = float(z["dt"]), float(z["omega"])
+    n = 120
+    steps = np.arange(n)
+    amp = np.exp(-(steps - 50) ** 2 / (2 * 20.0 ** 2))
+    phase = steps * dt * om
+    oms = om * np.linspace(0.9, 1.1, 8)
+    results = []
+    for kernel in (0, rows):
+        say("run API step kernel rows", kernel)
+        g = fd.Grid(I, J, K, float(z["disc"]))
+        g.a["ep"][:] = z["ep"]
+        g.a["mu"][:] = z["mu"]
+        g.push()
+        npml = (C.c_int * 6)(6, 6, 6, 6, 6, 6)
+        fd.check(L().fdtd3d_ext_set_cpml(g.tf, npml, 3, 1e-6, 1.0, 0.05), "cpml")
+        fd.check(L().fdtd3d_ext_set_step_kernel(g.tf, kernel, 0 if kernel == 0 else 5), "kernel")
+        m1 = fd.mode_from_array(30, 30, float(z["disc"]), z["mode1"])
+        assert L().fdtd3d_ext_run_add_mode_source(g.tf, I // 2, J // 2, 10, m1, 0, 0, n, amp.ctypes.data_as(DP),
+                                                  phase.ctypes.data_as(DP)) >= 0
+        ids = [L().fdtd3d_ext_run_add_overlap(g.tf, I // 2, J // 2, kz, m1, rev, len(oms), oms.ctypes.data_as(DP))
+               for kz, rev in ((15, 0), (15, 1), (45, 0))]
+        fd.check(L().fdtd3d_ext_run(g.tf, dt, 0, n, 0.0), "run")
+        out = []
+        for mid in ids:
+            cum = (fd.Complex * len(oms))()
+            fd.check(L().fdtd3d_ext_run_overlap(g.tf, mid, cum, None), "result")
+            out.append([complex(c.re, c.im) for c in cum])
+        results.append(np.array(out))
+        g.free()
+    err = np.abs(results[0] - results[1]).max() / np.abs(results[0]).max()
+    say("fused vs split run monitors max rel", err)
+    assert np.abs(results[0]).max() > 0
+    assert err <= 1e-5
+
+
+def test_run_long_call_matches_split_calls(golden_dir):
+    """A run call longer than the library's internal step chunk (4096) agrees with the same
+    steps issued as two calls with other chunk boundaries (monitors to relative 1e-10)."""
+    z = np.load(os.path.join(golden_dir, "F4_mode_source_overlap.npz"))
+    I, J, K = (int(v) for v in z["dims"])
+    dt, om = float(z["dt"]), float(z["omega"])
+    n = 4200
+    steps = np.arange(n)
+    amp = np.exp(-(steps - 60) ** 2 / (2 * 25.0 ** 2))
+    phase = steps * dt * om
+    oms = om * np.linspace(0.95, 1.05, 4)
+    results = []
+    for calls in ((0, n), (0, 100, n)):
+        say("long run, call boundaries", calls)
+        g = fd.Grid(I, J, K, float(z["disc"]))
+        g.a["ep"][:] = z["ep"]
+        g.a["mu"][:] = z["mu"]
+        g.push()
+        npml = (C.c_int * 6)(6, 6, 6, 6, 6, 6)
+        fd.check(L().fdtd3d_ext_set_cpml(g.tf, npml, 3, 1e-6, 1.0, 0.05), "cpml")
+        m1 = fd.mode_from_array(30, 30, float(z["disc"]), z["mode1"])
+        assert L().fdtd3d_ext_run_add_mode_source(g.tf, I // 2, J // 2, 10, m1, 0, 0, n, amp.ctypes.data_as(DP),
+                                                  phase.ctypes.data_as(DP)) >= 0
+        mid = L().fdtd3d_ext_run_add_overlap(g.tf, I // 2, J // 2, 45, m1, 0, len(oms), oms.ctypes.data_as(DP))
+        assert mid >= 0
+        for a, b in zip(calls[:-1], calls[1:]):
+            fd.check(L().fdtd3d_ext_run(g.tf, dt, a, b - a, 0.0), "run")
+        cum = (fd.Complex * len(oms))()
+        last = fd.Complex()
+        fd.check(L().fdtd3d_ext_run_overlap(g.tf, mid, cum, C.byref(last)), "result")
+        results.append(np.array([complex(c.re, c.im) for c in cum] + [complex(last.re, last.im)]))
+        g.free()
+    err = np.abs(results[0] - results[1]).max() / np.abs(results[1]).max()
+    say("long vs split calls max rel", err)
+    assert np.abs(results[1]).max() > 0
+    assert err <= 1e-10
 
 
 def test_run_dft_plane_vs_oracle(golden_dir):
