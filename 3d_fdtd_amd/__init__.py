@@ -151,6 +151,9 @@ SIGNATURES = {
     "fdtd3d_ext_run": ([TimeField, _d, _i, _i, _d], _i),
     "fdtd3d_ext_run_overlap": ([TimeField, _i, _P(Complex), _P(Complex)], _i),
     "fdtd3d_ext_run_dft": ([TimeField, _i, _P(Complex)], _i),
+    "fdtd3d_ext_run_add_dft_box": ([TimeField, _P(_i), _i, _i, _P(_d)], _i),
+    "fdtd3d_ext_run_add_adjoint_source": ([TimeField, _i, _i, _i, Mode, _i, _d, _i, _P(_d), _P(_d)], _i),
+    "fdtd3d_ext_dft_gradient": ([TimeField, _i, TimeField, _i, _i, Complex, _P(_d)], _i),
 }
 
 _lib = None

@@ -30,10 +30,13 @@ struct RunOverlap {
     bool ring_free;        // plane inside [1, K-2] and window clear of the i/j boundary ring
 };
 
+// A DFT monitor over the box [i0,i1) x [j0,j1) x [k0,k1) (a plane is the box of one z layer).
 struct RunDft {
-    int k, mask, ncomp, nfreq;
+    int box[6];
+    long long nb;          // cells in the box
+    int mask, ncomp, nfreq;
     std::vector<double> omega;
-    double2 *acc;          // ncomp x nfreq x P
+    double2 *acc;          // ncomp x nfreq x nb, box cells x fastest
     int freq_off;
 };
 
@@ -197,16 +200,22 @@ __global__ __launch_bounds__(kBlock) void k_run_mon_dft(const MonDev *mons, cons
     if (threadIdx.x == 0) *M.inst = x[n - 1];
 }
 
-__global__ void k_run_dft_plane(double2 *acc, const float *base, long long Nal, long long P, long long koff, int mask,
-                                int nfreq, const double2 *tw) {
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
+// acc[slot][f][q] += field_m(cell q of the box) * tw[f] for each selected component m (FP64).
+__global__ void k_run_dft_box(double2 *acc, const float *base, long long Nal, int I, int J, int i0, int ni, int j0,
+                              int nj, int k0, long long nb, int mask, int nfreq, const double2 *tw) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb) return;
+    const long long row = q / ni;
+    const int i = i0 + (int)(q - row * ni);
+    const int j = j0 + (int)(row % nj);
+    const int k = k0 + (int)(row / nj);
+    const long long c = i + (long long)I * (j + (long long)J * k);
     int slot = 0;
     for (int m = 0; m < 6; m++) {
         if (!(mask >> m & 1)) continue;
-        const double v = base[m * Nal + koff + p];
+        const double v = base[m * Nal + c];
         for (int f = 0; f < nfreq; f++) {
-            double2 *a = acc + ((long long)slot * nfreq + f) * P + p;
+            double2 *a = acc + ((long long)slot * nfreq + f) * nb + q;
             double2 t = tw[f], x = *a;
             x.x += v * t.x;
             x.y += v * t.y;
@@ -214,6 +223,26 @@ __global__ void k_run_dft_plane(double2 *acc, const float *base, long long Nal, 
         }
         slot++;
     }
+}
+
+// grad[q] = Re[w * sum over the E components both monitors hold of F[q] * A[q]] at frequency f.
+__global__ void k_dft_gradient(double *grad, const double2 *F, int fmask, int fnf, const double2 *A, int amask,
+                               int anf, int f, long long nb, double wr, double wi) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb) return;
+    double sr = 0, si = 0;
+    int fs = 0, as = 0;
+    for (int m = 0; m < 6; m++) {
+        const bool inf = fmask >> m & 1, ina = amask >> m & 1;
+        if (m < 3 && inf && ina) {
+            const double2 x = F[((long long)fs * fnf + f) * nb + q], y = A[((long long)as * anf + f) * nb + q];
+            sr += x.x * y.x - x.y * y.y;
+            si += x.x * y.y + x.y * y.x;
+        }
+        fs += inf;
+        as += ina;
+    }
+    grad[q] = wr * sr - wi * si;
 }
 
 int mode_window(fdtd_dev *d, int n, int c, double ratio, int mn) {
@@ -383,24 +412,96 @@ int fdtd_dev_run_add_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *
     return (int)r->ovl.size() - 1;
 }
 
-int fdtd_dev_run_add_dft_plane(fdtd_dev *d, int k, int comp_mask, int nfreq, const double *omega) {
-    if (k < 0 || k >= d->K || nfreq <= 0 || (comp_mask & 63) == 0) {
-        set_error("run DFT plane: bad plane, component mask or frequency list");
+int fdtd_dev_run_add_dft_box(fdtd_dev *d, const int box[6], int comp_mask, int nfreq, const double *omega) {
+    const int N[3] = {d->I, d->J, d->K};
+    for (int a = 0; a < 3; a++)
+        if (box[2 * a] < 0 || box[2 * a + 1] > N[a] || box[2 * a] >= box[2 * a + 1]) {
+            set_error("run DFT box: empty or outside the grid");
+            return -2;
+        }
+    if (nfreq <= 0 || (comp_mask & 63) == 0) {
+        set_error("run DFT box: bad component mask or frequency list");
         return -2;
     }
     Run *r = run_of(d);
     RunDft q;
-    q.k = k;
+    for (int a = 0; a < 6; a++) q.box[a] = box[a];
+    q.nb = (long long)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
     q.mask = comp_mask & 63;
     q.ncomp = __builtin_popcount(q.mask);
     q.nfreq = nfreq;
     q.omega.assign(omega, omega + nfreq);
     q.freq_off = 0;
-    size_t n = (size_t)q.ncomp * nfreq * d->P;
+    q.acc = nullptr;
+    size_t n = (size_t)q.ncomp * nfreq * q.nb;
     FDTD_TRY(hipMalloc(&q.acc, n * sizeof(double2)));
     FDTD_TRY(hipMemset(q.acc, 0, n * sizeof(double2)));
     r->dft.push_back(std::move(q));
     return (int)r->dft.size() - 1;
+}
+
+int fdtd_dev_run_add_dft_plane(fdtd_dev *d, int k, int comp_mask, int nfreq, const double *omega) {
+    const int box[6] = {0, d->I, 0, d->J, k, k + 1};
+    return fdtd_dev_run_add_dft_box(d, box, comp_mask, nfreq, omega);
+}
+
+// Exact adjoint source of an overlap monitor (same plane, mode window and reverse rule as
+// fdtd_dev_run_add_overlap).  With the overlap's weights w (inst = w_E.E + w_H.H on the plane):
+// J = w_E = dx^2/2 (mHy', -mHx', 0) and M = -e^{-i omega dt} w_H = e^{-i omega dt} dx^2/2 (mEy, -mEx, 0),
+// mH' = -conj(mH) when reverse; step n writes Re[(J, M) amplitude[n] e^{-i phase[n]}].
+int fdtd_dev_run_add_adjoint_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj,
+                                    double mdisc, int reverse, double omega_dt, int nsteps, const double *amplitude,
+                                    const double *phase) {
+    if (cz < 0 || cz >= d->K || nsteps <= 0) {
+        set_error("run adjoint source: plane outside the grid or empty schedule");
+        return -2;
+    }
+    Run *r = run_of(d);
+    const long long P = d->P;
+    const double ratio = d->disc / mdisc, h = 0.5 * d->disc * d->disc;
+    const double zr = std::cos(omega_dt), zi = -std::sin(omega_dt);  // e^{-i omega dt}
+    std::vector<float> PR((size_t)6 * P, 0.0f), PI((size_t)6 * P, 0.0f);
+    std::vector<unsigned char> win((size_t)P, 0);
+    for (int j = 0; j < d->J; j++)
+        for (int i = 0; i < d->I; i++) {
+            int mx = mode_window(d, i, cx, ratio, mi), my = mode_window(d, j, cy, ratio, mj);
+            if (mx < 0 || mx >= mi || my < 0 || my >= mj) continue;
+            long long p = i + (long long)d->I * j;
+            win[p] = 1;
+            const double *md = mode + 12 * (mx + (long long)mi * my);
+            double hxr = md[6], hxi = md[7], hyr = md[8], hyi = md[9];
+            if (reverse == 1) {  // mH -> -conj(mH)
+                hxr = -hxr;
+                hyr = -hyr;
+            }
+            const double Jv[3][2] = {{h * hyr, h * hyi}, {-h * hxr, -h * hxi}, {0, 0}};
+            const double Mraw[3][2] = {{h * md[2], h * md[3]}, {-h * md[0], -h * md[1]}, {0, 0}};
+            for (int m = 0; m < 3; m++) {
+                PR[m * P + p] = (float)Jv[m][0];
+                PI[m * P + p] = (float)Jv[m][1];
+                PR[(3 + m) * P + p] = (float)(zr * Mraw[m][0] - zi * Mraw[m][1]);
+                PI[(3 + m) * P + p] = (float)(zr * Mraw[m][1] + zi * Mraw[m][0]);
+            }
+        }
+    std::vector<float2> coef((size_t)nsteps);
+    for (int n = 0; n < nsteps; n++)
+        coef[n] = make_float2((float)(amplitude[n] * std::cos(phase[n])), (float)(amplitude[n] * std::sin(phase[n])));
+    int err;
+    if (!plane_for(d, cz, &err)) return err;
+    RunSource s;
+    s.k = cz;
+    s.nsteps = nsteps;
+    s.coef = std::move(coef);
+    s.PR = s.PI = nullptr;
+    s.win = nullptr;
+    FDTD_TRY(hipMalloc(&s.PR, PR.size() * sizeof(float)));
+    FDTD_TRY(hipMalloc(&s.PI, PI.size() * sizeof(float)));
+    FDTD_TRY(hipMalloc(&s.win, win.size()));
+    FDTD_TRY(hipMemcpy(s.PR, PR.data(), PR.size() * sizeof(float), hipMemcpyHostToDevice));
+    FDTD_TRY(hipMemcpy(s.PI, PI.data(), PI.size() * sizeof(float), hipMemcpyHostToDevice));
+    FDTD_TRY(hipMemcpy(s.win, win.data(), win.size(), hipMemcpyHostToDevice));
+    r->src.push_back(std::move(s));
+    return (int)r->src.size() - 1;
 }
 
 }  // extern "C"
@@ -535,8 +636,10 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
             FDTD_TRY(hipGetLastError());
         }
         for (auto &q : r->dft) {
-            hipLaunchKernelGGL(k_run_dft_plane, dim3(nbp), dim3(kBlock), 0, d->stream, q.acc, d->field_base, d->Nal,
-                               d->P, (long long)q.k * d->P, q.mask, q.nfreq, r->tw + (size_t)s * nf + q.freq_off);
+            hipLaunchKernelGGL(k_run_dft_box, dim3((unsigned)((q.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               d->stream, q.acc, (const float *)d->field_base, d->Nal, d->I, d->J, q.box[0],
+                               q.box[1] - q.box[0], q.box[2], q.box[3] - q.box[2], q.box[4], q.nb, q.mask, q.nfreq,
+                               (const double2 *)(r->tw + (size_t)s * nf + q.freq_off));
             FDTD_TRY(hipGetLastError());
         }
     }
@@ -588,12 +691,43 @@ int fdtd_dev_run_overlap_result(fdtd_dev *d, int id, double *cumulative, double 
 int fdtd_dev_run_dft_result(fdtd_dev *d, int id, double *out) {
     Run *r = run_of(d);
     if (id < 0 || id >= (int)r->dft.size()) {
-        set_error("no such DFT plane");
+        set_error("no such DFT monitor");
         return -2;
     }
     FDTD_TRY(hipStreamSynchronize(d->stream));
     const RunDft &q = r->dft[id];
-    FDTD_TRY(hipMemcpy(out, q.acc, (size_t)q.ncomp * q.nfreq * d->P * sizeof(double2), hipMemcpyDeviceToHost));
+    FDTD_TRY(hipMemcpy(out, q.acc, (size_t)q.ncomp * q.nfreq * q.nb * sizeof(double2), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fdtd_dev_dft_gradient(fdtd_dev *df, int id_f, fdtd_dev *da, int id_a, int freq, double w_re, double w_im,
+                          double *grad) {
+    Run *rf = run_of(df), *ra = run_of(da);
+    if (id_f < 0 || id_f >= (int)rf->dft.size() || id_a < 0 || id_a >= (int)ra->dft.size()) {
+        set_error("DFT gradient: no such DFT monitor");
+        return -2;
+    }
+    const RunDft &F = rf->dft[id_f], &A = ra->dft[id_a];
+    for (int a = 0; a < 6; a++)
+        if (F.box[a] != A.box[a]) {
+            set_error("DFT gradient: the two monitors cover different boxes");
+            return -2;
+        }
+    if (freq < 0 || freq >= F.nfreq || freq >= A.nfreq || (F.mask & A.mask & 7) == 0) {
+        set_error("DFT gradient: frequency index out of range or no common E component");
+        return -2;
+    }
+    FDTD_TRY(hipStreamSynchronize(da->stream));
+    double *g = nullptr;
+    FDTD_TRY(hipMalloc(&g, (size_t)F.nb * sizeof(double)));
+    hipLaunchKernelGGL(k_dft_gradient, dim3((unsigned)((F.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, df->stream, g,
+                       (const double2 *)F.acc, F.mask, F.nfreq, (const double2 *)A.acc, A.mask, A.nfreq, freq, F.nb,
+                       w_re, w_im);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(df->stream);
+    if (e == hipSuccess) e = hipMemcpy(grad, g, (size_t)F.nb * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(g);
+    if (e != hipSuccess) return fail("DFT gradient", e);
     return 0;
 }
 

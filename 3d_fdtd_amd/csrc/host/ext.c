@@ -108,6 +108,25 @@ int fdtd3d_ext_run_add_dft_plane(timefield3d psi, int k, int comp_mask, int nfre
     return fdtd_dev_run_add_dft_plane(tf_dev(psi), k, comp_mask, nfreq, omega);
 }
 
+int fdtd3d_ext_run_add_dft_box(timefield3d psi, const int box[6], int comp_mask, int nfreq, const double *omega) {
+    return fdtd_dev_run_add_dft_box(tf_dev(psi), box, comp_mask, nfreq, omega);
+}
+
+int fdtd3d_ext_run_add_adjoint_source(timefield3d psi, int cx, int cy, int cz, mode m, int reverse, double omega_dt,
+                                      int nsteps, const double *amplitude, const double *phase) {
+    fdtd_dev *d = tf_dev(psi);
+    double *f = normalised_copy(m);
+    int rc = fdtd_dev_run_add_adjoint_source(d, cx, cy, cz, f, m.i, m.j, m.disc, reverse, omega_dt, nsteps, amplitude,
+                                             phase);
+    free(f);
+    return rc;
+}
+
+int fdtd3d_ext_dft_gradient(timefield3d psi_f, int id_f, timefield3d psi_a, int id_a, int freq, complex w,
+                            double *grad) {
+    return fdtd_dev_dft_gradient(tf_dev(psi_f), id_f, tf_dev(psi_a), id_a, freq, w.re, w.im, grad);
+}
+
 int fdtd3d_ext_run(timefield3d psi, double dt, int n0, int count, double t_offset) {
     return fdtd_dev_run(tf_dev(psi), dt, n0, count, t_offset);
 }

@@ -58,8 +58,9 @@ int fdtd_dev_update_E_cpml(fdtd_dev *d, double dt);
 int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt);
 /* Kernel used by fdtd_dev_step under the PML boundary: rows = 0 selects the split E / H pair
  * (kchunk > 0 fixes their z planes per thread, 0 lets each launch choose), rows = 1, 2, 4 or 8
- * the fused single-kernel step with that many y rows per lane and kchunk z planes per
- * wavefront.  Default: the split pair with automatic chunking. */
+ * the fused single-kernel step in register strips with that many y rows per lane and kchunk z
+ * planes per wavefront, rows = 16 or 32 the fused step in LDS tiles of 8 or 16 rows and kchunk
+ * z planes per block.  Default: the split pair with automatic chunking. */
 int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk);
 
 /* Device-resident run (extension).  A run holds mode sources whose per-step amplitude and
@@ -73,6 +74,21 @@ int fdtd_dev_run_add_source(fdtd_dev *d, int cx, int cy, int cz, const double *m
 int fdtd_dev_run_add_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
                              int reverse, int nfreq, const double *omega);
 int fdtd_dev_run_add_dft_plane(fdtd_dev *d, int k, int comp_mask, int nfreq, const double *omega);
+/* DFT monitor over the box [box0,box1) x [box2,box3) x [box4,box5) (i, j, k); result layout
+ * [comp][freq][k][j][i] of the box. */
+int fdtd_dev_run_add_dft_box(fdtd_dev *d, const int box[6], int comp_mask, int nfreq, const double *omega);
+/* Exact adjoint source of the overlap monitor (cx, cy, cz, mode, reverse): on the monitor plane
+ * J = dx^2/2 (mHy', -mHx', 0) and M = e^{-i omega_dt} dx^2/2 (mEy, -mEx, 0) (mH' = -conj(mH) when
+ * reverse), step n writing Re[(J, M) amplitude[n] e^{-i phase[n]}].  This is the reference's CW
+ * adjoint source (timefield3d.c:3028-3068) with the overlap's scale and the one-step phase of
+ * the magnetic part that make the discrete adjoint exact. */
+int fdtd_dev_run_add_adjoint_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj,
+                                    double mdisc, int reverse, double omega_dt, int nsteps, const double *amplitude,
+                                    const double *phase);
+/* grad[q] = Re[(w_re + i w_im) * sum_m F_m(q) A_m(q)] over the E components m both DFT monitors
+ * hold, at frequency index freq; both monitors cover the same box (q indexes it x fastest). */
+int fdtd_dev_dft_gradient(fdtd_dev *df, int id_f, fdtd_dev *da, int id_a, int freq, double w_re, double w_im,
+                          double *grad);
 int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset);
 /* Monitor results: overlap -> cumulative[2*nfreq] (and the last instantaneous value);
  * dft plane -> [comp][freq][J][I] complex interleaved. */

@@ -31,8 +31,9 @@ int fdtd3d_ext_synchronize(timefield3d psi);
 int fdtd3d_ext_kernel_timing(timefield3d psi, int every);
 int fdtd3d_ext_kernel_times(timefield3d psi, int counts[3], double ms[3]);
 /* Under the PML boundary: rows = 0 selects the split E/H kernels (kchunk > 0: z planes per
- * thread, 0: automatic), rows = 1, 2, 4 or 8 the fused one-kernel step (rows per lane, kchunk z
- * planes per wavefront).  Default: split kernels, automatic chunking. */
+ * thread, 0: automatic), rows = 1, 2, 4 or 8 the fused one-kernel step in register strips (rows
+ * per lane, kchunk z planes per wavefront), rows = 16 or 32 the fused step in LDS tiles of 8 or 16
+ * rows (kchunk z planes per block).  Default: split kernels, automatic chunking. */
 int fdtd3d_ext_set_step_kernel(timefield3d psi, int rows, int kchunk);
 
 /* Run n reference steps (timefield3d_apply_timestep_omp_full semantics) without host work in
@@ -67,8 +68,32 @@ int fdtd3d_ext_run_add_dft_plane(timefield3d psi, int k, int comp_mask, int nfre
 int fdtd3d_ext_run(timefield3d psi, double delta_t, int n0, int count, double t_offset);
 /* cumulative[nfreq] and the last instantaneous overlap (either pointer may be NULL). */
 int fdtd3d_ext_run_overlap(timefield3d psi, int id, complex *cumulative, complex *last);
-/* [selected comp][freq][j][i] complex values of a DFT plane. */
+/* DFT monitor over the box [box[0],box[1]) x [box[2],box[3]) x [box[4],box[5]) in (i, j, k); shares
+ * the id space of the DFT planes. */
+int fdtd3d_ext_run_add_dft_box(timefield3d psi, const int box[6], int comp_mask, int nfreq, const double *omega);
+/* [selected comp][freq][j][i] complex values of a DFT plane ([comp][freq][k][j][i] of a box). */
 int fdtd3d_ext_run_dft(timefield3d psi, int id, complex *out);
+
+/* DFT-domain permittivity gradient.
+ * Adjoint source of an overlap monitor a(omega) = sum_n e^{i omega (n + t_offset) dt} inst(n):
+ * fdtd3d_ext_run_add_adjoint_source places, on the monitor plane, the electric and magnetic
+ * currents of the monitor's weights, J = dx^2/2 (mHy', -mHx', 0) and
+ * M = e^{-i omega_dt} dx^2/2 (mEy, -mEx, 0) (mH' = -conj(mH) when reverse), written at step n as
+ * Re[(J, M) amplitude[n] e^{-i phase[n]}] -- the reference's CW adjoint source form
+ * (timefield3d.c:3028-3068) with the scale and phase that make the discrete adjoint exact.
+ * Forward run: the monitor and a DFT box of E with t_offset 0.  Adjoint run (fresh fields, same
+ * materials): this source with phase[n] = omega n dt and a smooth envelope amplitude[n], and a DFT
+ * box of E with t_offset 1.  Then, with u = e^{-i omega dt} - 1 and E0 = sum_n amplitude[n],
+ *   d a / d eps(cell) = 2 u / (dt E0) * sum_m F_m(cell) A_m(cell)
+ * for a change of eps on all three components of the cell (first order, fields decayed at the end
+ * of both runs, sources and monitor outside the box).  fdtd3d_ext_dft_gradient returns
+ * grad[cell] = Re[w * sum_m F_m A_m] over the box (x fastest) for a complex weight w, e.g.
+ * w = 2 conj(a) 2u/(dt E0) for d|a|^2/d eps. */
+int fdtd3d_ext_run_add_adjoint_source(timefield3d psi, int center_x, int center_y, int center_z, mode overlap_mode,
+                                      int reverse, double omega_dt, int nsteps, const double *amplitude,
+                                      const double *phase);
+int fdtd3d_ext_dft_gradient(timefield3d psi_forward, int id_forward, timefield3d psi_adjoint, int id_adjoint,
+                            int freq, complex weight, double *grad);
 
 const char *fdtd3d_ext_last_error(void);
 

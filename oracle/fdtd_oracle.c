@@ -20,6 +20,9 @@
  *   orc_taper_*                 -- fixture F6 bit-exact
  *   orc_*_cpml, orc_cpml_profile-- zero-width == orc_step interior; reflection invariant
  *   orc_dft_scalar              -- pure-tone closed form
+ *   orc_dft_box                 -- single frequency over the whole grid == orc_dft_volume (F3-pinned)
+ *   orc_overlap_adjoint_source, -- finite differences: the gradient they give matches
+ *   orc_dft_gradient               (a(eps + d) - a(eps - d)) / 2d of the overlap monitor
  */
 #include "fdtd_oracle.h"
 
@@ -778,5 +781,84 @@ void orc_dft_scalar(int nfreq, const double *omega, double t, double value, doub
         double ph = omega[q] * t;
         acc[2 * q] += value * cos(ph);
         acc[2 * q + 1] += value * sin(ph);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* extension: DFT boxes and the DFT-domain permittivity gradient                              */
+/* ------------------------------------------------------------------------------------------ */
+
+void orc_dft_box(const orc_grid *g, const int box[6], int mask, int nfreq, const double *omega, double t,
+                 double *acc) {
+    const long ni = box[1] - box[0], nj = box[3] - box[2], nk = box[5] - box[4], nb = ni * nj * nk;
+    for (int f = 0; f < nfreq; f++) {
+        const double tr = cos(omega[f] * t), ti = sin(omega[f] * t);
+        int slot = 0;
+        for (int m = 0; m < 6; m++) {
+            if (!(mask >> m & 1)) continue;
+            double *a = acc + 2 * ((long)slot * nfreq + f) * nb;
+            for (long q = 0; q < nb; q++) {
+                const int i = box[0] + (int)(q % ni), j = box[2] + (int)((q / ni) % nj), k = box[4] + (int)(q / (ni * nj));
+                const double v = g->f[6 * cidx(g, i, j, k) + m];
+                a[2 * q] += v * tr;
+                a[2 * q + 1] += v * ti;
+            }
+            slot++;
+        }
+    }
+}
+
+/* The overlap inst = -dx^2/2 sum(Hx mEy - Hy mEx + Ey mHx' - Ex mHy') is w_E.E + w_H.H with
+ * w_E = dx^2/2 (mHy', -mHx', 0), w_H = -dx^2/2 (mEy, -mEx, 0).  In the frequency domain of the
+ * semi-implicit scheme (z = e^{i omega dt}, u = 1/z - 1) the monitor reads g.E with
+ * g = w_E - (dt/(z u)) C_H mu^-1 w_H, and a magnetic source M acts on E like the electric source
+ * (dt/u) C_H mu^-1 M.  The adjoint source J = w_E, M = -w_H / z therefore drives g, the field it
+ * radiates equals (by the symmetry of the discrete operator) the response of the monitor to a unit
+ * current at every cell, and the derivative of the monitor with respect to eps follows from it
+ * (see orc_dft_gradient). */
+void orc_overlap_adjoint_source(const orc_grid *g, int cx, int cy, int cz, int mi, int mj, double mdisc,
+                                const double *mode, int reverse, double omega_dt, double amplitude, double phase) {
+    const double ratio = g->disc / mdisc, h = 0.5 * g->disc * g->disc;
+    const double zr = cos(omega_dt), zi = -sin(omega_dt);
+    const double cp = cos(phase), sp = sin(phase);
+    for (int i = 0; i < g->I; i++) {
+        for (int j = 0; j < g->J; j++) {
+            int mx = mode_index_1d(i, cx, ratio, mi);
+            int my = mode_index_1d(j, cy, ratio, mj);
+            if (mx < 0 || mx >= mi || my < 0 || my >= mj) continue;
+            const long c = cidx(g, i, j, cz);
+            const double *md = mode + 12L * (mx + (long)mi * my);
+            double hxr = md[6], hxi = md[7], hyr = md[8], hyi = md[9];
+            if (reverse == 1) {
+                hxr = -hxr;
+                hyr = -hyr;
+            }
+            const double J[3][2] = {{h * hyr, h * hyi}, {-h * hxr, -h * hxi}, {0, 0}};
+            const double Mr[3][2] = {{h * md[2], h * md[3]}, {-h * md[0], -h * md[1]}, {0, 0}};
+            for (int m = 0; m < 3; m++) {
+                const double Mre = zr * Mr[m][0] - zi * Mr[m][1], Mim = zr * Mr[m][1] + zi * Mr[m][0];
+                g->src[6 * c + m] = (float)(amplitude * (J[m][0] * cp + J[m][1] * sp));
+                g->src[6 * c + 3 + m] = (float)(amplitude * (Mre * cp + Mim * sp));
+            }
+        }
+    }
+}
+
+void orc_dft_gradient(long nb, int fmask, int fnf, const double *F, int amask, int anf, const double *A, int f,
+                      double wr, double wi, double *grad) {
+    for (long q = 0; q < nb; q++) {
+        double sr = 0, si = 0;
+        int fs = 0, as = 0;
+        for (int m = 0; m < 6; m++) {
+            const int inf = fmask >> m & 1, ina = amask >> m & 1;
+            if (m < 3 && inf && ina) {
+                const double *x = F + 2 * (((long)fs * fnf + f) * nb + q), *y = A + 2 * (((long)as * anf + f) * nb + q);
+                sr += x[0] * y[0] - x[1] * y[1];
+                si += x[0] * y[1] + x[1] * y[0];
+            }
+            fs += inf;
+            as += ina;
+        }
+        grad[q] = wr * sr - wi * si;
     }
 }

@@ -103,6 +103,20 @@ void orc_step_cpml(const orc_grid *g, const orc_cpml *c, double dt);
 /* ---- extension: multi-frequency running DFT of a scalar and of a plane ---- */
 void orc_dft_scalar(int nfreq, const double *omega, double t, double value, double *acc);
 
+/* ---- extension: DFT boxes and the DFT-domain permittivity gradient ---- */
+/* acc[slot][f][q] += f_m(cell q) e^{i omega_f t} for each component m in mask (slot = rank of m),
+ * box [b0,b1) x [b2,b3) x [b4,b5), q x fastest; acc interleaved complex. */
+void orc_dft_box(const orc_grid *g, const int box[6], int mask, int nfreq, const double *omega, double t,
+                 double *acc);
+/* Exact adjoint source of an overlap monitor (mode already power-normalised): writes plane cz of
+ * the sources with Re[(J, M) amplitude e^{-i phase}], J = dx^2/2 (mHy', -mHx', 0),
+ * M = e^{-i omega_dt} dx^2/2 (mEy, -mEx, 0), mH' = -conj(mH) when reverse. */
+void orc_overlap_adjoint_source(const orc_grid *g, int cx, int cy, int cz, int mi, int mj, double mdisc,
+                                const double *mode, int reverse, double omega_dt, double amplitude, double phase);
+/* grad[q] = Re[(wr + i wi) sum_{m<3 in both masks} F_m(q) A_m(q)] at frequency index f. */
+void orc_dft_gradient(long nb, int fmask, int fnf, const double *F, int amask, int anf, const double *A, int f,
+                      double wr, double wi, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,9 @@ def lib():
         "orc_update_H_cpml": ([G, C.POINTER(OrcCpml), d], None),
         "orc_step_cpml": ([G, C.POINTER(OrcCpml), d], None),
         "orc_dft_scalar": ([i, _dp, d, d, _dp], None),
+        "orc_dft_box": ([G, _ip, i, i, _dp, d, _dp], None),
+        "orc_overlap_adjoint_source": ([G, i, i, i, i, i, d, _dp, i, d, d, d], None),
+        "orc_dft_gradient": ([C.c_long, i, i, _dp, i, i, _dp, i, d, d, _dp], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
