@@ -154,6 +154,28 @@ SIGNATURES = {
     "fdtd3d_ext_run_add_dft_box": ([TimeField, _P(_i), _i, _i, _P(_d)], _i),
     "fdtd3d_ext_run_add_adjoint_source": ([TimeField, _i, _i, _i, Mode, _i, _d, _i, _P(_d), _P(_d)], _i),
     "fdtd3d_ext_dft_gradient": ([TimeField, _i, TimeField, _i, _i, Complex, _P(_d)], _i),
+    # device-level C-ABI (include/fdtd3d_dev.h): grids without host mirrors, batched runs
+    "fdtd_dev_last_error": ([], C.c_char_p),
+    "fdtd_dev_create": ([_i, _i, _i, _d], C.c_void_p),
+    "fdtd_dev_destroy": ([C.c_void_p], None),
+    "fdtd_dev_set_materials": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float)], _i),
+    "fdtd_dev_get_fields": ([C.c_void_p, _P(_d)], _i),
+    "fdtd_dev_cpml_on": ([C.c_void_p, _P(_i), _i, _d, _d, _d], _i),
+    "fdtd_dev_set_step_kernel": ([C.c_void_p, _i, _i], _i),
+    "fdtd_dev_run_add_source": ([C.c_void_p, _i, _i, _i, _P(_d), _i, _i, _d, _i, _i, _i, _P(_d), _P(_d)], _i),
+    "fdtd_dev_run_add_overlap": ([C.c_void_p, _i, _i, _i, _P(_d), _i, _i, _d, _i, _i, _P(_d)], _i),
+    "fdtd_dev_run": ([C.c_void_p, _d, _i, _i, _d], _i),
+    "fdtd_dev_run_overlap_result": ([C.c_void_p, _i, _P(_d), _P(_d)], _i),
+    "fdtd_dev_synchronize": ([C.c_void_p], _i),
+    "fdtd_dev_timer_start": ([C.c_void_p], _i),
+    "fdtd_dev_timer_stop": ([C.c_void_p, _P(_d)], _i),
+    "fdtd_dev_batch_new": ([_P(C.c_void_p), _i], C.c_void_p),
+    "fdtd_dev_batch_free": ([C.c_void_p], None),
+    "fdtd_dev_batch_set_kchunk": ([C.c_void_p, _i], _i),
+    "fdtd_dev_batch_run": ([C.c_void_p, _d, _i, _i], _i),
+    "fdtd_dev_batch_info": ([C.c_void_p, _P(_i), _P(C.c_longlong), _P(_i), _P(_i), _P(_d)], _i),
+    "fdtd_dev_batch_timer_start": ([C.c_void_p], _i),
+    "fdtd_dev_batch_timer_stop": ([C.c_void_p, _P(_d)], _i),
 }
 
 _lib = None
@@ -238,3 +260,107 @@ class Grid:
     def fields(self):
         self.pull()
         return self.a["fields"]
+
+
+def normalised_mode(mi, mj, disc, values):
+    """Power-normalised copy (mode_power_normalize, mode.c semantics) of an interleaved mode array."""
+    md = mode_from_array(mi, mj, disc, values)
+    lib().mode_power_normalize(md)
+    out = mode_fields(md).copy()
+    lib().mode_free(md)
+    return out
+
+
+def dev_check(rc, what="fdtd_dev call"):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {lib().fdtd_dev_last_error().decode()}")
+
+
+class DevGrid:
+    """A device-only grid (fdtd_dev, no host mirrors): the unit of batched runs."""
+
+    def __init__(self, I, J, K, disc, ep, mu=None, sig=None):
+        n = I * J * K
+        self.I, self.J, self.K, self.disc = I, J, K, disc
+        self.d = lib().fdtd_dev_create(I, J, K, disc)
+        if not self.d:
+            raise RuntimeError("fdtd_dev_create: " + lib().fdtd_dev_last_error().decode())
+        ep = np.ascontiguousarray(ep, np.float32)
+        mu = np.ones(3 * n, np.float32) if mu is None else np.ascontiguousarray(mu, np.float32)
+        sig = np.zeros(6 * n, np.float32) if sig is None else np.ascontiguousarray(sig, np.float32)
+        fp = C.POINTER(C.c_float)
+        dev_check(lib().fdtd_dev_set_materials(self.d, ep.ctypes.data_as(fp), mu.ctypes.data_as(fp),
+                                               sig.ctypes.data_as(fp)), "set_materials")
+
+    def cpml(self, npml, order=3, R0=1e-8, kappa_max=1.0, alpha_max=0.05):
+        dev_check(lib().fdtd_dev_cpml_on(self.d, (C.c_int * 6)(*npml), order, R0, kappa_max, alpha_max), "cpml_on")
+
+    def add_source(self, cx, cy, cz, mode, mi, mj, mdisc, amplitude, phase, reverse=0, adjoint=0):
+        dp = C.POINTER(C.c_double)
+        amplitude = np.ascontiguousarray(amplitude, np.float64)
+        phase = np.ascontiguousarray(phase, np.float64)
+        mode = np.ascontiguousarray(mode, np.float64)
+        sid = lib().fdtd_dev_run_add_source(self.d, cx, cy, cz, mode.ctypes.data_as(dp), mi, mj, mdisc, reverse, adjoint,
+                                            len(amplitude), amplitude.ctypes.data_as(dp), phase.ctypes.data_as(dp))
+        dev_check(0 if sid >= 0 else sid, "run_add_source")
+        return sid
+
+    def add_overlap(self, cx, cy, cz, mode, mi, mj, mdisc, omegas, reverse=0):
+        dp = C.POINTER(C.c_double)
+        omegas = np.ascontiguousarray(omegas, np.float64)
+        mode = np.ascontiguousarray(mode, np.float64)
+        mid = lib().fdtd_dev_run_add_overlap(self.d, cx, cy, cz, mode.ctypes.data_as(dp), mi, mj, mdisc, reverse,
+                                             len(omegas), omegas.ctypes.data_as(dp))
+        dev_check(0 if mid >= 0 else mid, "run_add_overlap")
+        return mid
+
+    def run(self, dt, n0, n, t_offset=0.0):
+        dev_check(lib().fdtd_dev_run(self.d, dt, n0, n, t_offset), "run")
+
+    def overlap(self, mid, nfreq):
+        cum = np.zeros(2 * nfreq)
+        last = np.zeros(2)
+        dp = C.POINTER(C.c_double)
+        dev_check(lib().fdtd_dev_run_overlap_result(self.d, mid, cum.ctypes.data_as(dp), last.ctypes.data_as(dp)),
+                  "overlap result")
+        return cum[0::2] + 1j * cum[1::2], complex(last[0], last[1])
+
+    def fields(self):
+        out = np.zeros(6 * self.I * self.J * self.K)
+        dev_check(lib().fdtd_dev_get_fields(self.d, out.ctypes.data_as(C.POINTER(C.c_double))), "get_fields")
+        return out
+
+    def free(self):
+        if self.d:
+            lib().fdtd_dev_destroy(self.d)
+            self.d = None
+
+
+class Batch:
+    """Lockstep batched run over DevGrids (fdtd_dev_batch_*)."""
+
+    def __init__(self, grids):
+        self.grids = list(grids)
+        arr = (C.c_void_p * len(self.grids))(*[g.d for g in self.grids])
+        self.b = lib().fdtd_dev_batch_new(arr, len(self.grids))
+        if not self.b:
+            raise RuntimeError("fdtd_dev_batch_new: " + lib().fdtd_dev_last_error().decode())
+
+    def run(self, dt, n0, n):
+        dev_check(lib().fdtd_dev_batch_run(self.b, dt, n0, n), "batch_run")
+
+    def set_kchunk(self, kc):
+        dev_check(lib().fdtd_dev_batch_set_kchunk(self.b, kc), "batch_set_kchunk")
+
+    def info(self):
+        kc, nt = C.c_int(), C.c_int()
+        cells = C.c_longlong()
+        tk, tm = (C.c_int * 8)(), (C.c_double * 8)()
+        dev_check(lib().fdtd_dev_batch_info(self.b, C.byref(kc), C.byref(cells), C.byref(nt), tk, tm), "batch_info")
+        return {"kchunk": kc.value, "cells": cells.value,
+                "tuned": {int(tk[q]): float(tm[q]) for q in range(nt.value)}}
+
+    def free(self):
+        if self.b:
+            lib().fdtd_dev_batch_free(self.b)
+            self.b = None

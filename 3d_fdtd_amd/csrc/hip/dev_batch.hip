@@ -1,0 +1,328 @@
+// Batched multi-geometry runs: many grids of uneven sizes (e.g. taper candidates of different
+// lengths) advance in lockstep, one E launch and one H launch per step covering every grid.
+//
+// Each grid keeps its own device state (fdtd_dev) and run configuration (tabulated sources,
+// overlap monitors).  A device table holds one descriptor per grid (the split-kernel arguments
+// plus the grid's first virtual block); the launches are persistent grid-stride kernels over the
+// concatenated virtual blocks of all grids, remapped so that each XCD walks a contiguous range
+// (consecutive z chunks of one grid share that XCD's L2).  Per-step values (source amplitudes,
+// monitor partial slots) are read through a device step counter, so a segment of steps (E, H,
+// counter tick) is captured once in a hipGraph and replayed.  The z-chunk depth of the launches
+// is picked by a small autotuner that times real steps of the run with each candidate.
+#include <cstring>
+#include <vector>
+
+#include "dev_stencil.h"
+
+namespace fdtd {
+namespace {
+
+struct BatchGeom {
+    YeeArgs a;
+    PmlArgs pm;
+    long long vb0;  // first virtual block of this grid
+};
+
+// The grid holding virtual block vb (descriptors sorted by vb0).
+__device__ __forceinline__ int batch_grid(const BatchGeom *__restrict__ geo, int G, long long vb) {
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (geo[mid].vb0 <= vb) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename MatT, int KC>
+__global__ __launch_bounds__(kBlock) void k_batch_E(const BatchGeom *__restrict__ geo, int G, long long total) {
+    for (long long vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const long long lb = xcd_logical_block_ll(vb, total);
+        const BatchGeom &B = geo[batch_grid(geo, G, lb)];
+        update_E_block<MatT, true, KC>(B.a, B.pm, (int)(lb - B.vb0));
+    }
+}
+
+template <typename MatT, bool MON, int KC>
+__global__ __launch_bounds__(kBlock) void k_batch_H(const BatchGeom *__restrict__ geo, int G, long long total) {
+    for (long long vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const long long lb = xcd_logical_block_ll(vb, total);
+        const BatchGeom &B = geo[batch_grid(geo, G, lb)];
+        update_H_block<MatT, true, MON, KC>(B.a, B.pm, (int)(lb - B.vb0));
+        if (MON) __syncthreads();  // the next virtual block reuses the reduction slots
+    }
+}
+
+__global__ void k_step_tick(int *step) {
+    if (threadIdx.x == 0) *step += 1;
+}
+
+}  // namespace
+}  // namespace fdtd
+
+using namespace fdtd;
+
+struct fdtd_dev_batch {
+    std::vector<fdtd_dev *> grids;
+    hipStream_t stream = nullptr;
+    int *step = nullptr;        // device step counter
+    BatchGeom *geo = nullptr;   // device descriptor table
+    std::vector<BatchGeom> host;
+    long long total = 0;        // virtual blocks of the current chunk depth
+    int kc = 0;                 // z planes per thread (0 = not tuned yet: the run tunes it)
+    bool u8 = true, mon = false;
+    unsigned grid_blocks = 0;
+    // captured segment of kSeg steps
+    hipGraphExec_t exec = nullptr;
+    int exec_kc = -1;
+    bool exec_mon = false;
+    long long exec_total = -1;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double tuned_ms[8] = {};
+    int tuned_kc[8] = {};
+    int ntuned = 0;
+};
+
+namespace {
+
+constexpr int kSeg = 32;           // steps per captured graph segment
+constexpr int kBatchChunk = 4096;  // steps per prepare/finish cycle (bounds the monitor buffers)
+const int kTuneKc[] = {2, 3, 4};
+
+long long batch_layout(fdtd_dev_batch *b, int kc) {
+    long long vb = 0;
+    for (auto &g : b->host) {
+        g.a.kchunk = kc;
+        const int nz = g.a.k1 - g.a.k0;
+        const long long zchunks = (nz + kc - 1) / kc;
+        g.vb0 = vb;
+        vb += (long long)g.a.plane_blocks * zchunks;
+    }
+    return vb;
+}
+
+int upload_layout(fdtd_dev_batch *b, int kc) {
+    b->total = batch_layout(b, kc);
+    FDTD_TRY(hipMemcpyAsync(b->geo, b->host.data(), b->host.size() * sizeof(BatchGeom), hipMemcpyHostToDevice,
+                            b->stream));
+    FDTD_TRY(hipStreamSynchronize(b->stream));  // the host table is rewritten by the next layout
+    // persistent launch: up to 8 blocks of 256 threads per CU on 256 CUs, a multiple of 8 (XCDs)
+    long long g = b->total < 2048 ? b->total : 2048;
+    g = (g + 7) / 8 * 8;
+    b->grid_blocks = (unsigned)(g < 8 ? 8 : g);
+    return 0;
+}
+
+template <int KC>
+void launch_step_kc(fdtd_dev_batch *b) {
+    const int G = (int)b->host.size();
+    const dim3 grid(b->grid_blocks), block(kBlock);
+    if (b->u8) {
+        hipLaunchKernelGGL((k_batch_E<uint8_t, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo, G, b->total);
+        if (b->mon)
+            hipLaunchKernelGGL((k_batch_H<uint8_t, true, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo, G,
+                               b->total);
+        else
+            hipLaunchKernelGGL((k_batch_H<uint8_t, false, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo, G,
+                               b->total);
+    } else {
+        hipLaunchKernelGGL((k_batch_E<uint16_t, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo, G,
+                           b->total);
+        if (b->mon)
+            hipLaunchKernelGGL((k_batch_H<uint16_t, true, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo,
+                               G, b->total);
+        else
+            hipLaunchKernelGGL((k_batch_H<uint16_t, false, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo,
+                               G, b->total);
+    }
+    hipLaunchKernelGGL(k_step_tick, dim3(1), dim3(64), 0, b->stream, b->step);
+}
+
+int launch_step(fdtd_dev_batch *b, int kc) {
+    switch (kc) {
+        case 2: launch_step_kc<2>(b); break;
+        case 3: launch_step_kc<3>(b); break;
+        case 4: launch_step_kc<4>(b); break;
+        default: launch_step_kc<0>(b); break;
+    }
+    FDTD_TRY(hipGetLastError());
+    return 0;
+}
+
+// The captured segment for the current layout (re-captured when the kernel choice changes).
+int segment_graph(fdtd_dev_batch *b) {
+    if (b->exec && b->exec_kc == b->kc && b->exec_mon == b->mon && b->exec_total == b->total) return 0;
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    FDTD_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int s = 0; s < kSeg && rc == 0; s++) rc = launch_step(b, b->kc);
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail("hipStreamEndCapture", e);
+    e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail("hipGraphInstantiate", e);
+    b->exec_kc = b->kc;
+    b->exec_mon = b->mon;
+    b->exec_total = b->total;
+    return 0;
+}
+
+int set_step(fdtd_dev_batch *b, int n0) {
+    FDTD_TRY(hipMemcpyAsync(b->step, &n0, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    FDTD_TRY(hipStreamSynchronize(b->stream));  // n0 lives on the host stack
+    return 0;
+}
+
+// Steps n0..n0+n-1 of every grid (n <= kBatchChunk); the first call of an untuned batch spends its
+// first steps timing each chunk-depth candidate (real steps: results do not depend on the depth).
+int batch_chunk(fdtd_dev_batch *b, double dt, int n0, int n) {
+    for (auto *d : b->grids) FDTD_TRY(hipStreamSynchronize(d->stream));
+    b->mon = false;
+    for (size_t g = 0; g < b->grids.size(); g++) {
+        int rc = run_batch_prepare(b->grids[g], dt, n0, n, b->step, &b->host[g].a, &b->host[g].pm);
+        if (rc) return rc;
+        b->mon = b->mon || b->host[g].a.nmon > 0;
+    }
+    int rc = set_step(b, n0);
+    if (rc) return rc;
+    int done = 0;
+    if (b->kc == 0) {
+        const int per = 4;  // timed steps per candidate (after one untimed step)
+        int best = 2;
+        double best_ms = 1e30;
+        b->ntuned = 0;
+        for (int kc : kTuneKc) {
+            if (done + per + 1 > n) break;
+            if ((rc = upload_layout(b, kc))) return rc;
+            if ((rc = launch_step(b, kc))) return rc;
+            FDTD_TRY(hipEventRecord(b->ev0, b->stream));
+            for (int s = 0; s < per; s++)
+                if ((rc = launch_step(b, kc))) return rc;
+            FDTD_TRY(hipEventRecord(b->ev1, b->stream));
+            FDTD_TRY(hipEventSynchronize(b->ev1));
+            float ms = 0;
+            FDTD_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            b->tuned_kc[b->ntuned] = kc;
+            b->tuned_ms[b->ntuned++] = ms / per;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = kc;
+            }
+            done += per + 1;
+        }
+        if (b->ntuned > 0) b->kc = best;
+        else b->kc = 2;
+    }
+    if ((rc = upload_layout(b, b->kc))) return rc;
+    const int left = n - done;
+    const int segs = left / kSeg;
+    if (segs > 0) {
+        if ((rc = segment_graph(b))) return rc;
+        for (int s = 0; s < segs; s++) FDTD_TRY(hipGraphLaunch(b->exec, b->stream));
+    }
+    for (int s = segs * kSeg; s < left; s++)
+        if ((rc = launch_step(b, b->kc))) return rc;
+    for (auto *d : b->grids)
+        if ((rc = run_batch_finish(d, n0, n, b->stream))) return rc;
+    FDTD_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n) {
+    if (n <= 0) {
+        set_error("batch: no grids");
+        return nullptr;
+    }
+    for (int g = 0; g < n; g++)
+        if (!grids[g] || grids[g]->mat_bytes != grids[0]->mat_bytes) {
+            set_error("batch: null grid or grids with different material-id widths");
+            return nullptr;
+        }
+    fdtd_dev_batch *b = new fdtd_dev_batch();
+    b->grids.assign(grids, grids + n);
+    b->host.resize((size_t)n);
+    std::memset((void *)b->host.data(), 0, b->host.size() * sizeof(BatchGeom));
+    b->u8 = grids[0]->mat_bytes == 1;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&b->step, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&b->geo, (size_t)n * sizeof(BatchGeom));
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        fail("batch create", e);
+        fdtd_dev_batch_free(b);
+        return nullptr;
+    }
+    return b;
+}
+
+void fdtd_dev_batch_free(fdtd_dev_batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    (void)hipFree(b->step);
+    (void)hipFree(b->geo);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int fdtd_dev_batch_set_kchunk(fdtd_dev_batch *b, int kchunk) {
+    if (kchunk < 0) {
+        set_error("batch: z chunk must be >= 0 (0 = tune on the next run)");
+        return -2;
+    }
+    b->kc = kchunk;
+    return 0;
+}
+
+int fdtd_dev_batch_run(fdtd_dev_batch *b, double dt, int n0, int n) {
+    for (int s = 0; s < n; s += kBatchChunk) {
+        const int rc = batch_chunk(b, dt, n0 + s, n - s < kBatchChunk ? n - s : kBatchChunk);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int fdtd_dev_batch_info(fdtd_dev_batch *b, int *kchunk, long long *cells, int *ntuned, int tuned_kc[8],
+                        double tuned_ms[8]) {
+    long long c = 0;
+    for (auto *d : b->grids) c += d->N;
+    if (kchunk) *kchunk = b->kc;
+    if (cells) *cells = c;
+    if (ntuned) *ntuned = b->ntuned;
+    for (int q = 0; q < b->ntuned && q < 8; q++) {
+        if (tuned_kc) tuned_kc[q] = b->tuned_kc[q];
+        if (tuned_ms) tuned_ms[q] = b->tuned_ms[q];
+    }
+    return 0;
+}
+
+int fdtd_dev_batch_timer_start(fdtd_dev_batch *b) {
+    FDTD_TRY(hipEventRecord(b->ev0, b->stream));
+    return 0;
+}
+
+int fdtd_dev_batch_timer_stop(fdtd_dev_batch *b, double *elapsed_ms) {
+    FDTD_TRY(hipEventRecord(b->ev1, b->stream));
+    FDTD_TRY(hipEventSynchronize(b->ev1));
+    float ms = 0;
+    FDTD_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    *elapsed_ms = ms;
+    return 0;
+}
+
+}  // extern "C"

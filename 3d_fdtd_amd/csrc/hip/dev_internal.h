@@ -42,6 +42,10 @@ struct RunSrcArg {
     const float *PR, *PI;
     const unsigned char *win;
     float c1, c2;
+    // when tab is set (graph-captured batch steps), (c1, c2) = tab[step] for 0 <= step < ntab,
+    // else 0, with step read from the launch's device step counter
+    const float2 *tab;
+    int ntab;
 };
 
 // An overlap monitor evaluated inside the H update of a device-resident run: each block of the
@@ -52,7 +56,8 @@ constexpr int kMaxRunMon = 4;
 struct RunMonArg {
     int k;
     const float *W;
-    double2 *part;  // this step's [plane blocks] partials
+    double2 *part;     // this step's [plane blocks] partials
+    long long stride;  // with a device step counter: part + (step - step0) * stride is the step's row
 };
 
 struct SrcPlane {
@@ -202,6 +207,10 @@ const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, do
 // (workgroups are dispatched round-robin over the 8 XCDs of an MI355X).
 __device__ __forceinline__ int xcd_logical_block(int bid, int nb) {
     int x = bid & 7, s = bid >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + s;
+}
+__device__ __forceinline__ long long xcd_logical_block_ll(long long bid, long long nb) {
+    long long x = bid & 7, s = bid >> 3, q = nb >> 3, r = nb & 7;
     return x * q + (x < r ? x : r) + s;
 }
 

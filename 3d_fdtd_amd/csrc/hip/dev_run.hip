@@ -6,7 +6,7 @@
 #include <cstring>
 #include <vector>
 
-#include "dev_internal.h"
+#include "dev_stencil.h"
 
 namespace fdtd {
 namespace {
@@ -17,6 +17,7 @@ struct RunSource {
     unsigned char *win;    // P, 1 inside the mode window
     std::vector<float2> coef;  // nsteps x (A cos(phase), A sin(phase)), passed as kernel arguments
     int nsteps;
+    float2 *dcoef = nullptr;   // device copy of coef for batched (step-counter) launches
 };
 
 struct RunOverlap {
@@ -260,6 +261,7 @@ void run_release(fdtd_dev *d) {
         (void)hipFree(s.PR);
         (void)hipFree(s.PI);
         (void)hipFree(s.win);
+        (void)hipFree(s.dcoef);
     }
     for (auto &o : r->ovl) {
         (void)hipFree(o.W);
@@ -603,7 +605,7 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
         for (int q = 0; q < ninl; q++) {
             const RunSource &rs = r->src[q];
             const float2 c = coef_at(rs, step);
-            d->run_src[q] = RunSrcArg{rs.k, rs.PR, rs.PI, rs.win, c.x, c.y};
+            d->run_src[q] = RunSrcArg{rs.k, rs.PR, rs.PI, rs.win, c.x, c.y, nullptr, 0};
         }
         for (size_t q = ninl; q < r->src.size(); q++) {
             const RunSource &rs = r->src[q];
@@ -615,7 +617,7 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
             d->n_run_mon = nmon;
             for (int m = 0; m < nmon; m++)
                 d->run_mon[m] = RunMonArg{r->ovl[m].k, r->ovl[m].W,
-                                          r->mpart + ((size_t)m * r->mstep_cap + s) * nbp};
+                                          r->mpart + ((size_t)m * r->mstep_cap + s) * nbp, 0};
         }
         int rc = fdtd_dev_step(d, dt, 0);
         d->n_run_src = 0;
@@ -661,6 +663,132 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     return 0;
 }
 
+// Twiddles e^{i omega (step + t_offset) dt} of steps n0..n0+n-1 for every monitor frequency of the
+// run (overlaps first, then DFT boxes), uploaded to r->tw; returns the row length.
+int run_twiddles(fdtd_dev *d, Run *r, double dt, int n0, int n, double t_offset, int *nf_out) {
+    int nf = 0;
+    for (auto &o : r->ovl) {
+        o.freq_off = nf;
+        nf += o.nfreq;
+    }
+    for (auto &q : r->dft) {
+        q.freq_off = nf;
+        nf += q.nfreq;
+    }
+    *nf_out = nf;
+    if (nf == 0) return 0;
+    std::vector<double2> tw((size_t)n * nf);
+    for (int s = 0; s < n; s++) {
+        double t = ((double)(n0 + s) + t_offset) * dt;
+        int f = 0;
+        for (auto &o : r->ovl)
+            for (double w : o.omega) tw[(size_t)s * nf + f++] = make_double2(std::cos(w * t), std::sin(w * t));
+        for (auto &q : r->dft)
+            for (double w : q.omega) tw[(size_t)s * nf + f++] = make_double2(std::cos(w * t), std::sin(w * t));
+    }
+    if ((long long)tw.size() > r->tw_cap) {
+        FDTD_TRY(hipStreamSynchronize(d->stream));
+        (void)hipFree(r->tw);
+        r->tw = nullptr;
+        FDTD_TRY(hipMalloc(&r->tw, tw.size() * sizeof(double2)));
+        r->tw_cap = (long long)tw.size();
+    }
+    FDTD_TRY(hipMemcpyAsync(r->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, d->stream));
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+}  // namespace
+
+int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, YeeArgs *a, PmlArgs *pm) {
+    Run *r = run_of(d);
+    if (!d->cpml.on || !step_uses_split(d)) {
+        set_error("batched runs need grids under the PML boundary with the split step kernels");
+        return -2;
+    }
+    if (r->src.size() > (size_t)kMaxRunSrc || r->ovl.size() > (size_t)kMaxRunMon || !r->dft.empty()) {
+        set_error("batched runs take at most 2 run sources and 4 overlap monitors per grid, and no DFT monitors");
+        return -2;
+    }
+    for (auto &o : r->ovl)
+        if (!o.ring_free) {
+            set_error("batched runs need overlap monitors clear of the boundary ring");
+            return -2;
+        }
+    int nf = 0, rc = run_twiddles(d, r, dt, n0, n, 0.0, &nf);
+    if (rc) return rc;
+    const unsigned nbp = (unsigned)((d->P + kBlock - 1) / kBlock);
+    const int nmon = (int)r->ovl.size();
+    if (nmon > 0) {
+        std::vector<MonDev> md((size_t)nmon);
+        for (int m = 0; m < nmon; m++) {
+            const RunOverlap &o = r->ovl[m];
+            md[m] = MonDev{(long long)o.k * d->P, o.W, o.inst, o.acc, o.nfreq, o.freq_off};
+        }
+        if (nmon > r->mon_cap) {
+            (void)hipFree(r->mons);
+            r->mons = nullptr;
+            FDTD_TRY(hipMalloc(&r->mons, (size_t)nmon * sizeof(MonDev)));
+            r->mon_cap = nmon;
+        }
+        FDTD_TRY(hipMemcpy(r->mons, md.data(), md.size() * sizeof(MonDev), hipMemcpyHostToDevice));
+        if (n > r->mstep_cap) {
+            (void)hipFree(r->mpart);
+            (void)hipFree(r->minst);
+            r->mpart = r->minst = nullptr;
+            FDTD_TRY(hipMalloc(&r->mpart, (size_t)kMaxRunMon * n * nbp * sizeof(double2)));
+            FDTD_TRY(hipMalloc(&r->minst, (size_t)kMaxRunMon * n * sizeof(double2)));
+            r->mstep_cap = n;
+        }
+    }
+    for (auto &s : r->src) {
+        int err;
+        if (!plane_for(d, s.k, &err)) return err;
+        if (!s.dcoef) {
+            FDTD_TRY(hipMalloc(&s.dcoef, s.coef.size() * sizeof(float2)));
+            FDTD_TRY(hipMemcpy(s.dcoef, s.coef.data(), s.coef.size() * sizeof(float2), hipMemcpyHostToDevice));
+        }
+    }
+    if ((rc = fill_step_args(d, dt, a, pm))) return rc;
+    a->nrs = (int)r->src.size();
+    for (int q = 0; q < a->nrs; q++) {
+        const RunSource &s = r->src[q];
+        a->rs[q] = RunSrcArg{s.k, s.PR, s.PI, s.win, 0.0f, 0.0f, s.dcoef, s.nsteps};
+    }
+    a->nmon = nmon;
+    for (int m = 0; m < nmon; m++)
+        a->mon[m] = RunMonArg{r->ovl[m].k, r->ovl[m].W, r->mpart + (size_t)m * r->mstep_cap * nbp, (long long)nbp};
+    a->step = step;
+    a->step0 = n0;
+    return 0;
+}
+
+int run_batch_finish(fdtd_dev *d, int n0, int n, hipStream_t st) {
+    Run *r = run_of(d);
+    const unsigned nbp = (unsigned)((d->P + kBlock - 1) / kBlock);
+    const int nmon = (int)r->ovl.size();
+    int nf = 0;
+    for (auto &o : r->ovl) nf += o.nfreq;
+    if (nmon > 0) {
+        hipLaunchKernelGGL(k_run_mon_steps, dim3((n + kBlock - 1) / kBlock, nmon), dim3(kBlock), 0, st,
+                           (const double2 *)r->mpart, r->minst, n, (int)nbp, r->mstep_cap, -0.5 * d->disc * d->disc);
+        hipLaunchKernelGGL(k_run_mon_dft, dim3(nmon), dim3(kBlock), 0, st, (const MonDev *)r->mons,
+                           (const double2 *)r->minst, n, r->mstep_cap, (const double2 *)r->tw, nf);
+        FDTD_TRY(hipGetLastError());
+    }
+    for (auto &s : r->src) {
+        int err;
+        SrcPlane *sp = plane_for(d, s.k, &err);
+        if (!sp) return err;
+        const int last = n0 + n - 1;
+        const float2 c = last >= 0 && last < s.nsteps ? s.coef[last] : make_float2(0.0f, 0.0f);
+        hipLaunchKernelGGL(k_run_source, dim3(nbp), dim3(kBlock), 0, st, sp->J, sp->stride, s.PR, s.PI, s.win, d->P, c);
+        FDTD_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+namespace {
 }  // namespace
 }  // namespace fdtd
 

@@ -1,0 +1,385 @@
+// Stencil device code shared by the single-grid Yee kernels (dev_yee.hip) and the batched
+// multi-geometry kernels (dev_batch.hip): argument blocks, sources, PML terms, monitors, and the
+// E / H block bodies.  Reference semantics: timefield3d.c:1679-1837 (semi-implicit E/H updates),
+// 3176-3239 (overlap); the PML is the extension whose formulas oracle/fdtd_oracle.c states.
+//
+// Layout: six FP32 component arrays, cell c = i + I*(j + J*k).  A block body handles one logical
+// block: 256 (i, j) columns of one z chunk; each thread marches its column upward, keeping the k-1
+// (E update) or k+1 (H update) neighbours it needs in registers; the i-1/j-1 (i+1/j+1) neighbours
+// come through L1/L2.
+#pragma once
+
+#include "dev_internal.h"
+
+namespace fdtd {
+
+// Arguments of one split E / H launch over one grid (also one entry of a batch table).
+struct YeeArgs {
+    float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
+    const void *mat;
+    const float *coef;  // [nmat][kCoefStride] for the chosen scheme
+    int I, J, K;
+    long long P;
+    int k0, k1;          // z range updated, already clipped to [1, K-1)
+    int kchunk;          // z planes per block
+    int plane_blocks;    // blocks per z plane
+    int nsrc;
+    SrcPlaneArg src[kMaxSrcPlanes];
+    const float *dense;  // 6 x N source volume when sources span more than kMaxSrcPlanes planes
+    long long N;
+    int nrs;
+    RunSrcArg rs[kMaxRunSrc];
+    int nmon;
+    RunMonArg mon[kMaxRunMon];
+    // device step counter of graph-captured batch steps (null: per-step values are in the args)
+    const int *step;
+    int step0;
+};
+
+struct PmlArgs {
+    const float *tab[3];  // per axis: bE, aE, kE, bH, aH, kH (N_axis floats each)
+    int nlo[3], nhi[3], W[3];
+    float *psiE[3][2];
+    float *psiH[3][2];
+};
+
+// Fill the launch arguments of the full-range split step of grid d (PML per d's boundary,
+// semi-implicit scheme, stored source planes, no run sources or monitors); builds the
+// coefficient and PML tables for dt first.
+int fill_step_args(fdtd_dev *d, double dt, YeeArgs *a, PmlArgs *pm);
+
+// Batched runs (dev_run.hip).  run_batch_prepare readies steps n0..n0+n-1 of grid d's run for
+// launches on stream st that read the step index from the device counter `step` (twiddles,
+// device source-coefficient tables, monitor partial buffers) and fills d's launch arguments with
+// them; run_batch_finish forms the monitors' overlaps and running DFTs of those steps on st and
+// leaves each source plane as the last step wrote it.
+int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, YeeArgs *a, PmlArgs *pm);
+int run_batch_finish(fdtd_dev *d, int n0, int n, hipStream_t st);
+
+namespace {
+
+// Source components m0..m0+2 of plane cell p at plane k: tabulated run source inside its mode
+// window, else the stored source plane.  Returns false when plane k carries no source.
+__device__ __forceinline__ bool source3(const YeeArgs &a, int k, long long p, int m0, float v[3]);
+
+template <typename MatT>
+__device__ __forceinline__ int mat_id(const void *mat, long long c) {
+    return (int)static_cast<const MatT *>(mat)[c];
+}
+
+// Source values of plane k (component m of plane cell p at J[m*stride + p]), or null.
+__device__ __forceinline__ const float *src_plane(const YeeArgs &a, int k, long long &stride) {
+    if (a.dense) {
+        stride = a.N;
+        return a.dense + (long long)k * a.P;
+    }
+    const float *J = nullptr;
+    stride = a.P;
+    for (int s = 0; s < a.nsrc; s++)
+        if (a.src[s].k == k) J = a.src[s].J;
+    return J;
+}
+
+__device__ __forceinline__ bool source3(const YeeArgs &a, int k, long long p, int m0, float v[3]) {
+    for (int q = 0; q < a.nrs; q++) {
+        const RunSrcArg &r = a.rs[q];
+        if (r.k == k && r.win[p]) {
+            float c1 = r.c1, c2 = r.c2;
+            if (r.tab) {
+                const int st = *a.step;
+                const float2 c = st >= 0 && st < r.ntab ? r.tab[st] : make_float2(0.0f, 0.0f);
+                c1 = c.x;
+                c2 = c.y;
+            }
+            for (int m = 0; m < 3; m++) v[m] = c1 * r.PR[(m0 + m) * a.P + p] + c2 * r.PI[(m0 + m) * a.P + p];
+            return true;
+        }
+    }
+    long long cs;
+    const float *J = src_plane(a, k, cs);
+    if (!J) return false;
+    for (int m = 0; m < 3; m++) v[m] = J[(m0 + m) * cs + p];
+    return true;
+}
+
+__device__ __forceinline__ int slab_index(int n, int N, int nlo, int nhi) {
+    if (n < nlo) return n;
+    if (nhi > 0 && n >= N - 1 - nhi) return nlo + n - (N - 1 - nhi);
+    return -1;
+}
+
+
+// psi <- b psi + a d (stored only when `live`); returns the PML-corrected difference d/kappa + psi
+__device__ __forceinline__ float pml_term_l(float *psi, float b, float a, float kinv, float d, bool live) {
+    float v = b * *psi + a * d;
+    if (live) *psi = v;
+    return d * kinv + v;
+}
+
+// E <- caE*E + cbE*(backward-difference curl of H) - cjE*J   (timefield3d.c:1759-1837)
+// KC > 0: the z chunk is KC planes, all loaded before any is computed (more bytes in flight per
+// wavefront); iterations past the end of a short last chunk reload its last plane and store
+// nothing.  KC = 0: a plain loop over a runtime chunk.
+template <typename MatT, bool PML, int KC>
+__device__ __forceinline__ void update_E_block(const YeeArgs &a, const PmlArgs &pm, int lb) {
+    const int pb = lb % a.plane_blocks, kc = lb / a.plane_blocks;
+    const long long p = (long long)pb * kBlock + threadIdx.x;
+    if (p >= a.P) return;
+    const int j = (int)(p / a.I), i = (int)(p - (long long)j * a.I);
+    if (i < 1 || i > a.I - 2 || j < 1 || j > a.J - 2) return;
+    const int kb = a.k0 + kc * a.kchunk;
+    const int ke = min(a.k1, kb + a.kchunk);
+    if (kb >= ke) return;
+    float *__restrict__ Ex = a.Ex;
+    float *__restrict__ Ey = a.Ey;
+    float *__restrict__ Ez = a.Ez;
+    const float *__restrict__ Hx = a.Hx;
+    const float *__restrict__ Hy = a.Hy;
+    const float *__restrict__ Hz = a.Hz;
+    int sx = -1, sy = -1;
+    float bx = 0, ax = 0, kx = 1, by = 0, ay = 0, ky = 1;
+    if (PML) {
+        sx = slab_index(i, a.I, pm.nlo[0], pm.nhi[0]);
+        sy = slab_index(j, a.J, pm.nlo[1], pm.nhi[1]);
+        bx = pm.tab[0][i], ax = pm.tab[0][a.I + i], kx = pm.tab[0][2 * a.I + i];
+        by = pm.tab[1][j], ay = pm.tab[1][a.J + j], ky = pm.tab[1][2 * a.J + j];
+    }
+    // one cell-update of plane k from its loaded inputs; stores only when `live`
+    auto cell = [&](int k, bool live, long long c, float hx, float hy, float hz, float hy_im, float hz_im,
+                    float hx_jm, float hz_jm, float hx_km, float hy_km, float eox, float eoy, float eoz, int m) {
+        const float *cf = a.coef + (long long)m * kCoefStride;
+        // curl_x = dHz/dy - dHy/dz, curl_y = dHx/dz - dHz/dx, curl_z = dHy/dx - dHx/dy
+        float x1 = hz - hz_jm, x2 = hy - hy_km;
+        float y1 = hx - hx_km, y2 = hz - hz_im;
+        float z1 = hy - hy_im, z2 = hx - hx_jm;
+        if (PML) {
+            const int sz = slab_index(k, a.K, pm.nlo[2], pm.nhi[2]);
+            const float kz = pm.tab[2][2 * a.K + k];
+            float tx1 = x1 * ky, tx2 = x2 * kz, ty1 = y1 * kz, ty2 = y2 * kx, tz1 = z1 * kx, tz2 = z2 * ky;
+            if (sx >= 0) {
+                const long long q = sx + (long long)pm.W[0] * (j + (long long)a.J * k);
+                ty2 = pml_term_l(pm.psiE[0][0] + q, bx, ax, kx, y2, live);
+                tz1 = pml_term_l(pm.psiE[0][1] + q, bx, ax, kx, z1, live);
+            }
+            if (sy >= 0) {
+                const long long q = i + (long long)a.I * (sy + (long long)pm.W[1] * k);
+                tz2 = pml_term_l(pm.psiE[1][0] + q, by, ay, ky, z2, live);
+                tx1 = pml_term_l(pm.psiE[1][1] + q, by, ay, ky, x1, live);
+            }
+            if (sz >= 0) {
+                const float bz = pm.tab[2][k], az = pm.tab[2][a.K + k];
+                const long long q = p + a.P * sz;
+                tx2 = pml_term_l(pm.psiE[2][0] + q, bz, az, kz, x2, live);
+                ty1 = pml_term_l(pm.psiE[2][1] + q, bz, az, kz, y1, live);
+            }
+            x1 = tx1, x2 = tx2, y1 = ty1, y2 = ty2, z1 = tz1, z2 = tz2;
+        }
+        float ex = cf[0] * eox + cf[3] * (x1 - x2);
+        float ey = cf[1] * eoy + cf[4] * (y1 - y2);
+        float ez = cf[2] * eoz + cf[5] * (z1 - z2);
+        float sv[3];
+        if (source3(a, k, p, 0, sv)) {
+            ex -= cf[6] * sv[0];
+            ey -= cf[7] * sv[1];
+            ez -= cf[8] * sv[2];
+        }
+        if (live) {
+            Ex[c] = ex;
+            Ey[c] = ey;
+            Ez[c] = ez;
+        }
+    };
+    const long long c0 = p + (long long)kb * a.P;
+    if constexpr (KC > 0) {
+        float hx[KC], hy[KC], hz[KC], hy_im[KC], hz_im[KC], hx_jm[KC], hz_jm[KC], eox[KC], eoy[KC], eoz[KC];
+        int m[KC];
+        const float hx_k0 = Hx[c0 - a.P], hy_k0 = Hy[c0 - a.P];
+#pragma unroll
+        for (int t = 0; t < KC; t++) {
+            const int kk = kb + t < ke ? kb + t : ke - 1;
+            const long long c = p + (long long)kk * a.P;
+            hx[t] = Hx[c], hy[t] = Hy[c], hz[t] = Hz[c];
+            hy_im[t] = Hy[c - 1], hz_im[t] = Hz[c - 1];
+            hx_jm[t] = Hx[c - a.I], hz_jm[t] = Hz[c - a.I];
+            eox[t] = Ex[c], eoy[t] = Ey[c], eoz[t] = Ez[c];
+            m[t] = mat_id<MatT>(a.mat, c);
+        }
+#pragma unroll
+        for (int t = 0; t < KC; t++) {
+            const bool live = kb + t < ke;
+            const int kk = live ? kb + t : ke - 1;
+            cell(kk, live, p + (long long)kk * a.P, hx[t], hy[t], hz[t], hy_im[t], hz_im[t], hx_jm[t], hz_jm[t],
+                 t == 0 ? hx_k0 : hx[t - 1], t == 0 ? hy_k0 : hy[t - 1], eox[t], eoy[t], eoz[t], m[t]);
+        }
+    } else {
+        float hx_km = Hx[c0 - a.P], hy_km = Hy[c0 - a.P];
+        long long c = c0;
+        for (int k = kb; k < ke; k++, c += a.P) {
+            const float hx = Hx[c], hy = Hy[c], hz = Hz[c];
+            cell(k, true, c, hx, hy, hz, Hy[c - 1], Hz[c - 1], Hx[c - a.I], Hz[c - a.I], hx_km, hy_km, Ex[c], Ey[c],
+                 Ez[c], mat_id<MatT>(a.mat, c));
+            hx_km = hx;
+            hy_km = hy;
+        }
+    }
+}
+
+// Block-wide FP64 sum of (re, im); every thread of the block must call it.  Returns the sum in
+// all threads.
+__device__ __forceinline__ double2 block_sum2(double re, double im) {
+    __shared__ double sre[kBlock / 64], sim[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_down(re, off, 64);
+        im += __shfl_down(im, off, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sre[w] = re;
+        sim[w] = im;
+    }
+    __syncthreads();
+    double r = 0, s = 0;
+    for (int q = 0; q < kBlock / 64; q++) {
+        r += sre[q];
+        s += sim[q];
+    }
+    __syncthreads();  // the slots are reused by the next call
+    return make_double2(r, s);
+}
+
+// Overlap monitors of a MON launch, after the block's z march: for each monitor plane inside the
+// chunk, the block's FP64 partial of Hx mEy - Hy mEx + Ey mHx - Ex mHy (timefield3d.c:3195-3224)
+// over its cells, from E (final since the E update) and H (just stored by these same threads).
+__device__ __forceinline__ void monitor_epilogue(const YeeArgs &a, int kb, int ke, int pb, long long p, bool active) {
+    for (int q = 0; q < a.nmon; q++) {
+        const RunMonArg &M = a.mon[q];
+        if (M.k < kb || M.k >= ke) continue;  // uniform over the block
+        double re = 0.0, im = 0.0;
+        if (active) {
+            const long long c = p + (long long)M.k * a.P;
+            const double ex = a.Ex[c], ey = a.Ey[c], hx = a.Hx[c], hy = a.Hy[c];
+            const float *W = M.W + p;
+            re = (hx * (double)W[2 * a.P] - hy * (double)W[0]) + (ey * (double)W[4 * a.P] - ex * (double)W[6 * a.P]);
+            im = (hx * (double)W[3 * a.P] - hy * (double)W[1 * a.P]) +
+                 (ey * (double)W[5 * a.P] - ex * (double)W[7 * a.P]);
+        }
+        const double2 v = block_sum2(re, im);
+        double2 *part = a.step ? M.part + (long long)(*a.step - a.step0) * M.stride : M.part;
+        if (threadIdx.x == 0) part[pb] = v;
+    }
+}
+
+// H <- caH*H - cbH*(forward-difference curl of E) - cjH*M   (timefield3d.c:1679-1757)
+// KC as for the E update.  MON: the launch also evaluates the run's overlap monitors (every
+// thread of a block reaches the block reductions, so out-of-range threads skip the march
+// instead of returning).
+template <typename MatT, bool PML, bool MON, int KC>
+__device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &pm, int lb) {
+    const int pb = lb % a.plane_blocks, kc = lb / a.plane_blocks;
+    const long long p = (long long)pb * kBlock + threadIdx.x;
+    const int kb = a.k0 + kc * a.kchunk;
+    const int ke = min(a.k1, kb + a.kchunk);
+    if (kb >= ke) return;  // uniform over the block
+    const int j = (int)(p / a.I), i = (int)(p - (long long)j * a.I);
+    const bool active = p < a.P && i >= 1 && i <= a.I - 2 && j >= 1 && j <= a.J - 2;
+    if (!MON && !active) return;
+    if (active) {
+        const float *__restrict__ Ex = a.Ex;
+        const float *__restrict__ Ey = a.Ey;
+        const float *__restrict__ Ez = a.Ez;
+        float *__restrict__ Hx = a.Hx;
+        float *__restrict__ Hy = a.Hy;
+        float *__restrict__ Hz = a.Hz;
+        int sx = -1, sy = -1;
+        float bx = 0, ax = 0, kx = 1, by = 0, ay = 0, ky = 1;
+        if (PML) {
+            sx = slab_index(i, a.I, pm.nlo[0], pm.nhi[0]);
+            sy = slab_index(j, a.J, pm.nlo[1], pm.nhi[1]);
+            bx = pm.tab[0][3 * a.I + i], ax = pm.tab[0][4 * a.I + i], kx = pm.tab[0][5 * a.I + i];
+            by = pm.tab[1][3 * a.J + j], ay = pm.tab[1][4 * a.J + j], ky = pm.tab[1][5 * a.J + j];
+        }
+        auto cell = [&](int k, bool live, long long c, float ex, float ey, float ez, float ex_kp, float ey_kp,
+                        float ey_ip, float ez_ip, float ex_jp, float ez_jp, float hox, float hoy, float hoz, int m) {
+            const float *cf = a.coef + (long long)m * kCoefStride;
+            // curl_x = dEz/dy - dEy/dz, curl_y = dEx/dz - dEz/dx, curl_z = dEy/dx - dEx/dy
+            float x1 = ez_jp - ez, x2 = ey_kp - ey;
+            float y1 = ex_kp - ex, y2 = ez_ip - ez;
+            float z1 = ey_ip - ey, z2 = ex_jp - ex;
+            if (PML) {
+                const int sz = slab_index(k, a.K, pm.nlo[2], pm.nhi[2]);
+                const float kz = pm.tab[2][5 * a.K + k];
+                float tx1 = x1 * ky, tx2 = x2 * kz, ty1 = y1 * kz, ty2 = y2 * kx, tz1 = z1 * kx, tz2 = z2 * ky;
+                if (sx >= 0) {
+                    const long long q = sx + (long long)pm.W[0] * (j + (long long)a.J * k);
+                    ty2 = pml_term_l(pm.psiH[0][0] + q, bx, ax, kx, y2, live);
+                    tz1 = pml_term_l(pm.psiH[0][1] + q, bx, ax, kx, z1, live);
+                }
+                if (sy >= 0) {
+                    const long long q = i + (long long)a.I * (sy + (long long)pm.W[1] * k);
+                    tz2 = pml_term_l(pm.psiH[1][0] + q, by, ay, ky, z2, live);
+                    tx1 = pml_term_l(pm.psiH[1][1] + q, by, ay, ky, x1, live);
+                }
+                if (sz >= 0) {
+                    const float bz = pm.tab[2][3 * a.K + k], az = pm.tab[2][4 * a.K + k];
+                    const long long q = p + a.P * sz;
+                    tx2 = pml_term_l(pm.psiH[2][0] + q, bz, az, kz, x2, live);
+                    ty1 = pml_term_l(pm.psiH[2][1] + q, bz, az, kz, y1, live);
+                }
+                x1 = tx1, x2 = tx2, y1 = ty1, y2 = ty2, z1 = tz1, z2 = tz2;
+            }
+            float hx = cf[9] * hox - cf[12] * (x1 - x2);
+            float hy = cf[10] * hoy - cf[13] * (y1 - y2);
+            float hz = cf[11] * hoz - cf[14] * (z1 - z2);
+            float sv[3];
+            if (source3(a, k, p, 3, sv)) {
+                hx -= cf[15] * sv[0];
+                hy -= cf[16] * sv[1];
+                hz -= cf[17] * sv[2];
+            }
+            if (live) {
+                Hx[c] = hx;
+                Hy[c] = hy;
+                Hz[c] = hz;
+            }
+        };
+        const long long c0 = p + (long long)kb * a.P;
+        if constexpr (KC > 0) {
+            float ez[KC], ex_kp[KC], ey_kp[KC], ey_ip[KC], ez_ip[KC], ex_jp[KC], ez_jp[KC], hox[KC], hoy[KC], hoz[KC];
+            int m[KC];
+            const float ex_k0 = Ex[c0], ey_k0 = Ey[c0];
+#pragma unroll
+            for (int t = 0; t < KC; t++) {
+                const int kk = kb + t < ke ? kb + t : ke - 1;
+                const long long c = p + (long long)kk * a.P;
+                ez[t] = Ez[c];
+                ex_kp[t] = Ex[c + a.P], ey_kp[t] = Ey[c + a.P];
+                ey_ip[t] = Ey[c + 1], ez_ip[t] = Ez[c + 1];
+                ex_jp[t] = Ex[c + a.I], ez_jp[t] = Ez[c + a.I];
+                hox[t] = Hx[c], hoy[t] = Hy[c], hoz[t] = Hz[c];
+                m[t] = mat_id<MatT>(a.mat, c);
+            }
+#pragma unroll
+            for (int t = 0; t < KC; t++) {
+                const bool live = kb + t < ke;
+                const int kk = live ? kb + t : ke - 1;
+                cell(kk, live, p + (long long)kk * a.P, t == 0 ? ex_k0 : ex_kp[t - 1], t == 0 ? ey_k0 : ey_kp[t - 1],
+                     ez[t], ex_kp[t], ey_kp[t], ey_ip[t], ez_ip[t], ex_jp[t], ez_jp[t], hox[t], hoy[t], hoz[t], m[t]);
+            }
+        } else {
+            float ex = Ex[c0], ey = Ey[c0];
+            long long c = c0;
+            for (int k = kb; k < ke; k++, c += a.P) {
+                const float ex_kp = Ex[c + a.P], ey_kp = Ey[c + a.P];
+                cell(k, true, c, ex, ey, Ez[c], ex_kp, ey_kp, Ey[c + 1], Ez[c + 1], Ex[c + a.I], Ez[c + a.I], Hx[c],
+                     Hy[c], Hz[c], mat_id<MatT>(a.mat, c));
+                ex = ex_kp;
+                ey = ey_kp;
+            }
+        }
+    }
+    if (MON) monitor_epilogue(a, kb, ke, pb, p, active);
+}
+
+
+}  // namespace
+}  // namespace fdtd

@@ -95,6 +95,25 @@ int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset);
 int fdtd_dev_run_overlap_result(fdtd_dev *d, int id, double *cumulative, double *last_inst);
 int fdtd_dev_run_dft_result(fdtd_dev *d, int id, double *out);
 
+/* Batched runs (extension): the grids advance in lockstep, one E and one H launch per step over
+ * all of them (persistent kernels over a device table of per-grid descriptors; steps replayed
+ * from a captured hipGraph).  Every grid runs under the PML boundary with the split kernels and
+ * holds its run configuration (at most 2 run sources and 4 overlap monitors clear of the boundary
+ * ring, no DFT monitors).  fdtd_dev_batch_run(b, dt, n0, n) advances steps n0..n0+n-1 of every
+ * grid's run; results are read per grid (fdtd_dev_run_overlap_result).  The z planes per thread
+ * are picked on the first run by timing real steps with each candidate (2, 3, 4) unless
+ * fdtd_dev_batch_set_kchunk fixed them; fdtd_dev_batch_info reports the choice, the candidates'
+ * ms per step and the total cell count. */
+typedef struct fdtd_dev_batch fdtd_dev_batch;
+fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n);
+void fdtd_dev_batch_free(fdtd_dev_batch *b);
+int fdtd_dev_batch_set_kchunk(fdtd_dev_batch *b, int kchunk);
+int fdtd_dev_batch_run(fdtd_dev_batch *b, double dt, int n0, int n);
+int fdtd_dev_batch_info(fdtd_dev_batch *b, int *kchunk, long long *cells, int *ntuned, int tuned_kc[8],
+                        double tuned_ms[8]);
+int fdtd_dev_batch_timer_start(fdtd_dev_batch *b);
+int fdtd_dev_batch_timer_stop(fdtd_dev_batch *b, double *elapsed_ms);
+
 /* Soft sources on plane k = cz.  mode is interleaved complex double[6*mi*mj] (already
  * power-normalised by the caller).  adjoint selects the CW adjoint form (timefield3d.c:3028). */
 int fdtd_dev_mode_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,

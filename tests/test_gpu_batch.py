@@ -1,0 +1,95 @@
+"""GPU tests of batched multi-geometry runs (fdtd_dev_batch_*): grids of uneven length advanced in
+lockstep by persistent kernels over a descriptor table, steps replayed from a captured hipGraph,
+z-chunk depth picked by the autotuner.  The batched result must equal each grid run on its own
+through the single-grid path (same arithmetic, so to FP32 rounding: relative 1e-6) for fields and
+FP64 overlap monitors, including across several run calls and with a fixed chunk depth."""
+import importlib
+import sys
+
+import numpy as np
+import pytest
+
+from tests import gradient_case as GC
+
+pytestmark = pytest.mark.gpu
+
+fd = importlib.import_module("3d_fdtd_amd")
+
+KS = (30, 47, 64)
+NSTEPS = 170
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _schedule():
+    n = np.arange(NSTEPS)
+    return np.exp(-((n - 60.0) / 20.0) ** 2 / 2), GC.OMEGA * n * GC.DT
+
+
+def _make(K, w):
+    I = J = 20
+    ep = np.full((K, J, I, 3), 2.0, np.float32)
+    ep[:, 8:12, 10 - w:10 + w, :] = 6.0
+    g = fd.DevGrid(I, J, K, GC.DISC, ep.reshape(-1))
+    g.cpml((5, 5, 5, 5, 5, 5))
+    mode = fd.normalised_mode(GC.MI, GC.MJ, GC.DISC, GC.mode_array())
+    amp, phase = _schedule()
+    g.add_source(I // 2, J // 2, 7, mode, GC.MI, GC.MJ, GC.DISC, amp, phase)
+    oms = GC.OMEGA * np.array([0.9, 1.0, 1.1])
+    mons = [g.add_overlap(I // 2, J // 2, K - 8, mode, GC.MI, GC.MJ, GC.DISC, oms),
+            g.add_overlap(I // 2, J // 2, 10, mode, GC.MI, GC.MJ, GC.DISC, oms, reverse=1)]
+    return g, mons, len(oms)
+
+
+def _results(g, mons, nf):
+    out = []
+    for m in mons:
+        cum, last = g.overlap(m, nf)
+        out.extend(list(cum) + [last])
+    return np.array(out), g.fields()
+
+
+def _single():
+    res = []
+    for q, K in enumerate(KS):
+        g, mons, nf = _make(K, 1 + q)
+        g.run(GC.DT, 0, NSTEPS)
+        res.append(_results(g, mons, nf))
+        g.free()
+    return res
+
+
+@pytest.fixture(scope="module")
+def single_results():
+    say("batch: single-grid reference runs")
+    return _single()
+
+
+@pytest.mark.parametrize("calls,kchunk", [((0, NSTEPS), 0), ((0, 100, NSTEPS), 0), ((0, 37, NSTEPS), 3)])
+def test_batch_matches_single_grids(single_results, calls, kchunk):
+    say("batch: calls", calls, "kchunk", kchunk)
+    built = [_make(K, 1 + q) for q, K in enumerate(KS)]
+    b = fd.Batch([g for g, _, _ in built])
+    if kchunk:
+        b.set_kchunk(kchunk)
+    for a, c in zip(calls[:-1], calls[1:]):
+        b.run(GC.DT, a, c - a)
+    info = b.info()
+    say("batch info", info)
+    assert info["kchunk"] in (2, 3, 4)
+    assert info["cells"] == sum(20 * 20 * K for K in KS)
+    if not kchunk:
+        assert sorted(info["tuned"]) == [2, 3, 4]
+    for (g, mons, nf), (mon_ref, f_ref) in zip(built, single_results):
+        mon, f = _results(g, mons, nf)
+        em = np.abs(mon - mon_ref).max() / np.abs(mon_ref).max()
+        ef = np.linalg.norm(f - f_ref) / np.linalg.norm(f_ref)
+        say("grid K", g.K, "monitor max rel", em, "field rel L2", ef)
+        assert np.abs(mon_ref).max() > 0
+        assert em <= 1e-6 and ef <= 1e-6
+    b.free()
+    for g, _, _ in built:
+        g.free()
