@@ -171,11 +171,21 @@ SIGNATURES = {
     "fdtd_dev_timer_stop": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_batch_new": ([_P(C.c_void_p), _i], C.c_void_p),
     "fdtd_dev_batch_free": ([C.c_void_p], None),
-    "fdtd_dev_batch_set_kchunk": ([C.c_void_p, _i], _i),
+    "fdtd_dev_batch_set_shape": ([C.c_void_p, _i, _i, _i], _i),
     "fdtd_dev_batch_run": ([C.c_void_p, _d, _i, _i], _i),
     "fdtd_dev_batch_info": ([C.c_void_p, _P(_i), _P(C.c_longlong), _P(_i), _P(_i), _P(_d)], _i),
     "fdtd_dev_batch_timer_start": ([C.c_void_p], _i),
     "fdtd_dev_batch_timer_stop": ([C.c_void_p, _P(_d)], _i),
+    # transmission surrogate (include/fdtd3d_mlp.h)
+    "fdtd_mlp_new": ([_i, _i, _i, _i], C.c_void_p),
+    "fdtd_mlp_free": ([C.c_void_p], None),
+    "fdtd_mlp_param_count": ([C.c_void_p], C.c_longlong),
+    "fdtd_mlp_set_params": ([C.c_void_p, _P(C.c_float)], _i),
+    "fdtd_mlp_get_params": ([C.c_void_p, _P(C.c_float)], _i),
+    "fdtd_mlp_forward": ([C.c_void_p, _P(C.c_float), _i, _P(C.c_float)], _i),
+    "fdtd_mlp_loss_grad": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _i, _P(_d), _P(C.c_float)], _i),
+    "fdtd_mlp_adam_step": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _i, C.c_float, C.c_float, C.c_float, C.c_float,
+                            _P(_d)], _i),
 }
 
 _lib = None
@@ -349,18 +359,77 @@ class Batch:
     def run(self, dt, n0, n):
         dev_check(lib().fdtd_dev_batch_run(self.b, dt, n0, n), "batch_run")
 
-    def set_kchunk(self, kc):
-        dev_check(lib().fdtd_dev_batch_set_kchunk(self.b, kc), "batch_set_kchunk")
+    def set_shape(self, kchunk, unroll=True, persistent=True):
+        dev_check(lib().fdtd_dev_batch_set_shape(self.b, kchunk, int(unroll), int(persistent)), "batch_set_shape")
+
+    @staticmethod
+    def shape_name(kchunk, unroll, persistent):
+        return f"kc{kchunk}-{'unrolled' if unroll else 'loop'}-{'persistent' if persistent else 'full'}"
 
     def info(self):
-        kc, nt = C.c_int(), C.c_int()
+        sh, nt = (C.c_int * 3)(), C.c_int()
         cells = C.c_longlong()
-        tk, tm = (C.c_int * 8)(), (C.c_double * 8)()
-        dev_check(lib().fdtd_dev_batch_info(self.b, C.byref(kc), C.byref(cells), C.byref(nt), tk, tm), "batch_info")
-        return {"kchunk": kc.value, "cells": cells.value,
-                "tuned": {int(tk[q]): float(tm[q]) for q in range(nt.value)}}
+        tk, tm = (C.c_int * 16)(), (C.c_double * 16)()
+        dev_check(lib().fdtd_dev_batch_info(self.b, sh, C.byref(cells), C.byref(nt), tk, tm), "batch_info")
+        tuned = {self.shape_name(tk[q] % 16, tk[q] & 16, tk[q] & 32): float(tm[q]) for q in range(nt.value)}
+        return {"shape": self.shape_name(sh[0], sh[1], sh[2]), "kchunk": sh[0], "cells": cells.value, "tuned": tuned}
 
     def free(self):
         if self.b:
             lib().fdtd_dev_batch_free(self.b)
             self.b = None
+
+
+class Mlp:
+    """The transmission surrogate (fdtd_mlp_*): BF16-MFMA MLP trained and run on the GPU."""
+
+    def __init__(self, n_in, width, depth, n_out, params=None):
+        self.dims = (n_in, width, depth, n_out)
+        self.m = lib().fdtd_mlp_new(n_in, width, depth, n_out)
+        if not self.m:
+            raise RuntimeError("fdtd_mlp_new: " + lib().fdtd_dev_last_error().decode())
+        self.n = int(lib().fdtd_mlp_param_count(self.m))
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def _fp(a):
+        return a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n
+        dev_check(lib().fdtd_mlp_set_params(self.m, self._fp(p)), "mlp set_params")
+
+    def params(self):
+        p = np.zeros(self.n, np.float32)
+        dev_check(lib().fdtd_mlp_get_params(self.m, self._fp(p)), "mlp get_params")
+        return p
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.dims[3]), np.float32)
+        dev_check(lib().fdtd_mlp_forward(self.m, self._fp(x), x.shape[0], self._fp(y)), "mlp forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.n, np.float32)
+        loss = C.c_double()
+        dev_check(lib().fdtd_mlp_loss_grad(self.m, self._fp(x), self._fp(t), x.shape[0], C.byref(loss), self._fp(g)),
+                  "mlp loss_grad")
+        return loss.value, g
+
+    def adam_step(self, x, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        dev_check(lib().fdtd_mlp_adam_step(self.m, self._fp(x), self._fp(t), x.shape[0], lr, beta1, beta2, eps,
+                                           C.byref(loss)), "mlp adam_step")
+        return loss.value
+
+    def free(self):
+        if self.m:
+            lib().fdtd_mlp_free(self.m)
+            self.m = None

@@ -69,17 +69,21 @@ struct fdtd_dev_batch {
     BatchGeom *geo = nullptr;   // device descriptor table
     std::vector<BatchGeom> host;
     long long total = 0;        // virtual blocks of the current chunk depth
-    int kc = 0;                 // z planes per thread (0 = not tuned yet: the run tunes it)
+    // launch shape: z planes per thread, unrolled (all planes loaded first) or a runtime loop,
+    // persistent grid (2048 blocks) or one block per virtual block; kc = 0: tune on the next run
+    int kc = 0;
+    bool unroll = true, persistent = true;
     bool u8 = true, mon = false;
     unsigned grid_blocks = 0;
     // captured segment of kSeg steps
     hipGraphExec_t exec = nullptr;
     int exec_kc = -1;
-    bool exec_mon = false;
+    bool exec_unroll = false, exec_persistent = false, exec_mon = false;
     long long exec_total = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double tuned_ms[8] = {};
-    int tuned_kc[8] = {};
+    static constexpr int kMaxTuned = 16;
+    double tuned_ms[kMaxTuned] = {};
+    int tuned_code[kMaxTuned] = {};  // kc + 16 * unroll + 32 * persistent
     int ntuned = 0;
 };
 
@@ -87,7 +91,12 @@ namespace {
 
 constexpr int kSeg = 32;           // steps per captured graph segment
 constexpr int kBatchChunk = 4096;  // steps per prepare/finish cycle (bounds the monitor buffers)
-const int kTuneKc[] = {2, 3, 4};
+struct Shape {
+    int kc;
+    bool unroll, persistent;
+};
+const Shape kTune[] = {{2, true, true},  {4, true, true},  {2, false, true},  {4, false, true},  {8, false, true},
+                       {2, true, false}, {4, true, false}, {2, false, false}, {4, false, false}, {8, false, false}};
 
 long long batch_layout(fdtd_dev_batch *b, int kc) {
     long long vb = 0;
@@ -101,13 +110,14 @@ long long batch_layout(fdtd_dev_batch *b, int kc) {
     return vb;
 }
 
-int upload_layout(fdtd_dev_batch *b, int kc) {
+int upload_layout(fdtd_dev_batch *b, int kc, bool persistent) {
     b->total = batch_layout(b, kc);
     FDTD_TRY(hipMemcpyAsync(b->geo, b->host.data(), b->host.size() * sizeof(BatchGeom), hipMemcpyHostToDevice,
                             b->stream));
     FDTD_TRY(hipStreamSynchronize(b->stream));  // the host table is rewritten by the next layout
-    // persistent launch: up to 8 blocks of 256 threads per CU on 256 CUs, a multiple of 8 (XCDs)
-    long long g = b->total < 2048 ? b->total : 2048;
+    // persistent: up to 8 blocks of 256 threads per CU on 256 CUs; else one block per virtual
+    // block; either way a multiple of 8 (XCDs) so that a grid-stride step keeps the XCD
+    long long g = persistent && b->total > 2048 ? 2048 : b->total;
     g = (g + 7) / 8 * 8;
     b->grid_blocks = (unsigned)(g < 8 ? 8 : g);
     return 0;
@@ -138,8 +148,8 @@ void launch_step_kc(fdtd_dev_batch *b) {
     hipLaunchKernelGGL(k_step_tick, dim3(1), dim3(64), 0, b->stream, b->step);
 }
 
-int launch_step(fdtd_dev_batch *b, int kc) {
-    switch (kc) {
+int launch_step(fdtd_dev_batch *b, int kc, bool unroll) {
+    switch (unroll ? kc : 0) {
         case 2: launch_step_kc<2>(b); break;
         case 3: launch_step_kc<3>(b); break;
         case 4: launch_step_kc<4>(b); break;
@@ -151,7 +161,9 @@ int launch_step(fdtd_dev_batch *b, int kc) {
 
 // The captured segment for the current layout (re-captured when the kernel choice changes).
 int segment_graph(fdtd_dev_batch *b) {
-    if (b->exec && b->exec_kc == b->kc && b->exec_mon == b->mon && b->exec_total == b->total) return 0;
+    if (b->exec && b->exec_kc == b->kc && b->exec_unroll == b->unroll && b->exec_persistent == b->persistent &&
+        b->exec_mon == b->mon && b->exec_total == b->total)
+        return 0;
     if (b->exec) {
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
@@ -159,7 +171,7 @@ int segment_graph(fdtd_dev_batch *b) {
     hipGraph_t graph = nullptr;
     FDTD_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int s = 0; s < kSeg && rc == 0; s++) rc = launch_step(b, b->kc);
+    for (int s = 0; s < kSeg && rc == 0; s++) rc = launch_step(b, b->kc, b->unroll);
     hipError_t e = hipStreamEndCapture(b->stream, &graph);
     if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -170,6 +182,8 @@ int segment_graph(fdtd_dev_batch *b) {
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail("hipGraphInstantiate", e);
     b->exec_kc = b->kc;
+    b->exec_unroll = b->unroll;
+    b->exec_persistent = b->persistent;
     b->exec_mon = b->mon;
     b->exec_total = b->total;
     return 0;
@@ -196,32 +210,33 @@ int batch_chunk(fdtd_dev_batch *b, double dt, int n0, int n) {
     int done = 0;
     if (b->kc == 0) {
         const int per = 4;  // timed steps per candidate (after one untimed step)
-        int best = 2;
+        Shape best = kTune[0];
         double best_ms = 1e30;
         b->ntuned = 0;
-        for (int kc : kTuneKc) {
-            if (done + per + 1 > n) break;
-            if ((rc = upload_layout(b, kc))) return rc;
-            if ((rc = launch_step(b, kc))) return rc;
+        for (const Shape &c : kTune) {
+            if (done + per + 1 > n || b->ntuned >= fdtd_dev_batch::kMaxTuned) break;
+            if ((rc = upload_layout(b, c.kc, c.persistent))) return rc;
+            if ((rc = launch_step(b, c.kc, c.unroll))) return rc;
             FDTD_TRY(hipEventRecord(b->ev0, b->stream));
             for (int s = 0; s < per; s++)
-                if ((rc = launch_step(b, kc))) return rc;
+                if ((rc = launch_step(b, c.kc, c.unroll))) return rc;
             FDTD_TRY(hipEventRecord(b->ev1, b->stream));
             FDTD_TRY(hipEventSynchronize(b->ev1));
             float ms = 0;
             FDTD_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            b->tuned_kc[b->ntuned] = kc;
+            b->tuned_code[b->ntuned] = c.kc + 16 * c.unroll + 32 * c.persistent;
             b->tuned_ms[b->ntuned++] = ms / per;
             if (ms < best_ms) {
                 best_ms = ms;
-                best = kc;
+                best = c;
             }
             done += per + 1;
         }
-        if (b->ntuned > 0) b->kc = best;
-        else b->kc = 2;
+        b->kc = best.kc;
+        b->unroll = best.unroll;
+        b->persistent = best.persistent;
     }
-    if ((rc = upload_layout(b, b->kc))) return rc;
+    if ((rc = upload_layout(b, b->kc, b->persistent))) return rc;
     const int left = n - done;
     const int segs = left / kSeg;
     if (segs > 0) {
@@ -229,7 +244,7 @@ int batch_chunk(fdtd_dev_batch *b, double dt, int n0, int n) {
         for (int s = 0; s < segs; s++) FDTD_TRY(hipGraphLaunch(b->exec, b->stream));
     }
     for (int s = segs * kSeg; s < left; s++)
-        if ((rc = launch_step(b, b->kc))) return rc;
+        if ((rc = launch_step(b, b->kc, b->unroll))) return rc;
     for (auto *d : b->grids)
         if ((rc = run_batch_finish(d, n0, n, b->stream))) return rc;
     FDTD_TRY(hipStreamSynchronize(b->stream));
@@ -280,12 +295,14 @@ void fdtd_dev_batch_free(fdtd_dev_batch *b) {
     delete b;
 }
 
-int fdtd_dev_batch_set_kchunk(fdtd_dev_batch *b, int kchunk) {
-    if (kchunk < 0) {
-        set_error("batch: z chunk must be >= 0 (0 = tune on the next run)");
+int fdtd_dev_batch_set_shape(fdtd_dev_batch *b, int kchunk, int unroll, int persistent) {
+    if (kchunk < 0 || (unroll && kchunk > 4)) {
+        set_error("batch: z chunk must be >= 0 (0 = tune on the next run), unrolled chunks at most 4");
         return -2;
     }
     b->kc = kchunk;
+    b->unroll = unroll != 0;
+    b->persistent = persistent != 0;
     return 0;
 }
 
@@ -297,15 +314,19 @@ int fdtd_dev_batch_run(fdtd_dev_batch *b, double dt, int n0, int n) {
     return 0;
 }
 
-int fdtd_dev_batch_info(fdtd_dev_batch *b, int *kchunk, long long *cells, int *ntuned, int tuned_kc[8],
-                        double tuned_ms[8]) {
+int fdtd_dev_batch_info(fdtd_dev_batch *b, int shape[3], long long *cells, int *ntuned, int tuned_shape[16],
+                        double tuned_ms[16]) {
     long long c = 0;
     for (auto *d : b->grids) c += d->N;
-    if (kchunk) *kchunk = b->kc;
+    if (shape) {
+        shape[0] = b->kc;
+        shape[1] = b->unroll;
+        shape[2] = b->persistent;
+    }
     if (cells) *cells = c;
     if (ntuned) *ntuned = b->ntuned;
-    for (int q = 0; q < b->ntuned && q < 8; q++) {
-        if (tuned_kc) tuned_kc[q] = b->tuned_kc[q];
+    for (int q = 0; q < b->ntuned && q < 16; q++) {
+        if (tuned_shape) tuned_shape[q] = b->tuned_code[q];
         if (tuned_ms) tuned_ms[q] = b->tuned_ms[q];
     }
     return 0;

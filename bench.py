@@ -77,6 +77,104 @@ def workload(fd, steps_total):
     return g, dt, md, mons, omegas
 
 
+# ---- batched workload: 64 Si tapers of uneven length advanced in lockstep ----
+TAPER_SEED = 20261203
+TAPER_I, TAPER_J = 200, 100  # lateral x vertical cross-section (library i, j), 20 nm cells
+TAPER_NPML = 8
+
+
+def taper_geometry(rng, n):
+    """n tapers: length L ~ U[3, 15] um, 8 control-point widths ~ U[0.4, 3.2] um (seeded)."""
+    return rng.uniform(3.0, 15.0, n), rng.uniform(0.4, 3.2, (n, 8))
+
+
+def taper_permittivity(L, widths):
+    """eps of one taper grid in library axes [k][j][i][3]: SiO2 (n=1.44) cladding, Si (n=3.48) core
+    220 nm tall, whose width follows a cubic spline through the 8 control points along the
+    propagation axis (constant beyond the end points: the strip ports)."""
+    from scipy.interpolate import CubicSpline
+
+    K = int(round(L / DISC)) + 20
+    ep = np.full((K, TAPER_J, TAPER_I, 3), 1.44 ** 2, np.float32)
+    k = np.arange(K)
+    xs = np.linspace(10, K - 11, len(widths))
+    w = CubicSpline(xs, widths)(np.clip(k, xs[0], xs[-1])) / DISC  # width in cells
+    i = np.arange(TAPER_I) + 0.5 - TAPER_I / 2
+    core = np.abs(i)[None, :] < (w[:, None] / 2)  # [k][i]
+    j0 = TAPER_J // 2 - 5
+    ep[:, j0:j0 + 11, :, :][np.broadcast_to(core[:, None, :, None], (K, 11, TAPER_I, 3))] = 3.48 ** 2
+    return ep.reshape(-1), K
+
+
+def tapers_workload(fd, n_tapers, steps_total):
+    L = fd.lib()
+    rng = np.random.default_rng(TAPER_SEED)
+    lengths, widths = taper_geometry(rng, n_tapers)
+    lam0, dlam = 1.55, 0.2
+    w0 = 2 * math.pi / lam0
+    dt = 0.99 * DISC / math.sqrt(3)
+    md = L.mode_gaussian(60, 50, DISC, fd.Complex(1.44 ** 2, 0), fd.Complex(1.0, 0), w0, 8.0, 0)
+    mode = fd.normalised_mode(60, 50, DISC, fd.mode_fields(md).copy())
+    L.mode_free(md)
+    sig_t = lam0 * lam0 / (2 * math.pi * dlam)
+    t = np.arange(steps_total) * dt
+    amp = np.exp(-(t - 4 * sig_t) ** 2 / (2 * sig_t ** 2))
+    phase = w0 * t
+    omegas = 2 * math.pi / np.linspace(1.50, 1.60, 16)
+    grids = []
+    for q in range(n_tapers):
+        ep, K = taper_permittivity(lengths[q], widths[q])
+        g = fd.DevGrid(TAPER_I, TAPER_J, K, DISC, ep)
+        del ep
+        g.cpml((TAPER_NPML,) * 6, 3, 1e-8, 1.0, 0.05)
+        g.add_source(TAPER_I // 2, TAPER_J // 2, TAPER_NPML + 1, mode, 60, 50, DISC, amp, phase)
+        g.mon = g.add_overlap(TAPER_I // 2, TAPER_J // 2, K - TAPER_NPML - 2, mode, 60, 50, DISC, omegas)
+        grids.append(g)
+    return grids, dt, omegas, lengths
+
+
+def bench_tapers(fd, args):
+    """Batched taper workload: whole-job cell-updates/s over every grid (one JSON line)."""
+    W, S = args.warmup, args.steps
+    print(f"bench: building {args.tapers} taper grids", file=sys.stderr, flush=True)
+    grids, dt, omegas, lengths = tapers_workload(fd, args.tapers, W + S)
+    b = fd.Batch(grids)
+    if args.kchunk:
+        b.set_shape(args.kchunk, args.kchunk <= 4, True)
+    print(f"bench: warmup {W} steps (chunk-depth autotuning inside)", file=sys.stderr, flush=True)
+    b.run(dt, 0, W)
+    info = b.info()
+    print(f"bench: timing {S} steps", file=sys.stderr, flush=True)
+    fd.dev_check(fd.lib().fdtd_dev_batch_timer_start(b.b), "timer")
+    t0 = time.perf_counter()
+    b.run(dt, W, S)
+    ms = C.c_double()
+    fd.dev_check(fd.lib().fdtd_dev_batch_timer_stop(b.b, C.byref(ms)), "timer")
+    wall = time.perf_counter() - t0
+    elapsed_s = max(ms.value / 1e3, wall)
+    cells = info["cells"]
+    value = cells * S / elapsed_s
+    achieved = STEP_BYTES * cells / (elapsed_s / S) / 1e9
+    out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": S, "warmup": W,
+           "ms_per_step": elapsed_s * 1e3 / S, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{args.tapers} Si tapers (seed {TAPER_SEED}), lengths U[3,15] um, 8 spline "
+                                  "control-point widths U[0.4,3.2] um, 200x100 cross-section, 8-cell CPML, mode "
+                                  "source, FP32 fields, FP64 overlap monitor x 16 wavelengths per taper, batched "
+                                  "persistent kernels with hipGraph-captured steps",
+                      "grids": args.tapers, "cells_total": cells,
+                      "lengths_cells": [int(round(x / DISC)) + 20 for x in lengths]},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "batched step (E + H launches, graph replays)",
+                        "algorithmic_bytes_per_launch": STEP_BYTES * cells,
+                        "autotuned_shape": info["shape"], "tuned_ms_per_step": info["tuned"]}}
+    b.free()
+    for g in grids:
+        g.free()
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(target_s=12.0):
     """Reference FP64 step on the same grid size, host cores, bounded sample."""
     threads = int(os.environ.get("FDTD3D_CPU_THREADS", min(16, os.cpu_count() or 1)))
@@ -156,6 +254,9 @@ def main():
     ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
     ap.add_argument("--kchunk", type=int, default=None,
                     help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
+    ap.add_argument("--workload", choices=("waveguide", "tapers"), default="waveguide",
+                    help="waveguide: the strip waveguide (default); tapers: the batched taper set")
+    ap.add_argument("--tapers", type=int, default=64, help="number of taper grids of the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -168,6 +269,11 @@ def main():
         sys.exit(2)
     fd = importlib.import_module("3d_fdtd_amd")
     L = fd.lib()
+    if args.workload == "tapers":
+        if args.steps == N_RUN:
+            args.steps = 200
+        bench_tapers(fd, args)
+        return
     W, S = args.warmup, args.steps
     g, dt, md, mons, omegas = workload(fd, W + S)
     if args.rows is not None or args.kchunk is not None:

@@ -100,17 +100,18 @@ int fdtd_dev_run_dft_result(fdtd_dev *d, int id, double *out);
  * from a captured hipGraph).  Every grid runs under the PML boundary with the split kernels and
  * holds its run configuration (at most 2 run sources and 4 overlap monitors clear of the boundary
  * ring, no DFT monitors).  fdtd_dev_batch_run(b, dt, n0, n) advances steps n0..n0+n-1 of every
- * grid's run; results are read per grid (fdtd_dev_run_overlap_result).  The z planes per thread
- * are picked on the first run by timing real steps with each candidate (2, 3, 4) unless
- * fdtd_dev_batch_set_kchunk fixed them; fdtd_dev_batch_info reports the choice, the candidates'
- * ms per step and the total cell count. */
+ * grid's run; results are read per grid (fdtd_dev_run_overlap_result).  The launch shape (z
+ * planes per thread, planes unrolled or looped, persistent or one block per work unit) is picked
+ * on the first run by timing real steps with each candidate unless fdtd_dev_batch_set_shape fixed
+ * it (kchunk 0 = tune); fdtd_dev_batch_info reports the shape {kchunk, unroll, persistent}, each
+ * candidate's code (kchunk + 16 unroll + 32 persistent) and ms per step, and the cell count. */
 typedef struct fdtd_dev_batch fdtd_dev_batch;
 fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n);
 void fdtd_dev_batch_free(fdtd_dev_batch *b);
-int fdtd_dev_batch_set_kchunk(fdtd_dev_batch *b, int kchunk);
+int fdtd_dev_batch_set_shape(fdtd_dev_batch *b, int kchunk, int unroll, int persistent);
 int fdtd_dev_batch_run(fdtd_dev_batch *b, double dt, int n0, int n);
-int fdtd_dev_batch_info(fdtd_dev_batch *b, int *kchunk, long long *cells, int *ntuned, int tuned_kc[8],
-                        double tuned_ms[8]);
+int fdtd_dev_batch_info(fdtd_dev_batch *b, int shape[3], long long *cells, int *ntuned, int tuned_shape[16],
+                        double tuned_ms[16]);
 int fdtd_dev_batch_timer_start(fdtd_dev_batch *b);
 int fdtd_dev_batch_timer_stop(fdtd_dev_batch *b, double *elapsed_ms);
 

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 fd = importlib.import_module("3d_fdtd_amd")
 
 KS = (30, 47, 64)
-NSTEPS = 170
+NSTEPS = 170  # more than the 10 x 5 steps the shape autotuner times, so graph segments run too
 
 
 def say(*a):
@@ -68,21 +68,23 @@ def single_results():
     return _single()
 
 
-@pytest.mark.parametrize("calls,kchunk", [((0, NSTEPS), 0), ((0, 100, NSTEPS), 0), ((0, 37, NSTEPS), 3)])
-def test_batch_matches_single_grids(single_results, calls, kchunk):
-    say("batch: calls", calls, "kchunk", kchunk)
+@pytest.mark.parametrize("calls,shape", [((0, NSTEPS), None), ((0, 100, NSTEPS), None), ((0, 37, NSTEPS), (3, True, True)),
+                                         ((0, NSTEPS), (5, False, False)), ((0, 41, NSTEPS), (2, False, True))])
+def test_batch_matches_single_grids(single_results, calls, shape):
+    say("batch: calls", calls, "shape", shape)
     built = [_make(K, 1 + q) for q, K in enumerate(KS)]
     b = fd.Batch([g for g, _, _ in built])
-    if kchunk:
-        b.set_kchunk(kchunk)
+    if shape:
+        b.set_shape(*shape)
     for a, c in zip(calls[:-1], calls[1:]):
         b.run(GC.DT, a, c - a)
     info = b.info()
     say("batch info", info)
-    assert info["kchunk"] in (2, 3, 4)
     assert info["cells"] == sum(20 * 20 * K for K in KS)
-    if not kchunk:
-        assert sorted(info["tuned"]) == [2, 3, 4]
+    if shape:
+        assert info["shape"] == fd.Batch.shape_name(*shape)
+    else:
+        assert len(info["tuned"]) == 10 and info["shape"] in info["tuned"]
     for (g, mons, nf), (mon_ref, f_ref) in zip(built, single_results):
         mon, f = _results(g, mons, nf)
         em = np.abs(mon - mon_ref).max() / np.abs(mon_ref).max()
