@@ -166,6 +166,11 @@ SIGNATURES = {
     "fdtd_dev_run_add_overlap": ([C.c_void_p, _i, _i, _i, _P(_d), _i, _i, _d, _i, _i, _P(_d)], _i),
     "fdtd_dev_run": ([C.c_void_p, _d, _i, _i, _d], _i),
     "fdtd_dev_run_overlap_result": ([C.c_void_p, _i, _P(_d), _P(_d)], _i),
+    "fdtd_dev_run_add_dft_box": ([C.c_void_p, _P(_i), _i, _i, _P(_d)], _i),
+    "fdtd_dev_run_add_adjoint_source": ([C.c_void_p, _i, _i, _i, _P(_d), _i, _i, _d, _i, _d, _i, _P(_d), _P(_d)],
+                                        _i),
+    "fdtd_dev_run_dft_result": ([C.c_void_p, _i, _P(_d)], _i),
+    "fdtd_dev_dft_gradient": ([C.c_void_p, _i, C.c_void_p, _i, _i, _d, _d, _P(_d)], _i),
     "fdtd_dev_synchronize": ([C.c_void_p], _i),
     "fdtd_dev_timer_start": ([C.c_void_p], _i),
     "fdtd_dev_timer_stop": ([C.c_void_p, _P(_d)], _i),
@@ -323,6 +328,32 @@ class DevGrid:
                                              len(omegas), omegas.ctypes.data_as(dp))
         dev_check(0 if mid >= 0 else mid, "run_add_overlap")
         return mid
+
+    def add_dft_box(self, box, comp_mask, omegas):
+        dp = C.POINTER(C.c_double)
+        omegas = np.ascontiguousarray(omegas, np.float64)
+        bid = lib().fdtd_dev_run_add_dft_box(self.d, (C.c_int * 6)(*box), comp_mask, len(omegas),
+                                             omegas.ctypes.data_as(dp))
+        dev_check(0 if bid >= 0 else bid, "run_add_dft_box")
+        return bid
+
+    def add_adjoint_source(self, cx, cy, cz, mode, mi, mj, mdisc, omega_dt, amplitude, phase, reverse=0):
+        dp = C.POINTER(C.c_double)
+        amplitude = np.ascontiguousarray(amplitude, np.float64)
+        phase = np.ascontiguousarray(phase, np.float64)
+        mode = np.ascontiguousarray(mode, np.float64)
+        sid = lib().fdtd_dev_run_add_adjoint_source(self.d, cx, cy, cz, mode.ctypes.data_as(dp), mi, mj, mdisc, reverse,
+                                                    omega_dt, len(amplitude), amplitude.ctypes.data_as(dp),
+                                                    phase.ctypes.data_as(dp))
+        dev_check(0 if sid >= 0 else sid, "run_add_adjoint_source")
+        return sid
+
+    def gradient(self, bid, adjoint, aid, freq, weight, nb):
+        """Re[weight * sum_m F_m A_m] over the box (this grid forward, `adjoint` the adjoint grid)."""
+        out = np.zeros(nb)
+        dev_check(lib().fdtd_dev_dft_gradient(self.d, bid, adjoint.d, aid, freq, weight.real, weight.imag,
+                                              out.ctypes.data_as(C.POINTER(C.c_double))), "dft_gradient")
+        return out
 
     def run(self, dt, n0, n, t_offset=0.0):
         dev_check(lib().fdtd_dev_run(self.d, dt, n0, n, t_offset), "run")

@@ -82,6 +82,70 @@ class TaperFamily:
         g.mon_out = g.add_overlap(ci, cj, K - self.npml - 2, self.mode, self.mi, self.mj, self.disc, self.omegas)
         return g
 
+    def spline_basis(self, K):
+        """B [8][K]: d w(k) / d w_i for the cubic spline through the control points."""
+        k0, k1 = self.npml + self.port + 2, K - self.npml - self.port - 2
+        xs = np.linspace(k0, k1 - 1, 8)
+        kk = np.clip(np.arange(K), xs[0], xs[-1])
+        return np.array([CubicSpline(xs, np.eye(8)[i])(kk) for i in range(8)])
+
+    def adjoint_gradient(self, p):
+        """Band transmission T = mean_lambda |t|^2 of design p and dT / dw_i (per um, i = 1..8) from
+        one forward and one adjoint single-geometry run (no batching, no surrogate).
+
+        The adjoint run drives the output monitor's exact adjoint source with the forward pulse's
+        envelope; its E field, DFT'd with step offset 1 over the core rows of the taper section,
+        gives d a_out / d eps(cell) = 2 u / (dt E0(w)) sum_m F_m A_m per wavelength, with
+        E0(w) = sum_n A_n e^{i (w - w0) n dt} and u = e^{-i w dt} - 1 (a_in is held fixed).
+        Widening the core by dw moves each side by dw / 2: eps of the first cladding cell on
+        each side grows by (eps_core - eps_clad) dw / (2 dx).  The spline basis carries that to
+        the control points."""
+        ep, K = self.permittivity(p)
+        ci, cj = self.I // 2, self.J // 2
+        ks, km = self.npml + 1, K - self.npml - 2
+        j0 = self.J // 2 - self.core // 2
+        box = (0, self.I, j0, j0 + self.core, ks + 4, km)
+        nb = self.I * self.core * (km - ks - 4)
+        nf = len(self.omegas)
+        g = fd.DevGrid(self.I, self.J, K, self.disc, ep)
+        g.cpml((self.npml,) * 6, 3, 1e-8, 1.0, 0.05)
+        g.add_source(ci, cj, ks, self.mode, self.mi, self.mj, self.disc, self.amp, self.phase)
+        mon_in = g.add_overlap(ci, cj, ks + 3, self.mode, self.mi, self.mj, self.disc, self.omegas)
+        mon_out = g.add_overlap(ci, cj, km, self.mode, self.mi, self.mj, self.disc, self.omegas)
+        bid = g.add_dft_box(box, 7, self.omegas)
+        g.run(self.dt, 0, self.steps)
+        a_in, _ = g.overlap(mon_in, nf)
+        a_out, _ = g.overlap(mon_out, nf)
+        ga = fd.DevGrid(self.I, self.J, K, self.disc, ep)
+        ga.cpml((self.npml,) * 6, 3, 1e-8, 1.0, 0.05)
+        w0 = self.phase[1] / self.dt
+        ga.add_adjoint_source(ci, cj, km, self.mode, self.mi, self.mj, self.disc, w0 * self.dt, self.amp, self.phase)
+        aid = ga.add_dft_box(box, 7, self.omegas)
+        ga.run(self.dt, 0, self.steps, 1.0)
+        t = a_out / a_in
+        n = np.arange(self.steps)
+        grad_eps = np.zeros(nb)
+        for f, w in enumerate(self.omegas):
+            E0 = np.sum(self.amp * np.exp(1j * (w - w0) * n * self.dt))
+            u = np.exp(-1j * w * self.dt) - 1.0
+            weight = 2.0 * np.conj(t[f]) * (2.0 * u / (self.dt * E0)) / a_in[f] / nf
+            grad_eps += g.gradient(bid, ga, aid, f, weight, nb)
+        g.free()
+        ga.free()
+        grad_eps = grad_eps.reshape(km - ks - 4, self.core, self.I)
+        widths, L = np.asarray(p[:8]), float(p[8])
+        k0, k1 = self.npml + self.port + 2, K - self.npml - self.port - 2
+        xs = np.linspace(k0, k1 - 1, 8)
+        wk = CubicSpline(xs, widths)(np.clip(np.arange(K), xs[0], xs[-1])) / self.disc
+        dT_dw = np.zeros(K)
+        deps = (3.48 ** 2 - 1.44 ** 2) / (2 * self.disc)
+        for q, k in enumerate(range(ks + 4, km)):
+            ir = int(np.ceil(self.I / 2 + wk[k] / 2 - 0.5))  # first cladding cell right of the core
+            il = self.I - 1 - ir
+            if 0 <= ir < self.I and 0 <= il < self.I:
+                dT_dw[k] = deps * (grad_eps[q, :, ir].sum() + grad_eps[q, :, il].sum())
+        return float(band_transmission(t)), self.spline_basis(K) @ dT_dw
+
     def simulate(self, designs, batch=64):
         """Complex transmission t [n][nfreq] of each design, by batched FDTD on the GPU."""
         out = []
@@ -155,3 +219,18 @@ def screen(family, net, candidates, top):
     """Indices of the `top` candidates with the highest predicted band transmission."""
     pred = band_transmission(from_outputs(net.forward(family.normalise(candidates))))
     return np.argsort(-pred)[:top], pred
+
+
+def adjoint_optimize(family, p0, iters, spec, step=0.1):
+    """The plain method: gradient ascent on the band transmission of one design at a time with
+    the adjoint gradient (two single-geometry FDTD runs per iteration); widths move by at most
+    `step` um per iteration.  Returns the final design and the transmission history."""
+    p = np.array(p0, np.float64)
+    hist = []
+    for _ in range(iters):
+        T, g = family.adjoint_gradient(p)
+        hist.append(T)
+        if T >= spec:
+            break
+        p[:8] = np.clip(p[:8] + step * g / max(np.abs(g).max(), 1e-30), family.wmin, family.wmax)
+    return p, hist

@@ -175,6 +175,61 @@ def bench_tapers(fd, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_design(fd, args):
+    """Verified taper designs per hour: the surrogate loop (FDTD training set, BF16-MFMA network
+    trained on the GPU, rounds that screen 10^5 candidates and verify the top 64 by batched FDTD)
+    against the plain loop (one design at a time, gradient ascent with the adjoint gradient: two
+    single-geometry FDTD runs per iteration).  Accepted = band |t|^2 >= spec by FDTD, the judge of
+    every accepted design in both loops."""
+    design = importlib.import_module("3d_fdtd_amd.design")
+    fam = design.TaperFamily()
+    spec = 0.7
+    rng = np.random.default_rng(TAPER_SEED)
+    print("bench: plain loop (single-geometry FDTD + adjoint)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    plain_ok = 0
+    iters = 0
+    for s in range(args.plain_starts):
+        p, hist = design.adjoint_optimize(fam, fam.random(rng, 1)[0], 12, spec)
+        iters += len(hist)
+        plain_ok += int(hist[-1] >= spec)
+    t_plain = time.perf_counter() - t0
+    print(f"bench: surrogate loop ({args.train} training designs, {args.rounds} rounds)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    train = fam.random(rng, args.train)
+    t_train = fam.simulate(train)
+    t_sim = time.perf_counter() - t0
+    net = design.fit(fam, train, t_train, steps=1500)
+    t_fit = time.perf_counter() - t0 - t_sim
+    accepted, verified, errs = 0, 0, []
+    for r in range(args.rounds):
+        pool = fam.random(rng, 100000)
+        top, pred = design.screen(fam, net, pool, 64)
+        T = design.band_transmission(fam.simulate(pool[top]))
+        accepted += int((T >= spec).sum())
+        verified += len(T)
+        errs.extend(np.abs(pred[top] - T).tolist())
+    t_sur = time.perf_counter() - t0
+    net.free()
+    rate_sur = accepted / t_sur * 3600
+    rate_plain = plain_ok / t_plain * 3600 if plain_ok else 0.0
+    out = {"metric": METRIC, "value": rate_sur, "unit": "verified taper designs/hour", "n_gpus": 1,
+           "steps": fam.steps, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 fields, bf16 surrogate", "data": "synthetic",
+           "config": {"workload": "spline tapers (8 widths U[0.4,2.0] um, length U[1.5,4] um, 50 nm cells, 48x20 "
+                                  "cross-section, 16 wavelengths 1.50-1.60 um); surrogate MLP 9-256x4-32 GELU",
+                      "spec": "band |t|^2 >= 0.7 by FDTD", "train_designs": args.train, "rounds": args.rounds,
+                      "screened_per_round": 100000, "verified_per_round": 64, "plain_starts": args.plain_starts},
+           "design_loop": {"surrogate": {"accepted": accepted, "verified": verified, "seconds": round(t_sur, 2),
+                                         "fdtd_training_s": round(t_sim, 2), "training_s": round(t_fit, 2),
+                                         "designs_per_hour": round(rate_sur, 1),
+                                         "median_abs_error_on_verified": round(float(np.median(errs)), 4)},
+                           "plain": {"accepted": plain_ok, "starts": args.plain_starts, "iterations": iters,
+                                     "seconds": round(t_plain, 2), "designs_per_hour": round(rate_plain, 1)},
+                           "ratio": round(rate_sur / rate_plain, 2) if rate_plain else None}}
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(target_s=12.0):
     """Reference FP64 step on the same grid size, host cores, bounded sample."""
     threads = int(os.environ.get("FDTD3D_CPU_THREADS", min(16, os.cpu_count() or 1)))
@@ -254,8 +309,12 @@ def main():
     ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
     ap.add_argument("--kchunk", type=int, default=None,
                     help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
-    ap.add_argument("--workload", choices=("waveguide", "tapers"), default="waveguide",
-                    help="waveguide: the strip waveguide (default); tapers: the batched taper set")
+    ap.add_argument("--workload", choices=("waveguide", "tapers", "design"), default="waveguide",
+                    help="waveguide: the strip waveguide (default); tapers: the batched taper set; design: "
+                         "verified taper designs/hour, surrogate loop vs plain adjoint loop")
+    ap.add_argument("--train", type=int, default=4096, help="design workload: FDTD training designs")
+    ap.add_argument("--rounds", type=int, default=20, help="design workload: screening rounds")
+    ap.add_argument("--plain-starts", type=int, default=32, help="design workload: plain-loop starting designs")
     ap.add_argument("--tapers", type=int, default=64, help="number of taper grids of the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB (HBM counter calibration)")
@@ -273,6 +332,9 @@ def main():
         if args.steps == N_RUN:
             args.steps = 200
         bench_tapers(fd, args)
+        return
+    if args.workload == "design":
+        bench_design(fd, args)
         return
     W, S = args.warmup, args.steps
     g, dt, md, mons, omegas = workload(fd, W + S)
