@@ -172,6 +172,7 @@ SIGNATURES = {
     "fdtd_dev_run_dft_result": ([C.c_void_p, _i, _P(_d)], _i),
     "fdtd_dev_dft_gradient": ([C.c_void_p, _i, C.c_void_p, _i, _i, _d, _d, _P(_d)], _i),
     "fdtd_dev_synchronize": ([C.c_void_p], _i),
+    "fdtd_dev_release_cached_memory": ([], _i),
     "fdtd_dev_timer_start": ([C.c_void_p], _i),
     "fdtd_dev_timer_stop": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_batch_new": ([_P(C.c_void_p), _i], C.c_void_p),
@@ -300,12 +301,22 @@ class DevGrid:
         self.d = lib().fdtd_dev_create(I, J, K, disc)
         if not self.d:
             raise RuntimeError("fdtd_dev_create: " + lib().fdtd_dev_last_error().decode())
-        ep = np.ascontiguousarray(ep, np.float32)
-        mu = np.ones(3 * n, np.float32) if mu is None else np.ascontiguousarray(mu, np.float32)
-        sig = np.zeros(6 * n, np.float32) if sig is None else np.ascontiguousarray(sig, np.float32)
         fp = C.POINTER(C.c_float)
-        dev_check(lib().fdtd_dev_set_materials(self.d, ep.ctypes.data_as(fp), mu.ctypes.data_as(fp),
-                                               sig.ctypes.data_as(fp)), "set_materials")
+
+        def arr(a, size):  # None -> NULL (the library's mu = 1 / sig = 0 defaults)
+            if a is None:
+                return None, None
+            a = np.ascontiguousarray(a, np.float32)
+            if a.size != size:
+                raise ValueError(f"material array of {a.size} values for {size}")
+            return a, a.ctypes.data_as(fp)
+
+        if ep is None:
+            raise ValueError("DevGrid needs a permittivity array")
+        ep, ep_p = arr(ep, 3 * n)
+        mu, mu_p = arr(mu, 3 * n)
+        sig, sig_p = arr(sig, 6 * n)
+        dev_check(lib().fdtd_dev_set_materials(self.d, ep_p, mu_p, sig_p), "set_materials")
 
     def cpml(self, npml, order=3, R0=1e-8, kappa_max=1.0, alpha_max=0.05):
         dev_check(lib().fdtd_dev_cpml_on(self.d, (C.c_int * 6)(*npml), order, R0, kappa_max, alpha_max), "cpml_on")

@@ -270,9 +270,9 @@ fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n) {
     b->host.resize((size_t)n);
     std::memset((void *)b->host.data(), 0, b->host.size() * sizeof(BatchGeom));
     b->u8 = grids[0]->mat_bytes == 1;
-    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&b->step, sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&b->geo, (size_t)n * sizeof(BatchGeom));
+    hipError_t e = dev_stream_acquire(&b->stream);
+    if (e == hipSuccess) e = dev_malloc(&b->step, sizeof(int));
+    if (e == hipSuccess) e = dev_malloc(&b->geo, (size_t)n * sizeof(BatchGeom));
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e != hipSuccess) {
@@ -287,11 +287,11 @@ void fdtd_dev_batch_free(fdtd_dev_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    (void)hipFree(b->step);
-    (void)hipFree(b->geo);
+    (void)dev_free(b->step);
+    (void)dev_free(b->geo);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream) (void)dev_stream_release(b->stream);
     delete b;
 }
 

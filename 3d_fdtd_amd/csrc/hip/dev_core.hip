@@ -128,7 +128,7 @@ const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, do
         size_t victim = 0;
         for (size_t q = 1; q < d->modes.size(); q++)
             if (d->modes[q].last_use < d->modes[victim].last_use) victim = q;
-        (void)hipFree(d->modes[victim].dev);
+        (void)dev_free(d->modes[victim].dev);
         d->modes.erase(d->modes.begin() + victim);
     }
     ModeCache mc;
@@ -138,7 +138,7 @@ const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, do
     mc.mdisc = mdisc;
     mc.shadow.assign(mode, mode + n);
     mc.last_use = d->use_clock;
-    hipError_t e = hipMalloc(&mc.dev, n * sizeof(double));
+    hipError_t e = dev_malloc(&mc.dev, n * sizeof(double));
     if (e == hipSuccess) e = hipMemcpyAsync(mc.dev, mode, n * sizeof(double), hipMemcpyHostToDevice, d->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
     if (e != hipSuccess) {
@@ -192,10 +192,10 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->fused_rows = 0;  // split E/H kernels: measured faster than the fused step on MI355X
     d->fused_kchunk = 8;
     d->split_kchunk = 0;
-    hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    hipError_t e = dev_stream_acquire(&d->stream);
     // fields: 6 arrays, each N floats, each starting on a 256-byte boundary
     d->Nal = (d->N + 63) / 64 * 64;
-    if (e == hipSuccess) e = hipMalloc(&d->field_base, (size_t)6 * d->Nal * sizeof(float));
+    if (e == hipSuccess) e = dev_malloc(&d->field_base, (size_t)6 * d->Nal * sizeof(float));
     if (e == hipSuccess) e = hipMemsetAsync(d->field_base, 0, (size_t)6 * d->Nal * sizeof(float), d->stream);
     for (int m = 0; m < 6; m++) d->F[m] = d->field_base + m * d->Nal;
     // shells: faces x0,y0,z0,xend,yend,zend; each 2 layers x 4 components x area
@@ -205,9 +205,9 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
         d->shell_off[f] = off;
         off += 8 * area[f % 3];
     }
-    if (e == hipSuccess) e = hipMalloc(&d->shell, (size_t)off * sizeof(float));
+    if (e == hipSuccess) e = dev_malloc(&d->shell, (size_t)off * sizeof(float));
     if (e == hipSuccess) e = hipMemsetAsync(d->shell, 0, (size_t)off * sizeof(float), d->stream);
-    if (e == hipSuccess) e = hipMalloc(&d->red_out, sizeof(double2) * 4);
+    if (e == hipSuccess) e = dev_malloc(&d->red_out, sizeof(double2) * 4);
     if (e == hipSuccess) e = hipEventCreate(&d->ev0);
     if (e == hipSuccess) e = hipEventCreate(&d->ev1);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
@@ -222,23 +222,23 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
 void fdtd_dev_destroy(fdtd_dev *d) {
     if (!d) return;
     if (d->stream) (void)hipStreamSynchronize(d->stream);
-    (void)hipFree(d->field_base);
-    (void)hipFree(d->alt_base);
-    (void)hipFree(d->mat);
-    (void)hipFree(d->d_props);
-    (void)hipFree(d->d_coef);
-    (void)hipFree(d->shell);
-    (void)hipFree(d->dense_src);
+    (void)dev_free(d->field_base);
+    (void)dev_free(d->alt_base);
+    (void)dev_free(d->mat);
+    (void)dev_free(d->d_props);
+    (void)dev_free(d->d_coef);
+    (void)dev_free(d->shell);
+    (void)dev_free(d->dense_src);
     fdtd_dev_cpml_off(d);
     run_release(d);
-    (void)hipFree(d->fourier);
-    (void)hipFree(d->partials);
-    (void)hipFree(d->red_out);
-    for (auto &p : d->src) (void)hipFree(p.J);
-    for (auto &m : d->modes) (void)hipFree(m.dev);
+    (void)dev_free(d->fourier);
+    (void)dev_free(d->partials);
+    (void)dev_free(d->red_out);
+    for (auto &p : d->src) (void)dev_free(p.J);
+    for (auto &m : d->modes) (void)dev_free(m.dev);
     if (d->ev0) (void)hipEventDestroy(d->ev0);
     if (d->ev1) (void)hipEventDestroy(d->ev1);
-    if (d->stream) (void)hipStreamDestroy(d->stream);
+    if (d->stream) (void)dev_stream_release(d->stream);
     delete d;
 }
 
@@ -248,42 +248,47 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
     std::unordered_map<MatKey, int, MatKeyHash> ids;
     std::vector<uint16_t> cell_id((size_t)d->N);
     std::vector<float> props;
+    static const float kOne3[3] = {1.0f, 1.0f, 1.0f};
+    static const float kZero6[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     MatKey prev;
     int prev_id = -1;
     for (long long c = 0; c < d->N; c++) {
-        MatKey k;
-        std::memcpy(k.v, ep + 3 * c, 3 * sizeof(float));
-        std::memcpy(k.v + 3, mu + 3 * c, 3 * sizeof(float));
-        std::memcpy(k.v + 6, sig + 6 * c, 6 * sizeof(float));
-        int id;
-        if (prev_id >= 0 && k == prev) {
-            id = prev_id;
-        } else {
-            auto it = ids.find(k);
-            if (it == ids.end()) {
-                id = (int)ids.size();
-                if (id >= 65536) {
-                    set_error("more than 65536 distinct (ep, mu, sigma) tuples are not supported");
-                    return -2;
-                }
-                ids.emplace(k, id);
-                props.resize((size_t)(id + 1) * kPropStride, 0.0f);
-                std::memcpy(&props[(size_t)id * kPropStride], k.v, sizeof k.v);
-            } else {
-                id = it->second;
-            }
-            prev = k;
-            prev_id = id;
+        // runs of equal cells are the common case: compare in place before building a key
+        const float *e3 = ep + 3 * c, *m3 = mu ? mu + 3 * c : kOne3, *s6 = sig ? sig + 6 * c : kZero6;
+        if (prev_id >= 0 && std::memcmp(e3, prev.v, 3 * sizeof(float)) == 0 &&
+            std::memcmp(m3, prev.v + 3, 3 * sizeof(float)) == 0 && std::memcmp(s6, prev.v + 6, 6 * sizeof(float)) == 0) {
+            cell_id[c] = (uint16_t)prev_id;
+            continue;
         }
+        MatKey k;
+        std::memcpy(k.v, e3, 3 * sizeof(float));
+        std::memcpy(k.v + 3, m3, 3 * sizeof(float));
+        std::memcpy(k.v + 6, s6, 6 * sizeof(float));
+        int id;
+        auto it = ids.find(k);
+        if (it == ids.end()) {
+            id = (int)ids.size();
+            if (id >= 65536) {
+                set_error("more than 65536 distinct (ep, mu, sigma) tuples are not supported");
+                return -2;
+            }
+            ids.emplace(k, id);
+            props.resize((size_t)(id + 1) * kPropStride, 0.0f);
+            std::memcpy(&props[(size_t)id * kPropStride], k.v, sizeof k.v);
+        } else {
+            id = it->second;
+        }
+        prev = k;
+        prev_id = id;
         cell_id[c] = (uint16_t)id;
     }
     int nmat = (int)ids.size();
     int bytes = nmat <= 256 ? 1 : 2;
     if (d->mat && bytes != d->mat_bytes) {
-        FDTD_TRY(hipFree(d->mat));
+        FDTD_TRY(dev_free(d->mat));
         d->mat = nullptr;
     }
-    if (!d->mat) FDTD_TRY(hipMalloc(&d->mat, (size_t)d->N * bytes));
+    if (!d->mat) FDTD_TRY(dev_malloc(&d->mat, (size_t)d->N * bytes));
     if (bytes == 1) {
         std::vector<uint8_t> c8((size_t)d->N);
         for (long long c = 0; c < d->N; c++) c8[c] = (uint8_t)cell_id[c];
@@ -292,12 +297,12 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
         FDTD_TRY(hipMemcpy(d->mat, cell_id.data(), (size_t)d->N * 2, hipMemcpyHostToDevice));
     }
     if (nmat != d->nmat || !d->d_props) {
-        (void)hipFree(d->d_props);
-        (void)hipFree(d->d_coef);
+        (void)dev_free(d->d_props);
+        (void)dev_free(d->d_coef);
         d->d_props = nullptr;
         d->d_coef = nullptr;
-        FDTD_TRY(hipMalloc(&d->d_props, (size_t)nmat * kPropStride * sizeof(float)));
-        FDTD_TRY(hipMalloc(&d->d_coef, (size_t)2 * nmat * kCoefStride * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->d_props, (size_t)nmat * kPropStride * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->d_coef, (size_t)2 * nmat * kCoefStride * sizeof(float)));
     }
     FDTD_TRY(hipMemcpy(d->d_props, props.data(), props.size() * sizeof(float), hipMemcpyHostToDevice));
     d->props = std::move(props);
@@ -311,7 +316,7 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
 int fdtd_dev_set_fields(fdtd_dev *d, const double *fields) {
     d->alt_valid = false;
     double *tmp = nullptr;
-    FDTD_TRY(hipMalloc(&tmp, (size_t)6 * d->N * sizeof(double)));
+    FDTD_TRY(dev_malloc(&tmp, (size_t)6 * d->N * sizeof(double)));
     hipError_t e = hipMemcpyAsync(tmp, fields, (size_t)6 * d->N * sizeof(double), hipMemcpyHostToDevice, d->stream);
     if (e == hipSuccess) {
         unsigned nb = (unsigned)((d->N + kBlock - 1) / kBlock);
@@ -319,21 +324,21 @@ int fdtd_dev_set_fields(fdtd_dev *d, const double *fields) {
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
-    (void)hipFree(tmp);
+    (void)dev_free(tmp);
     if (e != hipSuccess) return fail("fdtd_dev_set_fields", e);
     return 0;
 }
 
 int fdtd_dev_get_fields(fdtd_dev *d, double *fields) {
     double *tmp = nullptr;
-    FDTD_TRY(hipMalloc(&tmp, (size_t)6 * d->N * sizeof(double)));
+    FDTD_TRY(dev_malloc(&tmp, (size_t)6 * d->N * sizeof(double)));
     unsigned nb = (unsigned)((d->N + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_soa_to_aos, dim3(nb), dim3(kBlock), 0, d->stream, d->field_base, tmp, d->N, d->Nal);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipMemcpyAsync(fields, tmp, (size_t)6 * d->N * sizeof(double), hipMemcpyDeviceToHost, d->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
-    (void)hipFree(tmp);
+    (void)dev_free(tmp);
     if (e != hipSuccess) return fail("fdtd_dev_get_fields", e);
     return 0;
 }
@@ -348,16 +353,16 @@ int fdtd_dev_set_sources(fdtd_dev *d, const float *sources) {
         if (nz) planes.push_back(k);
     }
     FDTD_TRY(hipStreamSynchronize(d->stream));
-    for (auto &p : d->src) (void)hipFree(p.J);
+    for (auto &p : d->src) (void)dev_free(p.J);
     d->src.clear();
-    (void)hipFree(d->dense_src);
+    (void)dev_free(d->dense_src);
     d->dense_src = nullptr;
     if ((int)planes.size() > kMaxSrcPlanes) {
         // sources spread over many planes: keep the whole 6 x N volume (extra stream per cell)
         std::vector<float> soa((size_t)6 * d->N);
         for (long long c = 0; c < d->N; c++)
             for (int m = 0; m < 6; m++) soa[m * d->N + c] = sources[6 * c + m];
-        FDTD_TRY(hipMalloc(&d->dense_src, soa.size() * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->dense_src, soa.size() * sizeof(float)));
         FDTD_TRY(hipMemcpy(d->dense_src, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
         return 0;
     }
@@ -369,7 +374,7 @@ int fdtd_dev_set_sources(fdtd_dev *d, const float *sources) {
         SrcPlane sp;
         sp.k = k;
         sp.stride = d->P;
-        FDTD_TRY(hipMalloc(&sp.J, (size_t)6 * d->P * sizeof(float)));
+        FDTD_TRY(dev_malloc(&sp.J, (size_t)6 * d->P * sizeof(float)));
         FDTD_TRY(hipMemcpy(sp.J, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
         d->src.push_back(sp);
     }
@@ -428,7 +433,7 @@ int fdtd_dev_get_fourier(fdtd_dev *d, double *out) {
 }
 
 int fdtd_dev_set_fourier(fdtd_dev *d, const double *in) {
-    if (!d->fourier) FDTD_TRY(hipMalloc(&d->fourier, (size_t)6 * d->N * sizeof(double2)));
+    if (!d->fourier) FDTD_TRY(dev_malloc(&d->fourier, (size_t)6 * d->N * sizeof(double2)));
     std::vector<double2> soa((size_t)6 * d->N);
     for (long long c = 0; c < d->N; c++)
         for (int m = 0; m < 6; m++) soa[m * d->N + c] = make_double2(in[2 * (6 * c + m)], in[2 * (6 * c + m) + 1]);

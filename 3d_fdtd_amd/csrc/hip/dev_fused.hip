@@ -562,7 +562,7 @@ __global__ __launch_bounds__(256) void k_step_tiled(FusedArgs a) {
 // ping-pong buffers agree on every cell the fused kernel does not write.
 int fused_sync_alt(fdtd_dev *d) {
     if (!d->alt_base) {
-        FDTD_TRY(hipMalloc(&d->alt_base, (size_t)6 * d->Nal * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->alt_base, (size_t)6 * d->Nal * sizeof(float)));
         d->alt_valid = false;
     }
     if (!d->alt_valid) {
@@ -588,7 +588,7 @@ int fused_step(fdtd_dev *d, double dt, int ty_rows, int kc_len) {
     Cpml &cp = d->cpml;
     const long long pmem = 4 * (cp.psi_n[0] + cp.psi_n[1] + cp.psi_n[2]);
     if (cp.on && cp.psi_base && !cp.psi_alt) {
-        FDTD_TRY(hipMalloc(&cp.psi_alt, (size_t)pmem * sizeof(float)));
+        FDTD_TRY(dev_malloc(&cp.psi_alt, (size_t)pmem * sizeof(float)));
         FDTD_TRY(hipMemsetAsync(cp.psi_alt, 0, (size_t)pmem * sizeof(float), d->stream));
     }
     FusedArgs a{};

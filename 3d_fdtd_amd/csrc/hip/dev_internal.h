@@ -14,6 +14,19 @@
 
 namespace fdtd {
 
+// Device memory and streams of the device layer come from a cache (dev_alloc.hip): a released
+// block is reused after one device synchronisation, so release it only once work that uses it
+// has been enqueued (never while the host still needs its content); release a stream only after
+// synchronising it.
+hipError_t dev_malloc_bytes(void **p, size_t bytes);
+template <class T>
+inline hipError_t dev_malloc(T **p, size_t bytes) {
+    return dev_malloc_bytes(reinterpret_cast<void **>(p), bytes);
+}
+hipError_t dev_free(void *p);
+hipError_t dev_stream_acquire(hipStream_t *s);
+hipError_t dev_stream_release(hipStream_t s);
+
 // Component order inside a cell: Ex, Ey, Ez, Hx, Hy, Hz (timefield3d.h:19-25).
 enum { EX = 0, EY, EZ, HX, HY, HZ };
 

@@ -206,15 +206,15 @@ int gemm(fdtd_mlp *m, int epi, const unsigned short *A, long long a_m, long long
 }
 
 void free_buffers(fdtd_mlp *m) {
-    (void)hipFree(m->xin);
-    for (auto *q : m->z) (void)hipFree(q);
-    for (auto *q : m->h) (void)hipFree(q);
-    (void)hipFree(m->dz);
-    (void)hipFree(m->dx);
-    (void)hipFree(m->dzb);
-    (void)hipFree(m->xf);
-    (void)hipFree(m->tf);
-    (void)hipFree(m->partial);
+    (void)dev_free(m->xin);
+    for (auto *q : m->z) (void)dev_free(q);
+    for (auto *q : m->h) (void)dev_free(q);
+    (void)dev_free(m->dz);
+    (void)dev_free(m->dx);
+    (void)dev_free(m->dzb);
+    (void)dev_free(m->xf);
+    (void)dev_free(m->tf);
+    (void)dev_free(m->partial);
     m->xin = nullptr;
     m->z.assign(m->nl, nullptr);
     m->h.assign(m->nl, nullptr);
@@ -231,18 +231,18 @@ int ensure_capacity(fdtd_mlp *m, int n) {
     free_buffers(m);
     const size_t N = (size_t)n;
     const int wmax = m->width > m->n_out ? m->width : m->n_out;
-    FDTD_TRY(hipMalloc(&m->xin, N * m->kin * sizeof(unsigned short)));
+    FDTD_TRY(dev_malloc(&m->xin, N * m->kin * sizeof(unsigned short)));
     for (int l = 0; l < m->nl; l++) {
-        FDTD_TRY(hipMalloc(&m->z[l], N * m->fout[l] * sizeof(float)));
-        if (l < m->nl - 1) FDTD_TRY(hipMalloc(&m->h[l], N * m->fout[l] * sizeof(unsigned short)));
+        FDTD_TRY(dev_malloc(&m->z[l], N * m->fout[l] * sizeof(float)));
+        if (l < m->nl - 1) FDTD_TRY(dev_malloc(&m->h[l], N * m->fout[l] * sizeof(unsigned short)));
     }
     const int dmax = wmax > m->kin ? wmax : m->kin;
-    FDTD_TRY(hipMalloc(&m->dz, N * dmax * sizeof(float)));
-    FDTD_TRY(hipMalloc(&m->dx, N * dmax * sizeof(float)));
-    FDTD_TRY(hipMalloc(&m->dzb, N * dmax * sizeof(unsigned short)));
-    FDTD_TRY(hipMalloc(&m->xf, N * m->n_in * sizeof(float)));
-    FDTD_TRY(hipMalloc(&m->tf, N * m->n_out * sizeof(float)));
-    FDTD_TRY(hipMalloc(&m->partial, (N * m->n_out / 256 + 2) * sizeof(double)));
+    FDTD_TRY(dev_malloc(&m->dz, N * dmax * sizeof(float)));
+    FDTD_TRY(dev_malloc(&m->dx, N * dmax * sizeof(float)));
+    FDTD_TRY(dev_malloc(&m->dzb, N * dmax * sizeof(unsigned short)));
+    FDTD_TRY(dev_malloc(&m->xf, N * m->n_in * sizeof(float)));
+    FDTD_TRY(dev_malloc(&m->tf, N * m->n_out * sizeof(float)));
+    FDTD_TRY(dev_malloc(&m->partial, (N * m->n_out / 256 + 2) * sizeof(double)));
     m->cap = n;
     return 0;
 }
@@ -344,13 +344,13 @@ fdtd_mlp *fdtd_mlp_new(int n_in, int width, int depth, int n_out) {
     // gradients use the padded layout for layer 0's weights too; the parameter vector keeps the
     // real input count, so layer 0's gradient GEMM writes into a padded scratch first (see
     // fdtd_mlp_loss_grad)
-    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&m->p, (size_t)m->np * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&m->g, (size_t)(m->np + (long long)m->fout[0] * m->kin) * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&m->mom, (size_t)m->np * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&m->vel, (size_t)m->np * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&m->wb, (size_t)m->nwb * sizeof(unsigned short));
-    if (e == hipSuccess) e = hipMalloc(&m->wseg, (size_t)3 * m->nl * sizeof(long long));
+    hipError_t e = dev_stream_acquire(&m->stream);
+    if (e == hipSuccess) e = dev_malloc(&m->p, (size_t)m->np * sizeof(float));
+    if (e == hipSuccess) e = dev_malloc(&m->g, (size_t)(m->np + (long long)m->fout[0] * m->kin) * sizeof(float));
+    if (e == hipSuccess) e = dev_malloc(&m->mom, (size_t)m->np * sizeof(float));
+    if (e == hipSuccess) e = dev_malloc(&m->vel, (size_t)m->np * sizeof(float));
+    if (e == hipSuccess) e = dev_malloc(&m->wb, (size_t)m->nwb * sizeof(unsigned short));
+    if (e == hipSuccess) e = dev_malloc(&m->wseg, (size_t)3 * m->nl * sizeof(long long));
     if (e == hipSuccess) e = hipMemset(m->mom, 0, (size_t)m->np * sizeof(float));
     if (e == hipSuccess) e = hipMemset(m->vel, 0, (size_t)m->np * sizeof(float));
     if (e == hipSuccess) e = hipMemset(m->wb, 0, (size_t)m->nwb * sizeof(unsigned short));
@@ -366,13 +366,13 @@ void fdtd_mlp_free(fdtd_mlp *m) {
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     free_buffers(m);
-    (void)hipFree(m->p);
-    (void)hipFree(m->g);
-    (void)hipFree(m->mom);
-    (void)hipFree(m->vel);
-    (void)hipFree(m->wb);
-    (void)hipFree(m->wseg);
-    if (m->stream) (void)hipStreamDestroy(m->stream);
+    (void)dev_free(m->p);
+    (void)dev_free(m->g);
+    (void)dev_free(m->mom);
+    (void)dev_free(m->vel);
+    (void)dev_free(m->wb);
+    (void)dev_free(m->wseg);
+    if (m->stream) (void)dev_stream_release(m->stream);
     delete m;
 }
 

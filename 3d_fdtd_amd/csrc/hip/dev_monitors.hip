@@ -236,7 +236,7 @@ SrcPlane *plane_for(fdtd_dev *d, int k, int *err) {
     SrcPlane sp;
     sp.k = k;
     sp.stride = d->P;
-    hipError_t e = hipMalloc(&sp.J, (size_t)6 * d->P * sizeof(float));
+    hipError_t e = dev_malloc(&sp.J, (size_t)6 * d->P * sizeof(float));
     if (e == hipSuccess) e = hipMemsetAsync(sp.J, 0, (size_t)6 * d->P * sizeof(float), d->stream);
     if (e != hipSuccess) {
         *err = fail("source plane", e);
@@ -295,10 +295,10 @@ int fdtd_dev_set_source_plane(fdtd_dev *d, int k, const float *plane, const unsi
         for (int m = 0; m < 6; m++) soa[m * d->P + c] = plane[6 * c + m];
     float *tmp = nullptr;
     unsigned char *dmask = nullptr;
-    FDTD_TRY(hipMalloc(&tmp, soa.size() * sizeof(float)));
+    FDTD_TRY(dev_malloc(&tmp, soa.size() * sizeof(float)));
     hipError_t e = hipMemcpy(tmp, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess && mask) {
-        e = hipMalloc(&dmask, (size_t)d->P);
+        e = dev_malloc(&dmask, (size_t)d->P);
         if (e == hipSuccess) e = hipMemcpy(dmask, mask, (size_t)d->P, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) {
@@ -308,8 +308,8 @@ int fdtd_dev_set_source_plane(fdtd_dev *d, int k, const float *plane, const unsi
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
-    (void)hipFree(tmp);
-    (void)hipFree(dmask);
+    (void)dev_free(tmp);
+    (void)dev_free(dmask);
     if (e != hipSuccess) return fail("fdtd_dev_set_source_plane", e);
     return 0;
 }
@@ -325,9 +325,9 @@ int fdtd_dev_mode_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mod
     if (!dm) return err;
     int nb = (int)((d->P + kBlock - 1) / kBlock);
     if (nb > d->partial_cap) {
-        (void)hipFree(d->partials);
+        (void)dev_free(d->partials);
         d->partials = nullptr;
-        FDTD_TRY(hipMalloc(&d->partials, (size_t)nb * sizeof(double2)));
+        FDTD_TRY(dev_malloc(&d->partials, (size_t)nb * sizeof(double2)));
         d->partial_cap = nb;
     }
     double ratio = d->disc / mdisc;
@@ -347,7 +347,7 @@ int fdtd_dev_mode_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mod
 
 int fdtd_dev_dft(fdtd_dev *d, double tw_re, double tw_im, int y_slice, int k0, int k1) {
     if (!d->fourier) {
-        FDTD_TRY(hipMalloc(&d->fourier, (size_t)6 * d->N * sizeof(double2)));
+        FDTD_TRY(dev_malloc(&d->fourier, (size_t)6 * d->N * sizeof(double2)));
         FDTD_TRY(hipMemsetAsync(d->fourier, 0, (size_t)6 * d->N * sizeof(double2), d->stream));
     }
     if (y_slice >= d->J) {
@@ -388,11 +388,11 @@ int fdtd_dev_taper_store(fdtd_dev *d, const void *key, int which, int seg_num, i
         th.L = L;
         th.T = duration;
         size_t n = (size_t)12 * H * L * duration;
-        FDTD_TRY(hipMalloc(&th.hist[0], n * sizeof(float)));
-        FDTD_TRY(hipMalloc(&th.hist[1], n * sizeof(float)));
+        FDTD_TRY(dev_malloc(&th.hist[0], n * sizeof(float)));
+        FDTD_TRY(dev_malloc(&th.hist[1], n * sizeof(float)));
         FDTD_TRY(hipMemset(th.hist[0], 0, n * sizeof(float)));
         FDTD_TRY(hipMemset(th.hist[1], 0, n * sizeof(float)));
-        FDTD_TRY(hipMalloc(&th.edge_i, (size_t)2 * L * sizeof(int)));
+        FDTD_TRY(dev_malloc(&th.edge_i, (size_t)2 * L * sizeof(int)));
         it = g_tapers.emplace(key, th).first;
     }
     TaperHist &th = it->second;
@@ -441,13 +441,13 @@ int fdtd_dev_taper_ep_gradient(const void *key, int height, int length, int dura
     }
     double *g = nullptr;
     FDTD_TRY(hipDeviceSynchronize());
-    FDTD_TRY(hipMalloc(&g, sizeof(double) * 2 * height * length));
+    FDTD_TRY(dev_malloc(&g, sizeof(double) * 2 * height * length));
     unsigned nb = (unsigned)((2 * height * length + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_taper_grad, dim3(nb), dim3(kBlock), 0, 0, th.hist[0], th.hist[1], height, length, duration,
                        g);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(out, g, sizeof(double) * 2 * height * length, hipMemcpyDeviceToHost);
-    (void)hipFree(g);
+    (void)dev_free(g);
     if (e != hipSuccess) return fail("fdtd_dev_taper_ep_gradient", e);
     return 0;
 }
@@ -456,9 +456,9 @@ void fdtd_dev_taper_release(const void *key) {
     auto it = g_tapers.find(key);
     if (it == g_tapers.end()) return;
     (void)hipDeviceSynchronize();
-    (void)hipFree(it->second.hist[0]);
-    (void)hipFree(it->second.hist[1]);
-    (void)hipFree(it->second.edge_i);
+    (void)dev_free(it->second.hist[0]);
+    (void)dev_free(it->second.hist[1]);
+    (void)dev_free(it->second.edge_i);
     g_tapers.erase(it);
 }
 
@@ -480,9 +480,9 @@ int fdtd_dev_block_store(fdtd_dev *d, const void *key, const int box[6], int dur
         std::memcpy(bh.box, box, sizeof bh.box);
         bh.T = duration;
         size_t n = (size_t)nx * ny * nz;
-        FDTD_TRY(hipMalloc(&bh.fwd, 3 * n * duration * sizeof(float)));
+        FDTD_TRY(dev_malloc(&bh.fwd, 3 * n * duration * sizeof(float)));
         FDTD_TRY(hipMemset(bh.fwd, 0, 3 * n * duration * sizeof(float)));
-        FDTD_TRY(hipMalloc(&bh.grad, n * sizeof(double)));
+        FDTD_TRY(dev_malloc(&bh.grad, n * sizeof(double)));
         FDTD_TRY(hipMemset(bh.grad, 0, n * sizeof(double)));
         it = g_blocks.emplace(key, bh).first;
     }
@@ -534,8 +534,8 @@ void fdtd_dev_block_release(const void *key) {
     auto it = g_blocks.find(key);
     if (it == g_blocks.end()) return;
     (void)hipDeviceSynchronize();
-    (void)hipFree(it->second.fwd);
-    (void)hipFree(it->second.grad);
+    (void)dev_free(it->second.fwd);
+    (void)dev_free(it->second.grad);
     g_blocks.erase(it);
 }
 

@@ -258,22 +258,22 @@ void run_release(fdtd_dev *d) {
     Run *r = static_cast<Run *>(d->run);
     (void)hipStreamSynchronize(d->stream);
     for (auto &s : r->src) {
-        (void)hipFree(s.PR);
-        (void)hipFree(s.PI);
-        (void)hipFree(s.win);
-        (void)hipFree(s.dcoef);
+        (void)dev_free(s.PR);
+        (void)dev_free(s.PI);
+        (void)dev_free(s.win);
+        (void)dev_free(s.dcoef);
     }
     for (auto &o : r->ovl) {
-        (void)hipFree(o.W);
-        (void)hipFree(o.acc);
-        (void)hipFree(o.inst);
+        (void)dev_free(o.W);
+        (void)dev_free(o.acc);
+        (void)dev_free(o.inst);
     }
-    for (auto &q : r->dft) (void)hipFree(q.acc);
-    (void)hipFree(r->partials);
-    (void)hipFree(r->mons);
-    (void)hipFree(r->tw);
-    (void)hipFree(r->mpart);
-    (void)hipFree(r->minst);
+    for (auto &q : r->dft) (void)dev_free(q.acc);
+    (void)dev_free(r->partials);
+    (void)dev_free(r->mons);
+    (void)dev_free(r->tw);
+    (void)dev_free(r->mpart);
+    (void)dev_free(r->minst);
     delete r;
     d->run = nullptr;
 }
@@ -353,9 +353,9 @@ int fdtd_dev_run_add_source(fdtd_dev *d, int cx, int cy, int cz, const double *m
     s.k = cz;
     s.nsteps = nsteps;
     s.coef = std::move(coef);
-    FDTD_TRY(hipMalloc(&s.PR, PR.size() * sizeof(float)));
-    FDTD_TRY(hipMalloc(&s.PI, PI.size() * sizeof(float)));
-    FDTD_TRY(hipMalloc(&s.win, win.size()));
+    FDTD_TRY(dev_malloc(&s.PR, PR.size() * sizeof(float)));
+    FDTD_TRY(dev_malloc(&s.PI, PI.size() * sizeof(float)));
+    FDTD_TRY(dev_malloc(&s.win, win.size()));
     FDTD_TRY(hipMemcpy(s.PR, PR.data(), PR.size() * sizeof(float), hipMemcpyHostToDevice));
     FDTD_TRY(hipMemcpy(s.PI, PI.data(), PI.size() * sizeof(float), hipMemcpyHostToDevice));
     FDTD_TRY(hipMemcpy(s.win, win.data(), win.size(), hipMemcpyHostToDevice));
@@ -403,11 +403,11 @@ int fdtd_dev_run_add_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *
     const size_t nf1 = (size_t)(nfreq > 0 ? nfreq : 1);
     o.W = nullptr;
     o.acc = o.inst = nullptr;
-    FDTD_TRY(hipMalloc(&o.W, W.size() * sizeof(float)));
+    FDTD_TRY(dev_malloc(&o.W, W.size() * sizeof(float)));
     FDTD_TRY(hipMemcpy(o.W, W.data(), W.size() * sizeof(float), hipMemcpyHostToDevice));
-    FDTD_TRY(hipMalloc(&o.acc, nf1 * sizeof(double2)));
+    FDTD_TRY(dev_malloc(&o.acc, nf1 * sizeof(double2)));
     FDTD_TRY(hipMemset(o.acc, 0, nf1 * sizeof(double2)));
-    FDTD_TRY(hipMalloc(&o.inst, sizeof(double2)));
+    FDTD_TRY(dev_malloc(&o.inst, sizeof(double2)));
     FDTD_TRY(hipMemset(o.inst, 0, sizeof(double2)));
 
     r->ovl.push_back(std::move(o));
@@ -436,7 +436,7 @@ int fdtd_dev_run_add_dft_box(fdtd_dev *d, const int box[6], int comp_mask, int n
     q.freq_off = 0;
     q.acc = nullptr;
     size_t n = (size_t)q.ncomp * nfreq * q.nb;
-    FDTD_TRY(hipMalloc(&q.acc, n * sizeof(double2)));
+    FDTD_TRY(dev_malloc(&q.acc, n * sizeof(double2)));
     FDTD_TRY(hipMemset(q.acc, 0, n * sizeof(double2)));
     r->dft.push_back(std::move(q));
     return (int)r->dft.size() - 1;
@@ -496,9 +496,9 @@ int fdtd_dev_run_add_adjoint_source(fdtd_dev *d, int cx, int cy, int cz, const d
     s.coef = std::move(coef);
     s.PR = s.PI = nullptr;
     s.win = nullptr;
-    FDTD_TRY(hipMalloc(&s.PR, PR.size() * sizeof(float)));
-    FDTD_TRY(hipMalloc(&s.PI, PI.size() * sizeof(float)));
-    FDTD_TRY(hipMalloc(&s.win, win.size()));
+    FDTD_TRY(dev_malloc(&s.PR, PR.size() * sizeof(float)));
+    FDTD_TRY(dev_malloc(&s.PI, PI.size() * sizeof(float)));
+    FDTD_TRY(dev_malloc(&s.win, win.size()));
     FDTD_TRY(hipMemcpy(s.PR, PR.data(), PR.size() * sizeof(float), hipMemcpyHostToDevice));
     FDTD_TRY(hipMemcpy(s.PI, PI.data(), PI.size() * sizeof(float), hipMemcpyHostToDevice));
     FDTD_TRY(hipMemcpy(s.win, win.data(), win.size(), hipMemcpyHostToDevice));
@@ -539,9 +539,9 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
         }
         if ((long long)tw.size() > r->tw_cap) {
             FDTD_TRY(hipStreamSynchronize(d->stream));
-            (void)hipFree(r->tw);
+            (void)dev_free(r->tw);
             r->tw = nullptr;
-            FDTD_TRY(hipMalloc(&r->tw, tw.size() * sizeof(double2)));
+            FDTD_TRY(dev_malloc(&r->tw, tw.size() * sizeof(double2)));
             r->tw_cap = (long long)tw.size();
         }
         FDTD_TRY(hipMemcpyAsync(r->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, d->stream));
@@ -551,9 +551,9 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     const int nmon = (int)r->ovl.size();
     if (nmon > 0 && (long long)nbp * nmon > r->partial_cap) {
         FDTD_TRY(hipStreamSynchronize(d->stream));
-        (void)hipFree(r->partials);
+        (void)dev_free(r->partials);
         r->partials = nullptr;
-        FDTD_TRY(hipMalloc(&r->partials, (size_t)nbp * nmon * sizeof(double2)));
+        FDTD_TRY(dev_malloc(&r->partials, (size_t)nbp * nmon * sizeof(double2)));
         r->partial_cap = (int)(nbp * nmon);
     }
     if (nmon > 0) {
@@ -564,9 +564,9 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
         }
         if (nmon > r->mon_cap) {
             FDTD_TRY(hipStreamSynchronize(d->stream));
-            (void)hipFree(r->mons);
+            (void)dev_free(r->mons);
             r->mons = nullptr;
-            FDTD_TRY(hipMalloc(&r->mons, (size_t)nmon * sizeof(MonDev)));
+            FDTD_TRY(dev_malloc(&r->mons, (size_t)nmon * sizeof(MonDev)));
             r->mon_cap = nmon;
         }
         FDTD_TRY(hipMemcpyAsync(r->mons, md.data(), md.size() * sizeof(MonDev), hipMemcpyHostToDevice, d->stream));
@@ -592,11 +592,11 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     for (auto &o : r->ovl) fold = fold && o.ring_free;
     if (fold && n > r->mstep_cap) {
         FDTD_TRY(hipStreamSynchronize(d->stream));
-        (void)hipFree(r->mpart);
-        (void)hipFree(r->minst);
+        (void)dev_free(r->mpart);
+        (void)dev_free(r->minst);
         r->mpart = r->minst = nullptr;
-        FDTD_TRY(hipMalloc(&r->mpart, (size_t)kMaxRunMon * n * nbp * sizeof(double2)));
-        FDTD_TRY(hipMalloc(&r->minst, (size_t)kMaxRunMon * n * sizeof(double2)));
+        FDTD_TRY(dev_malloc(&r->mpart, (size_t)kMaxRunMon * n * nbp * sizeof(double2)));
+        FDTD_TRY(dev_malloc(&r->minst, (size_t)kMaxRunMon * n * sizeof(double2)));
         r->mstep_cap = n;
     }
     for (int s = 0; s < n; s++) {
@@ -688,9 +688,9 @@ int run_twiddles(fdtd_dev *d, Run *r, double dt, int n0, int n, double t_offset,
     }
     if ((long long)tw.size() > r->tw_cap) {
         FDTD_TRY(hipStreamSynchronize(d->stream));
-        (void)hipFree(r->tw);
+        (void)dev_free(r->tw);
         r->tw = nullptr;
-        FDTD_TRY(hipMalloc(&r->tw, tw.size() * sizeof(double2)));
+        FDTD_TRY(dev_malloc(&r->tw, tw.size() * sizeof(double2)));
         r->tw_cap = (long long)tw.size();
     }
     FDTD_TRY(hipMemcpyAsync(r->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, d->stream));
@@ -726,18 +726,18 @@ int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, Ye
             md[m] = MonDev{(long long)o.k * d->P, o.W, o.inst, o.acc, o.nfreq, o.freq_off};
         }
         if (nmon > r->mon_cap) {
-            (void)hipFree(r->mons);
+            (void)dev_free(r->mons);
             r->mons = nullptr;
-            FDTD_TRY(hipMalloc(&r->mons, (size_t)nmon * sizeof(MonDev)));
+            FDTD_TRY(dev_malloc(&r->mons, (size_t)nmon * sizeof(MonDev)));
             r->mon_cap = nmon;
         }
         FDTD_TRY(hipMemcpy(r->mons, md.data(), md.size() * sizeof(MonDev), hipMemcpyHostToDevice));
         if (n > r->mstep_cap) {
-            (void)hipFree(r->mpart);
-            (void)hipFree(r->minst);
+            (void)dev_free(r->mpart);
+            (void)dev_free(r->minst);
             r->mpart = r->minst = nullptr;
-            FDTD_TRY(hipMalloc(&r->mpart, (size_t)kMaxRunMon * n * nbp * sizeof(double2)));
-            FDTD_TRY(hipMalloc(&r->minst, (size_t)kMaxRunMon * n * sizeof(double2)));
+            FDTD_TRY(dev_malloc(&r->mpart, (size_t)kMaxRunMon * n * nbp * sizeof(double2)));
+            FDTD_TRY(dev_malloc(&r->minst, (size_t)kMaxRunMon * n * sizeof(double2)));
             r->mstep_cap = n;
         }
     }
@@ -745,7 +745,7 @@ int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, Ye
         int err;
         if (!plane_for(d, s.k, &err)) return err;
         if (!s.dcoef) {
-            FDTD_TRY(hipMalloc(&s.dcoef, s.coef.size() * sizeof(float2)));
+            FDTD_TRY(dev_malloc(&s.dcoef, s.coef.size() * sizeof(float2)));
             FDTD_TRY(hipMemcpy(s.dcoef, s.coef.data(), s.coef.size() * sizeof(float2), hipMemcpyHostToDevice));
         }
     }
@@ -847,14 +847,14 @@ int fdtd_dev_dft_gradient(fdtd_dev *df, int id_f, fdtd_dev *da, int id_a, int fr
     }
     FDTD_TRY(hipStreamSynchronize(da->stream));
     double *g = nullptr;
-    FDTD_TRY(hipMalloc(&g, (size_t)F.nb * sizeof(double)));
+    FDTD_TRY(dev_malloc(&g, (size_t)F.nb * sizeof(double)));
     hipLaunchKernelGGL(k_dft_gradient, dim3((unsigned)((F.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, df->stream, g,
                        (const double2 *)F.acc, F.mask, F.nfreq, (const double2 *)A.acc, A.mask, A.nfreq, freq, F.nb,
                        w_re, w_im);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(df->stream);
     if (e == hipSuccess) e = hipMemcpy(grad, g, (size_t)F.nb * sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(g);
+    (void)dev_free(g);
     if (e != hipSuccess) return fail("DFT gradient", e);
     return 0;
 }

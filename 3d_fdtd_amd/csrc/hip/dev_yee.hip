@@ -472,10 +472,10 @@ int fdtd_dev_cpml_on(fdtd_dev *d, const int npml[6], int order, double R0, doubl
         off += 6LL * N[a];
         c.psi_n[a] = (long long)c.W[a] * N[(a + 1) % 3] * N[(a + 2) % 3];
     }
-    FDTD_TRY(hipMalloc(&c.tab, (size_t)off * sizeof(float)));
+    FDTD_TRY(dev_malloc(&c.tab, (size_t)off * sizeof(float)));
     long long total = c.psi_n[0] + c.psi_n[1] + c.psi_n[2];
     if (total > 0) {
-        FDTD_TRY(hipMalloc(&c.psi_base, (size_t)4 * total * sizeof(float)));
+        FDTD_TRY(dev_malloc(&c.psi_base, (size_t)4 * total * sizeof(float)));
         FDTD_TRY(hipMemsetAsync(c.psi_base, 0, (size_t)4 * total * sizeof(float), d->stream));
     }
     float *p = c.psi_base;
@@ -524,9 +524,9 @@ int fdtd_dev_kernel_times(fdtd_dev *d, int counts[3], double ms_out[3]) {
 int fdtd_dev_cpml_off(fdtd_dev *d) {
     Cpml &c = d->cpml;
     if (c.tab || c.psi_base) FDTD_TRY(hipStreamSynchronize(d->stream));
-    (void)hipFree(c.tab);
-    (void)hipFree(c.psi_base);
-    (void)hipFree(c.psi_alt);
+    (void)dev_free(c.tab);
+    (void)dev_free(c.psi_base);
+    (void)dev_free(c.psi_alt);
     c = Cpml();
     return 0;
 }

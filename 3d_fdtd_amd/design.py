@@ -13,6 +13,7 @@ all run on the GPU (fdtd_dev_*, fdtd_dev_batch_*, fdtd_mlp_*).
 """
 import importlib
 import math
+import time
 
 import numpy as np
 from scipy.interpolate import CubicSpline
@@ -28,6 +29,7 @@ class TaperFamily:
         self.disc, self.I, self.J, self.core, self.npml, self.port = disc, I, J, core_cells, npml, port
         self.wmin, self.wmax, self.lmin, self.lmax = wmin, wmax, lmin, lmax
         self.steps = steps
+        self.eps_core, self.eps_clad = 3.48 ** 2, 1.44 ** 2
         self.dt = 0.99 * disc / math.sqrt(3)
         self.omegas = 2 * math.pi / np.linspace(lam[0], lam[1], nfreq)
         lam0, dlam = 0.5 * (lam[0] + lam[1]), 0.2
@@ -55,20 +57,38 @@ class TaperFamily:
         L = (p[:, 8:9] - self.lmin) / (self.lmax - self.lmin) * 2 - 1
         return np.hstack([w, L]).astype(np.float32)
 
-    def permittivity(self, p):
-        """eps [k][j][i][3] (float32, flattened) and K for design p."""
-        widths, L = np.asarray(p[:8]), float(p[8])
-        K = int(round(L / self.disc)) + 2 * (self.npml + self.port + 2)
+    def widths_cells(self, p, K):
+        """Core width w(k) in cells along the grid: the cubic spline through the control points,
+        held at the end widths over the straight ports."""
         k0, k1 = self.npml + self.port + 2, K - self.npml - self.port - 2
-        ep = np.full((K, self.J, self.I, 3), 1.44 ** 2, np.float32)
         xs = np.linspace(k0, k1 - 1, 8)
-        w = CubicSpline(xs, widths)(np.clip(np.arange(K), xs[0], xs[-1])) / self.disc
-        i = np.arange(self.I) + 0.5 - self.I / 2
-        core = np.abs(i)[None, :] < (w[:, None] / 2)
+        return CubicSpline(xs, np.asarray(p[:8]))(np.clip(np.arange(K), xs[0], xs[-1])) / self.disc
+
+    def length_cells(self, p):
+        return int(round(float(p[8]) / self.disc)) + 2 * (self.npml + self.port + 2)
+
+    def permittivity(self, p):
+        """eps [k][j][i][3] (float32, flattened) and K for design p.  Each cell of the core rows
+        takes eps_clad + f (eps_core - eps_clad), f the fraction of the cell's x extent inside the
+        core |x| < w(k)/2, so eps (and the transmission) is a continuous, piecewise-linear
+        function of the widths; only the two cells that hold the core edges are fractional."""
+        K = self.length_cells(p)
+        w = self.widths_cells(p, K)
+        ep = np.full((K, self.J, self.I, 3), self.eps_clad, np.float32)
+        x = np.arange(self.I) - self.I / 2  # left edge of cell i
+        frac = np.clip(np.minimum(x[None, :] + 1, w[:, None] / 2) - np.maximum(x[None, :], -w[:, None] / 2), 0, 1)
         j0 = self.J // 2 - self.core // 2
-        sel = np.broadcast_to(core[:, None, :, None], (K, self.core, self.I, 3))
-        ep[:, j0:j0 + self.core][sel] = 3.48 ** 2
+        ep[:, j0:j0 + self.core] = (self.eps_clad + frac * (self.eps_core - self.eps_clad))[:, None, :, None]
         return ep.reshape(-1), K
+
+    def run_inputs(self, p):
+        """Everything that defines the FDTD run of design p, as plain arrays and numbers: the
+        permittivity, grid, time step, CPML depth, mode window, source schedule, wavelengths,
+        source plane and the launched / transmitted monitor planes."""
+        ep, K = self.permittivity(p)
+        return dict(ep=ep, I=self.I, J=self.J, K=K, disc=self.disc, dt=self.dt, npml=self.npml, mode=self.mode,
+                    mi=self.mi, mj=self.mj, amp=self.amp, phase=self.phase, omegas=self.omegas,
+                    src_k=self.npml + 1, mon_ks=(self.npml + 4, K - self.npml - 2))
 
     def grid(self, p):
         """A device grid for design p with its source and the launched / transmitted monitors."""
@@ -97,9 +117,9 @@ class TaperFamily:
         envelope; its E field, DFT'd with step offset 1 over the core rows of the taper section,
         gives d a_out / d eps(cell) = 2 u / (dt E0(w)) sum_m F_m A_m per wavelength, with
         E0(w) = sum_n A_n e^{i (w - w0) n dt} and u = e^{-i w dt} - 1 (a_in is held fixed).
-        Widening the core by dw moves each side by dw / 2: eps of the first cladding cell on
-        each side grows by (eps_core - eps_clad) dw / (2 dx).  The spline basis carries that to
-        the control points."""
+        Widening the core by dw moves each edge by dw / 2, so eps of the fractional cell that
+        holds each edge grows by (eps_core - eps_clad) dw / (2 dx) (the raster is the fill-fraction
+        one of permittivity()); the spline basis carries d T / d w(k) to the control points."""
         ep, K = self.permittivity(p)
         ci, cj = self.I // 2, self.J // 2
         ks, km = self.npml + 1, K - self.npml - 2
@@ -133,16 +153,13 @@ class TaperFamily:
         g.free()
         ga.free()
         grad_eps = grad_eps.reshape(km - ks - 4, self.core, self.I)
-        widths, L = np.asarray(p[:8]), float(p[8])
-        k0, k1 = self.npml + self.port + 2, K - self.npml - self.port - 2
-        xs = np.linspace(k0, k1 - 1, 8)
-        wk = CubicSpline(xs, widths)(np.clip(np.arange(K), xs[0], xs[-1])) / self.disc
+        wk = self.widths_cells(p, K)
         dT_dw = np.zeros(K)
-        deps = (3.48 ** 2 - 1.44 ** 2) / (2 * self.disc)
+        deps = (self.eps_core - self.eps_clad) / (2 * self.disc)
         for q, k in enumerate(range(ks + 4, km)):
-            ir = int(np.ceil(self.I / 2 + wk[k] / 2 - 0.5))  # first cladding cell right of the core
-            il = self.I - 1 - ir
-            if 0 <= ir < self.I and 0 <= il < self.I:
+            ir = int(np.floor(self.I / 2 + wk[k] / 2))  # the cells that hold the core edges
+            il = int(np.floor(self.I / 2 - wk[k] / 2))
+            if 0 <= il < ir < self.I:
                 dT_dw[k] = deps * (grad_eps[q, :, ir].sum() + grad_eps[q, :, il].sum())
         return float(band_transmission(t)), self.spline_basis(K) @ dT_dw
 
@@ -218,7 +235,13 @@ def fit(family, designs, t, width=256, depth=4, steps=2000, lr=1e-3, seed=0):
 def screen(family, net, candidates, top):
     """Indices of the `top` candidates with the highest predicted band transmission."""
     pred = band_transmission(from_outputs(net.forward(family.normalise(candidates))))
-    return np.argsort(-pred)[:top], pred
+    top = np.argpartition(-pred, top - 1)[:top] if top < len(pred) else np.arange(len(pred))
+    return top[np.argsort(-pred[top])], pred
+
+
+def predicted_spectrum(family, net, designs):
+    """Surrogate |t(lambda)|^2 [n][nfreq]."""
+    return np.abs(from_outputs(net.forward(family.normalise(designs)))) ** 2
 
 
 def adjoint_optimize(family, p0, iters, spec, step=0.1):
@@ -234,3 +257,87 @@ def adjoint_optimize(family, p0, iters, spec, step=0.1):
             break
         p[:8] = np.clip(p[:8] + step * g / max(np.abs(g).max(), 1e-30), family.wmin, family.wmax)
     return p, hist
+
+
+class SurrogateLoop:
+    """The design loop with the surrogate inside it, full FDTD as the judge and the fallback.
+
+    Each iteration
+      1. draws `pool` fresh candidates and ranks them with the surrogate;
+      2. verifies the `verify` best by batched FDTD: a candidate is accepted when its FDTD band
+         |t|^2 reaches `spec` (the surrogate's prediction never accepts anything);
+      3. appends the FDTD results to the training data and refines the network with `refit`
+         Adam steps on the new results plus an equal replay sample of the older ones;
+      4. falls back to the plain method when fewer than `min_hit` of the verified picks pass:
+         adjoint ascent with full single-geometry FDTD from the best verified near miss, whose
+         result counts when its FDTD transmission reaches the spec.
+    Accepted designs, their FDTD spectra and the surrogate's spectra for them are kept, so the
+    surrogate's error on every accepted design is known."""
+
+    def __init__(self, family, net, train_x, train_t, spec=0.7, pool=100000, verify=64, refit=20, lr=3e-4,
+                 min_hit=0.25, fallback_iters=12, seed=0):
+        self.fam, self.net, self.spec = family, net, spec
+        self.pool, self.verify, self.refit, self.lr = pool, verify, refit, lr
+        self.min_hit, self.fallback_iters = min_hit, fallback_iters
+        self.rng = np.random.default_rng(seed)
+        self.x = [family.normalise(train_x)]
+        self.y = [to_targets(train_t)]
+        self.accepted, self.T_fdtd, self.spec_fdtd, self.spec_sur = [], [], [], []
+        self.iterations = self.verified = self.fallbacks = self.fallback_accepted = 0
+
+    def step(self):
+        fam = self.fam
+        cand = fam.random(self.rng, self.pool)
+        top, _ = screen(fam, self.net, cand, self.verify)
+        picks = cand[top]
+        sur = predicted_spectrum(fam, self.net, picks)
+        t = fam.simulate(picks)
+        fdtd = np.abs(t) ** 2
+        T = fdtd.mean(axis=1)
+        ok = T >= self.spec
+        self._accept(picks[ok], T[ok], fdtd[ok], sur[ok])
+        self.iterations += 1
+        self.verified += len(picks)
+        # online refinement on what FDTD just measured
+        xn, yn = fam.normalise(picks), to_targets(t)
+        old_x, old_y = np.concatenate(self.x), np.concatenate(self.y)
+        self.x.append(xn)
+        self.y.append(yn)
+        if self.refit:
+            r = self.rng.choice(len(old_x), min(len(old_x), len(xn)), replace=False)
+            bx, by = np.concatenate([xn, old_x[r]]), np.concatenate([yn, old_y[r]])
+            for _ in range(self.refit):
+                self.net.adam_step(bx, by, lr=self.lr)
+        if ok.mean() < self.min_hit:
+            self.fallbacks += 1
+            p0 = picks[np.argmax(np.where(ok, -np.inf, T))]
+            p, hist = adjoint_optimize(fam, p0, self.fallback_iters, self.spec)
+            if hist[-1] >= self.spec:
+                t1 = fam.simulate(p[None])
+                f1 = np.abs(t1) ** 2
+                if f1.mean() >= self.spec:
+                    self.fallback_accepted += 1
+                    self._accept(p[None], f1.mean(axis=1), f1, predicted_spectrum(fam, self.net, p[None]))
+        return int(ok.sum())
+
+    def _accept(self, p, T, fdtd, sur):
+        self.accepted.extend(list(p))
+        self.T_fdtd.extend(list(T))
+        self.spec_fdtd.extend(list(fdtd))
+        self.spec_sur.extend(list(sur))
+
+    def run(self, seconds=None, iterations=None, clock=time.perf_counter):
+        t0 = clock()
+        while True:
+            if iterations is not None and self.iterations >= iterations:
+                break
+            if seconds is not None and clock() - t0 >= seconds:
+                break
+            self.step()
+        return self
+
+    def surrogate_error(self):
+        """max over wavelengths of |T_surrogate - T_FDTD| for every accepted design."""
+        if not self.accepted:
+            return np.zeros(0)
+        return np.abs(np.array(self.spec_sur) - np.array(self.spec_fdtd)).max(axis=1)

@@ -25,7 +25,8 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc);
 void fdtd_dev_destroy(fdtd_dev *d);                                           /* timefield3d.c:43  */
 
 /* Materials from host AoS float arrays: unique (ep[3], mu[3], sig[6]) tuples become a table and
- * each cell stores a 1-byte id (2-byte when more than 256 tuples). */
+ * each cell stores a 1-byte id (2-byte when more than 256 tuples).  mu or sig may be NULL:
+ * mu = 1 and sig = 0 everywhere. */
 int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const float *sig);
 
 /* Fields / sources / shells / DFT accumulators between host AoS arrays and the device. */
@@ -155,6 +156,12 @@ int fdtd_dev_kernel_times(fdtd_dev *d, int counts[3], double ms[3]);
 int fdtd_dev_timer_start(fdtd_dev *d);
 int fdtd_dev_timer_stop(fdtd_dev *d, double *elapsed_ms);
 int fdtd_dev_synchronize(fdtd_dev *d);
+
+/* Device memory and streams released by fdtd_dev_destroy / fdtd_dev_batch_free / fdtd_mlp_free
+ * are cached for the next grid (a design loop creates and destroys grids by the hundred); this
+ * returns every cached block to the HIP runtime.  No counterpart in the reference (its grids are
+ * host memory). */
+int fdtd_dev_release_cached_memory(void);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,35 @@
+"""TEST INFRASTRUCTURE ONLY -- FP64 CPU run of one taper design through the oracle (fdtd_oracle.c):
+the checker of the device's batched taper transmissions, and the CPU baseline of the design
+workload.  Nothing in the product imports it.
+
+One run: for each step n, the pulsed mode source on plane src_k (orc_mode_source, phase[n],
+amp[n]; timefield3d.c:2977-3026 semantics), one CPML step (orc_step_cpml), then the overlap of
+every monitor plane (orc_mode_overlap, timefield3d.c:3176-3239), DFT'd in FP64 with e^{i w n dt}.
+The transmission is the ratio of the last monitor's running DFT to the first's.
+"""
+import numpy as np
+
+from oracle import oracle_py as O
+
+
+def transmission(ep, I, J, K, disc, dt, npml, mode, mi, mj, amp, phase, omegas, src_k, mon_ks,
+                 cpml=dict(order=3, R0=1e-8, kappa_max=1.0, alpha_max=0.05)):
+    """Complex t(omega) = a(mon_ks[-1]) / a(mon_ks[0]) of the design with permittivity ep
+    ([K][J][I][3] float32, flattened); mode is the power-normalised interleaved complex
+    [mj][mi][6] window centred on (I/2, J/2)."""
+    g = O.Grid(I, J, K, disc)
+    g.ep[:] = ep
+    cp = O.Cpml(g, (npml,) * 6, dt, **cpml)
+    mode = np.array(mode, np.float64)
+    omegas = np.asarray(omegas, np.float64)
+    ci, cj = I // 2, J // 2
+    acc = [np.zeros(len(omegas), complex) for _ in mon_ks]
+    cum, inst = np.zeros(2), np.zeros(2)
+    L = O.lib()
+    for n in range(len(amp)):
+        L.orc_mode_source(g.ptr, ci, cj, src_k, mi, mj, disc, O.dptr(mode), float(phase[n]), float(amp[n]), 0)
+        O.step_cpml(g, cp, dt)
+        for q, kz in enumerate(mon_ks):
+            L.orc_mode_overlap(g.ptr, ci, cj, kz, mi, mj, disc, O.dptr(mode), 0.0, 0, O.dptr(cum), O.dptr(inst))
+            acc[q] += np.exp(1j * omegas * n * dt) * complex(inst[0], inst[1])
+    return acc[-1] / acc[0]

@@ -52,3 +52,29 @@ def test_surrogate_screening_beats_unscreened_picks():
     net.free()
     assert T_sur.mean() >= T_plain.mean() + 0.1
     assert n_sur >= max(2 * n_plain, 4)
+
+
+def test_surrogate_loop_accepts_only_fdtd_verified_designs():
+    """Two iterations of the design loop (3d_fdtd_amd/design.py SurrogateLoop) with the fallback
+    forced on (min_hit above 1): every accepted design's FDTD band |t|^2 meets the spec and
+    re-simulating it gives the same transmission; the fallback ran once per iteration; the
+    verified results joined the training data."""
+    fam = design.TaperFamily()
+    rng = np.random.default_rng(77)
+    train = fam.random(rng, 512)
+    t_train = fam.simulate(train)
+    net = design.fit(fam, train, t_train, steps=600)
+    loop = design.SurrogateLoop(fam, net, train, t_train, spec=0.7, pool=20000, verify=32, refit=10,
+                                min_hit=1.01, fallback_iters=8, seed=5)
+    loop.run(iterations=2)
+    say("accepted %d of %d verified, fallbacks %d (accepted %d)"
+        % (len(loop.accepted), loop.verified, loop.fallbacks, loop.fallback_accepted))
+    assert loop.iterations == 2 and loop.verified == 64 and loop.fallbacks == 2
+    assert sum(len(x) for x in loop.x) == 512 + 64
+    assert len(loop.accepted) > 0
+    T = design.band_transmission(fam.simulate(np.array(loop.accepted)))
+    assert (T >= 0.7).all()
+    assert np.allclose(T, loop.T_fdtd, rtol=0, atol=1e-6)
+    err = loop.surrogate_error()
+    say("surrogate max_lambda |dT| on accepted designs: median %.3f" % np.median(err))
+    net.free()
