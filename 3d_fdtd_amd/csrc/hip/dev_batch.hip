@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_E(const BatchGeom *__restrict_
     for (long long vb = blockIdx.x; vb < total; vb += gridDim.x) {
         const long long lb = xcd_logical_block_ll(vb, total);
         const BatchGeom &B = geo[batch_grid(geo, G, lb)];
-        update_E_block<MatT, true, KC>(B.a, B.pm, (int)(lb - B.vb0));
+        update_E_block<MatT, true, KC, uint32_t>(B.a, B.pm, (int)(lb - B.vb0));
     }
 }
 
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_H(const BatchGeom *__restrict_
     for (long long vb = blockIdx.x; vb < total; vb += gridDim.x) {
         const long long lb = xcd_logical_block_ll(vb, total);
         const BatchGeom &B = geo[batch_grid(geo, G, lb)];
-        update_H_block<MatT, true, MON, KC>(B.a, B.pm, (int)(lb - B.vb0));
+        update_H_block<MatT, true, MON, KC, uint32_t>(B.a, B.pm, (int)(lb - B.vb0));
         if (MON) __syncthreads();  // the next virtual block reuses the reduction slots
     }
 }
@@ -260,11 +260,16 @@ fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n) {
         set_error("batch: no grids");
         return nullptr;
     }
-    for (int g = 0; g < n; g++)
+    for (int g = 0; g < n; g++) {
         if (!grids[g] || grids[g]->mat_bytes != grids[0]->mat_bytes) {
             set_error("batch: null grid or grids with different material-id widths");
             return nullptr;
         }
+        if (grids[g]->N >= kSmallGridCells) {
+            set_error("batch: every grid must have fewer than 2^30 cells");
+            return nullptr;
+        }
+    }
     fdtd_dev_batch *b = new fdtd_dev_batch();
     b->grids.assign(grids, grids + n);
     b->host.resize((size_t)n);

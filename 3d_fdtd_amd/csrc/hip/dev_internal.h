@@ -32,6 +32,9 @@ enum { EX = 0, EY, EZ, HX, HY, HZ };
 
 constexpr int kMaxSrcPlanes = 8;
 constexpr int kBlock = 256;
+// Grids with fewer cells than this run the split and batched kernels with 32-bit cell offsets
+// (every per-grid array is then below 4 GiB).
+constexpr long long kSmallGridCells = 1LL << 30;
 
 // Per-material coefficient record (floats), one for each conductivity scheme:
 //   [0..2] caE, [3..5] cbE, [6..8] cjE, [9..11] caH, [12..14] cbH, [15..17] cjH,

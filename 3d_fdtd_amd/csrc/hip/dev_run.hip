@@ -755,6 +755,7 @@ int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, Ye
         const RunSource &s = r->src[q];
         a->rs[q] = RunSrcArg{s.k, s.PR, s.PI, s.win, 0.0f, 0.0f, s.dcoef, s.nsteps};
     }
+    set_source_range(*a);
     a->nmon = nmon;
     for (int m = 0; m < nmon; m++)
         a->mon[m] = RunMonArg{r->ovl[m].k, r->ovl[m].W, r->mpart + (size_t)m * r->mstep_cap * nbp, (long long)nbp};

@@ -9,6 +9,8 @@
 // come through L1/L2.
 #pragma once
 
+#include <algorithm>
+
 #include "dev_internal.h"
 
 namespace fdtd {
@@ -34,6 +36,9 @@ struct YeeArgs {
     // device step counter of graph-captured batch steps (null: per-step values are in the args)
     const int *step;
     int step0;
+    // z planes [src_k0, src_k1) may carry a source (dense volume, stored planes or run sources);
+    // set by set_source_range once the source fields are final
+    int src_k0, src_k1;
 };
 
 struct PmlArgs {
@@ -42,6 +47,16 @@ struct PmlArgs {
     float *psiE[3][2];
     float *psiH[3][2];
 };
+
+// Host side: the z range of the planes that may carry a source, from a's source fields.
+inline void set_source_range(YeeArgs &a) {
+    int k0 = a.K, k1 = 0;
+    if (a.dense) k0 = 0, k1 = a.K;
+    for (int s = 0; s < a.nsrc; s++) k0 = std::min(k0, a.src[s].k), k1 = std::max(k1, a.src[s].k + 1);
+    for (int q = 0; q < a.nrs; q++) k0 = std::min(k0, a.rs[q].k), k1 = std::max(k1, a.rs[q].k + 1);
+    a.src_k0 = k0 < k1 ? k0 : 0;
+    a.src_k1 = k0 < k1 ? k1 : 0;
+}
 
 // Fill the launch arguments of the full-range split step of grid d (PML per d's boundary,
 // semi-implicit scheme, stored source planes, no run sources or monitors); builds the
@@ -58,47 +73,65 @@ int run_batch_finish(fdtd_dev *d, int n0, int n, hipStream_t st);
 
 namespace {
 
+// Cell offsets inside one grid's arrays.  Off = uint32_t when every array of the grid is below
+// 4 GiB (fewer than 2^30 cells): a load is then a uniform 64-bit base in scalar registers plus a
+// 32-bit byte offset in one vector register, and fixed neighbour deltas (c - 1, c - I) become
+// the instruction's immediate offset, which removes the per-access 64-bit address arithmetic.
+// Off = long long covers larger grids.
+template <typename Off, typename T>
+__device__ __forceinline__ T ld(const T *base, Off i) {
+    if constexpr (sizeof(Off) == 4)
+        return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)i * (uint32_t)sizeof(T));
+    else
+        return base[i];
+}
+
+template <typename Off, typename T>
+__device__ __forceinline__ void st(T *base, Off i, T v) {
+    if constexpr (sizeof(Off) == 4)
+        *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)i * (uint32_t)sizeof(T)) = v;
+    else
+        base[i] = v;
+}
+
+template <typename MatT, typename Off>
+__device__ __forceinline__ int mat_at(const void *mat, Off c) {
+    return (int)ld(static_cast<const MatT *>(mat), c);
+}
+
 // Source components m0..m0+2 of plane cell p at plane k: tabulated run source inside its mode
-// window, else the stored source plane.  Returns false when plane k carries no source.
-__device__ __forceinline__ bool source3(const YeeArgs &a, int k, long long p, int m0, float v[3]);
-
-template <typename MatT>
-__device__ __forceinline__ int mat_id(const void *mat, long long c) {
-    return (int)static_cast<const MatT *>(mat)[c];
-}
-
-// Source values of plane k (component m of plane cell p at J[m*stride + p]), or null.
-__device__ __forceinline__ const float *src_plane(const YeeArgs &a, int k, long long &stride) {
-    if (a.dense) {
-        stride = a.N;
-        return a.dense + (long long)k * a.P;
-    }
-    const float *J = nullptr;
-    stride = a.P;
-    for (int s = 0; s < a.nsrc; s++)
-        if (a.src[s].k == k) J = a.src[s].J;
-    return J;
-}
-
-__device__ __forceinline__ bool source3(const YeeArgs &a, int k, long long p, int m0, float v[3]) {
+// window, else the stored source plane.  Returns false when plane k carries no source.  The
+// per-component planes are addressed from uniform bases (the dense 6 x N source volume in 64-bit
+// arithmetic), so only the in-plane offset p is per lane.
+template <typename Off>
+__device__ __forceinline__ bool source3(const YeeArgs &a, int k, Off p, int m0, float v[3]) {
+    if (k < a.src_k0 || k >= a.src_k1) return false;  // most planes: two scalar compares
     for (int q = 0; q < a.nrs; q++) {
         const RunSrcArg &r = a.rs[q];
-        if (r.k == k && r.win[p]) {
+        if (r.k == k && ld(r.win, p)) {
             float c1 = r.c1, c2 = r.c2;
             if (r.tab) {
-                const int st = *a.step;
-                const float2 c = st >= 0 && st < r.ntab ? r.tab[st] : make_float2(0.0f, 0.0f);
+                const int s = *a.step;
+                const float2 c = s >= 0 && s < r.ntab ? r.tab[s] : make_float2(0.0f, 0.0f);
                 c1 = c.x;
                 c2 = c.y;
             }
-            for (int m = 0; m < 3; m++) v[m] = c1 * r.PR[(m0 + m) * a.P + p] + c2 * r.PI[(m0 + m) * a.P + p];
+            for (int m = 0; m < 3; m++)
+                v[m] = c1 * ld(r.PR + (long long)(m0 + m) * a.P, p) + c2 * ld(r.PI + (long long)(m0 + m) * a.P, p);
             return true;
         }
     }
-    long long cs;
-    const float *J = src_plane(a, k, cs);
+    const float *J = nullptr;
+    long long stride = a.P;
+    if (a.dense) {
+        J = a.dense + (long long)k * a.P;
+        stride = a.N;
+    } else {
+        for (int s = 0; s < a.nsrc; s++)
+            if (a.src[s].k == k) J = a.src[s].J;
+    }
     if (!J) return false;
-    for (int m = 0; m < 3; m++) v[m] = J[(m0 + m) * cs + p];
+    for (int m = 0; m < 3; m++) v[m] = ld(J + (long long)(m0 + m) * stride, p);
     return true;
 }
 
@@ -108,24 +141,67 @@ __device__ __forceinline__ int slab_index(int n, int N, int nlo, int nhi) {
     return -1;
 }
 
-
-// psi <- b psi + a d (stored only when `live`); returns the PML-corrected difference d/kappa + psi
-__device__ __forceinline__ float pml_term_l(float *psi, float b, float a, float kinv, float d, bool live) {
-    float v = b * *psi + a * d;
-    if (live) *psi = v;
+// psi <- b psi + a d from the psi value `old` loaded earlier (stored only when `live`); returns the
+// PML-corrected difference d/kappa + psi
+template <typename Off>
+__device__ __forceinline__ float pml_term(float *psi, Off q, float old, float b, float a, float kinv, float d,
+                                          bool live) {
+    float v = b * old + a * d;
+    if (live) st(psi, q, v);
     return d * kinv + v;
+}
+
+// psi offsets of cell (i, j, k) in the x-, y- and z-slab arrays (meaningful only inside that slab)
+template <typename Off>
+__device__ __forceinline__ Off psi_x(const PmlArgs &pm, const YeeArgs &a, int j, int k, int sx) {
+    return (Off)sx + (Off)pm.W[0] * ((Off)j + (Off)a.J * (Off)k);
+}
+template <typename Off>
+__device__ __forceinline__ Off psi_y(const PmlArgs &pm, const YeeArgs &a, int i, int k, int sy) {
+    return (Off)i + (Off)a.I * ((Off)sy + (Off)pm.W[1] * (Off)k);
+}
+template <typename Off>
+__device__ __forceinline__ Off psi_z(const YeeArgs &a, Off p, int sz) {
+    return p + (Off)a.P * (Off)sz;
+}
+
+// The six psi values of cell (i, j, k) (two per PML axis the cell lies in; 0 for the others),
+// loaded together with the cell's field inputs so that their latency overlaps the fields' instead
+// of following it.  psi = psiE or psiH of the PML arguments.
+template <typename Off>
+__device__ __forceinline__ void psi_load(float *const psi[3][2], const PmlArgs &pm, const YeeArgs &a, int i, int j,
+                                         Off p, int k, int sx, int sy, float v[6]) {
+    v[0] = v[1] = v[2] = v[3] = v[4] = v[5] = 0.0f;
+    const int sz = slab_index(k, a.K, pm.nlo[2], pm.nhi[2]);
+    if (sx >= 0) {
+        const Off q = psi_x<Off>(pm, a, j, k, sx);
+        v[0] = ld(psi[0][0], q);
+        v[1] = ld(psi[0][1], q);
+    }
+    if (sy >= 0) {
+        const Off q = psi_y<Off>(pm, a, i, k, sy);
+        v[2] = ld(psi[1][0], q);
+        v[3] = ld(psi[1][1], q);
+    }
+    if (sz >= 0) {
+        const Off q = psi_z<Off>(a, p, sz);
+        v[4] = ld(psi[2][0], q);
+        v[5] = ld(psi[2][1], q);
+    }
 }
 
 // E <- caE*E + cbE*(backward-difference curl of H) - cjE*J   (timefield3d.c:1759-1837)
 // KC > 0: the z chunk is KC planes, all loaded before any is computed (more bytes in flight per
 // wavefront); iterations past the end of a short last chunk reload its last plane and store
-// nothing.  KC = 0: a plain loop over a runtime chunk.
-template <typename MatT, bool PML, int KC>
+// nothing.  KC = 0: a plain loop over a runtime chunk.  Inside the PML the differences take the
+// CPML correction d/kappa + psi; a cell outside every slab has kappa = 1 and no psi on all axes
+// (the tables hold 1, 0, 0 there), so it skips the correction altogether.
+template <typename MatT, bool PML, int KC, typename Off>
 __device__ __forceinline__ void update_E_block(const YeeArgs &a, const PmlArgs &pm, int lb) {
     const int pb = lb % a.plane_blocks, kc = lb / a.plane_blocks;
-    const long long p = (long long)pb * kBlock + threadIdx.x;
-    if (p >= a.P) return;
-    const int j = (int)(p / a.I), i = (int)(p - (long long)j * a.I);
+    const Off p = (Off)pb * (Off)kBlock + (Off)threadIdx.x;
+    if (p >= (Off)a.P) return;
+    const int j = (int)(p / (Off)a.I), i = (int)(p - (Off)j * (Off)a.I);
     if (i < 1 || i > a.I - 2 || j < 1 || j > a.J - 2) return;
     const int kb = a.k0 + kc * a.kchunk;
     const int ke = min(a.k1, kb + a.kchunk);
@@ -136,41 +212,42 @@ __device__ __forceinline__ void update_E_block(const YeeArgs &a, const PmlArgs &
     const float *__restrict__ Hx = a.Hx;
     const float *__restrict__ Hy = a.Hy;
     const float *__restrict__ Hz = a.Hz;
+    const Off P = (Off)a.P, I = (Off)a.I;
     int sx = -1, sy = -1;
     float bx = 0, ax = 0, kx = 1, by = 0, ay = 0, ky = 1;
     if (PML) {
         sx = slab_index(i, a.I, pm.nlo[0], pm.nhi[0]);
         sy = slab_index(j, a.J, pm.nlo[1], pm.nhi[1]);
-        bx = pm.tab[0][i], ax = pm.tab[0][a.I + i], kx = pm.tab[0][2 * a.I + i];
-        by = pm.tab[1][j], ay = pm.tab[1][a.J + j], ky = pm.tab[1][2 * a.J + j];
+        if (sx >= 0) bx = pm.tab[0][i], ax = pm.tab[0][a.I + i], kx = pm.tab[0][2 * a.I + i];
+        if (sy >= 0) by = pm.tab[1][j], ay = pm.tab[1][a.J + j], ky = pm.tab[1][2 * a.J + j];
     }
     // one cell-update of plane k from its loaded inputs; stores only when `live`
-    auto cell = [&](int k, bool live, long long c, float hx, float hy, float hz, float hy_im, float hz_im,
-                    float hx_jm, float hz_jm, float hx_km, float hy_km, float eox, float eoy, float eoz, int m) {
-        const float *cf = a.coef + (long long)m * kCoefStride;
+    auto cell = [&](int k, bool live, Off c, float hx, float hy, float hz, float hy_im, float hz_im, float hx_jm,
+                    float hz_jm, float hx_km, float hy_km, float eox, float eoy, float eoz, int m, const float *pv) {
+        const float *cf = a.coef + m * kCoefStride;
         // curl_x = dHz/dy - dHy/dz, curl_y = dHx/dz - dHz/dx, curl_z = dHy/dx - dHx/dy
         float x1 = hz - hz_jm, x2 = hy - hy_km;
         float y1 = hx - hx_km, y2 = hz - hz_im;
         float z1 = hy - hy_im, z2 = hx - hx_jm;
-        if (PML) {
-            const int sz = slab_index(k, a.K, pm.nlo[2], pm.nhi[2]);
-            const float kz = pm.tab[2][2 * a.K + k];
+        const int sz = PML ? slab_index(k, a.K, pm.nlo[2], pm.nhi[2]) : -1;
+        if (PML && (sx >= 0 || sy >= 0 || sz >= 0)) {
+            const float kz = sz >= 0 ? pm.tab[2][2 * a.K + k] : 1.0f;
             float tx1 = x1 * ky, tx2 = x2 * kz, ty1 = y1 * kz, ty2 = y2 * kx, tz1 = z1 * kx, tz2 = z2 * ky;
             if (sx >= 0) {
-                const long long q = sx + (long long)pm.W[0] * (j + (long long)a.J * k);
-                ty2 = pml_term_l(pm.psiE[0][0] + q, bx, ax, kx, y2, live);
-                tz1 = pml_term_l(pm.psiE[0][1] + q, bx, ax, kx, z1, live);
+                const Off q = psi_x<Off>(pm, a, j, k, sx);
+                ty2 = pml_term(pm.psiE[0][0], q, pv[0], bx, ax, kx, y2, live);
+                tz1 = pml_term(pm.psiE[0][1], q, pv[1], bx, ax, kx, z1, live);
             }
             if (sy >= 0) {
-                const long long q = i + (long long)a.I * (sy + (long long)pm.W[1] * k);
-                tz2 = pml_term_l(pm.psiE[1][0] + q, by, ay, ky, z2, live);
-                tx1 = pml_term_l(pm.psiE[1][1] + q, by, ay, ky, x1, live);
+                const Off q = psi_y<Off>(pm, a, i, k, sy);
+                tz2 = pml_term(pm.psiE[1][0], q, pv[2], by, ay, ky, z2, live);
+                tx1 = pml_term(pm.psiE[1][1], q, pv[3], by, ay, ky, x1, live);
             }
             if (sz >= 0) {
                 const float bz = pm.tab[2][k], az = pm.tab[2][a.K + k];
-                const long long q = p + a.P * sz;
-                tx2 = pml_term_l(pm.psiE[2][0] + q, bz, az, kz, x2, live);
-                ty1 = pml_term_l(pm.psiE[2][1] + q, bz, az, kz, y1, live);
+                const Off q = psi_z<Off>(a, p, sz);
+                tx2 = pml_term(pm.psiE[2][0], q, pv[4], bz, az, kz, x2, live);
+                ty1 = pml_term(pm.psiE[2][1], q, pv[5], bz, az, kz, y1, live);
             }
             x1 = tx1, x2 = tx2, y1 = ty1, y2 = ty2, z1 = tz1, z2 = tz2;
         }
@@ -184,40 +261,44 @@ __device__ __forceinline__ void update_E_block(const YeeArgs &a, const PmlArgs &
             ez -= cf[8] * sv[2];
         }
         if (live) {
-            Ex[c] = ex;
-            Ey[c] = ey;
-            Ez[c] = ez;
+            st(Ex, c, ex);
+            st(Ey, c, ey);
+            st(Ez, c, ez);
         }
     };
-    const long long c0 = p + (long long)kb * a.P;
+    const Off c0 = p + (Off)kb * P;
     if constexpr (KC > 0) {
         float hx[KC], hy[KC], hz[KC], hy_im[KC], hz_im[KC], hx_jm[KC], hz_jm[KC], eox[KC], eoy[KC], eoz[KC];
+        float pv[KC][6];
         int m[KC];
-        const float hx_k0 = Hx[c0 - a.P], hy_k0 = Hy[c0 - a.P];
+        const float hx_k0 = ld(Hx, c0 - P), hy_k0 = ld(Hy, c0 - P);
 #pragma unroll
         for (int t = 0; t < KC; t++) {
             const int kk = kb + t < ke ? kb + t : ke - 1;
-            const long long c = p + (long long)kk * a.P;
-            hx[t] = Hx[c], hy[t] = Hy[c], hz[t] = Hz[c];
-            hy_im[t] = Hy[c - 1], hz_im[t] = Hz[c - 1];
-            hx_jm[t] = Hx[c - a.I], hz_jm[t] = Hz[c - a.I];
-            eox[t] = Ex[c], eoy[t] = Ey[c], eoz[t] = Ez[c];
-            m[t] = mat_id<MatT>(a.mat, c);
+            const Off c = p + (Off)kk * P;
+            hx[t] = ld(Hx, c), hy[t] = ld(Hy, c), hz[t] = ld(Hz, c);
+            hy_im[t] = ld(Hy, c - 1), hz_im[t] = ld(Hz, c - 1);
+            hx_jm[t] = ld(Hx, c - I), hz_jm[t] = ld(Hz, c - I);
+            eox[t] = ld(Ex, c), eoy[t] = ld(Ey, c), eoz[t] = ld(Ez, c);
+            m[t] = mat_at<MatT>(a.mat, c);
+            if (PML) psi_load(pm.psiE, pm, a, i, j, p, kk, sx, sy, pv[t]);
         }
 #pragma unroll
         for (int t = 0; t < KC; t++) {
             const bool live = kb + t < ke;
             const int kk = live ? kb + t : ke - 1;
-            cell(kk, live, p + (long long)kk * a.P, hx[t], hy[t], hz[t], hy_im[t], hz_im[t], hx_jm[t], hz_jm[t],
-                 t == 0 ? hx_k0 : hx[t - 1], t == 0 ? hy_k0 : hy[t - 1], eox[t], eoy[t], eoz[t], m[t]);
+            cell(kk, live, p + (Off)kk * P, hx[t], hy[t], hz[t], hy_im[t], hz_im[t], hx_jm[t], hz_jm[t],
+                 t == 0 ? hx_k0 : hx[t - 1], t == 0 ? hy_k0 : hy[t - 1], eox[t], eoy[t], eoz[t], m[t], pv[t]);
         }
     } else {
-        float hx_km = Hx[c0 - a.P], hy_km = Hy[c0 - a.P];
-        long long c = c0;
-        for (int k = kb; k < ke; k++, c += a.P) {
-            const float hx = Hx[c], hy = Hy[c], hz = Hz[c];
-            cell(k, true, c, hx, hy, hz, Hy[c - 1], Hz[c - 1], Hx[c - a.I], Hz[c - a.I], hx_km, hy_km, Ex[c], Ey[c],
-                 Ez[c], mat_id<MatT>(a.mat, c));
+        float hx_km = ld(Hx, c0 - P), hy_km = ld(Hy, c0 - P);
+        Off c = c0;
+        for (int k = kb; k < ke; k++, c += P) {
+            const float hx = ld(Hx, c), hy = ld(Hy, c), hz = ld(Hz, c);
+            float pv[6];
+            if (PML) psi_load(pm.psiE, pm, a, i, j, p, k, sx, sy, pv);
+            cell(k, true, c, hx, hy, hz, ld(Hy, c - 1), ld(Hz, c - 1), ld(Hx, c - I), ld(Hz, c - I), hx_km, hy_km,
+                 ld(Ex, c), ld(Ey, c), ld(Ez, c), mat_at<MatT>(a.mat, c), pv);
             hx_km = hx;
             hy_km = hy;
         }
@@ -250,18 +331,19 @@ __device__ __forceinline__ double2 block_sum2(double re, double im) {
 // Overlap monitors of a MON launch, after the block's z march: for each monitor plane inside the
 // chunk, the block's FP64 partial of Hx mEy - Hy mEx + Ey mHx - Ex mHy (timefield3d.c:3195-3224)
 // over its cells, from E (final since the E update) and H (just stored by these same threads).
-__device__ __forceinline__ void monitor_epilogue(const YeeArgs &a, int kb, int ke, int pb, long long p, bool active) {
+template <typename Off>
+__device__ __forceinline__ void monitor_epilogue(const YeeArgs &a, int kb, int ke, int pb, Off p, bool active) {
     for (int q = 0; q < a.nmon; q++) {
         const RunMonArg &M = a.mon[q];
         if (M.k < kb || M.k >= ke) continue;  // uniform over the block
         double re = 0.0, im = 0.0;
         if (active) {
-            const long long c = p + (long long)M.k * a.P;
-            const double ex = a.Ex[c], ey = a.Ey[c], hx = a.Hx[c], hy = a.Hy[c];
-            const float *W = M.W + p;
-            re = (hx * (double)W[2 * a.P] - hy * (double)W[0]) + (ey * (double)W[4 * a.P] - ex * (double)W[6 * a.P]);
-            im = (hx * (double)W[3 * a.P] - hy * (double)W[1 * a.P]) +
-                 (ey * (double)W[5 * a.P] - ex * (double)W[7 * a.P]);
+            const Off c = p + (Off)M.k * (Off)a.P;
+            const double ex = ld(a.Ex, c), ey = ld(a.Ey, c), hx = ld(a.Hx, c), hy = ld(a.Hy, c);
+            double w[8];
+            for (int r = 0; r < 8; r++) w[r] = ld(M.W + (long long)r * a.P, p);
+            re = (hx * w[2] - hy * w[0]) + (ey * w[4] - ex * w[6]);
+            im = (hx * w[3] - hy * w[1]) + (ey * w[5] - ex * w[7]);
         }
         const double2 v = block_sum2(re, im);
         double2 *part = a.step ? M.part + (long long)(*a.step - a.step0) * M.stride : M.part;
@@ -270,18 +352,18 @@ __device__ __forceinline__ void monitor_epilogue(const YeeArgs &a, int kb, int k
 }
 
 // H <- caH*H - cbH*(forward-difference curl of E) - cjH*M   (timefield3d.c:1679-1757)
-// KC as for the E update.  MON: the launch also evaluates the run's overlap monitors (every
-// thread of a block reaches the block reductions, so out-of-range threads skip the march
-// instead of returning).
-template <typename MatT, bool PML, bool MON, int KC>
+// KC and the PML treatment as for the E update.  MON: the launch also evaluates the run's overlap
+// monitors (every thread of a block reaches the block reductions, so out-of-range threads skip
+// the march instead of returning).
+template <typename MatT, bool PML, bool MON, int KC, typename Off>
 __device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &pm, int lb) {
     const int pb = lb % a.plane_blocks, kc = lb / a.plane_blocks;
-    const long long p = (long long)pb * kBlock + threadIdx.x;
+    const Off p = (Off)pb * (Off)kBlock + (Off)threadIdx.x;
     const int kb = a.k0 + kc * a.kchunk;
     const int ke = min(a.k1, kb + a.kchunk);
     if (kb >= ke) return;  // uniform over the block
-    const int j = (int)(p / a.I), i = (int)(p - (long long)j * a.I);
-    const bool active = p < a.P && i >= 1 && i <= a.I - 2 && j >= 1 && j <= a.J - 2;
+    const int j = (int)(p / (Off)a.I), i = (int)(p - (Off)j * (Off)a.I);
+    const bool active = p < (Off)a.P && i >= 1 && i <= a.I - 2 && j >= 1 && j <= a.J - 2;
     if (!MON && !active) return;
     if (active) {
         const float *__restrict__ Ex = a.Ex;
@@ -290,40 +372,41 @@ __device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &
         float *__restrict__ Hx = a.Hx;
         float *__restrict__ Hy = a.Hy;
         float *__restrict__ Hz = a.Hz;
+        const Off P = (Off)a.P, I = (Off)a.I;
         int sx = -1, sy = -1;
         float bx = 0, ax = 0, kx = 1, by = 0, ay = 0, ky = 1;
         if (PML) {
             sx = slab_index(i, a.I, pm.nlo[0], pm.nhi[0]);
             sy = slab_index(j, a.J, pm.nlo[1], pm.nhi[1]);
-            bx = pm.tab[0][3 * a.I + i], ax = pm.tab[0][4 * a.I + i], kx = pm.tab[0][5 * a.I + i];
-            by = pm.tab[1][3 * a.J + j], ay = pm.tab[1][4 * a.J + j], ky = pm.tab[1][5 * a.J + j];
+            if (sx >= 0) bx = pm.tab[0][3 * a.I + i], ax = pm.tab[0][4 * a.I + i], kx = pm.tab[0][5 * a.I + i];
+            if (sy >= 0) by = pm.tab[1][3 * a.J + j], ay = pm.tab[1][4 * a.J + j], ky = pm.tab[1][5 * a.J + j];
         }
-        auto cell = [&](int k, bool live, long long c, float ex, float ey, float ez, float ex_kp, float ey_kp,
-                        float ey_ip, float ez_ip, float ex_jp, float ez_jp, float hox, float hoy, float hoz, int m) {
-            const float *cf = a.coef + (long long)m * kCoefStride;
+        auto cell = [&](int k, bool live, Off c, float ex, float ey, float ez, float ex_kp, float ey_kp, float ey_ip,
+                        float ez_ip, float ex_jp, float ez_jp, float hox, float hoy, float hoz, int m, const float *pv) {
+            const float *cf = a.coef + m * kCoefStride;
             // curl_x = dEz/dy - dEy/dz, curl_y = dEx/dz - dEz/dx, curl_z = dEy/dx - dEx/dy
             float x1 = ez_jp - ez, x2 = ey_kp - ey;
             float y1 = ex_kp - ex, y2 = ez_ip - ez;
             float z1 = ey_ip - ey, z2 = ex_jp - ex;
-            if (PML) {
-                const int sz = slab_index(k, a.K, pm.nlo[2], pm.nhi[2]);
-                const float kz = pm.tab[2][5 * a.K + k];
+            const int sz = PML ? slab_index(k, a.K, pm.nlo[2], pm.nhi[2]) : -1;
+            if (PML && (sx >= 0 || sy >= 0 || sz >= 0)) {
+                const float kz = sz >= 0 ? pm.tab[2][5 * a.K + k] : 1.0f;
                 float tx1 = x1 * ky, tx2 = x2 * kz, ty1 = y1 * kz, ty2 = y2 * kx, tz1 = z1 * kx, tz2 = z2 * ky;
                 if (sx >= 0) {
-                    const long long q = sx + (long long)pm.W[0] * (j + (long long)a.J * k);
-                    ty2 = pml_term_l(pm.psiH[0][0] + q, bx, ax, kx, y2, live);
-                    tz1 = pml_term_l(pm.psiH[0][1] + q, bx, ax, kx, z1, live);
+                    const Off q = psi_x<Off>(pm, a, j, k, sx);
+                    ty2 = pml_term(pm.psiH[0][0], q, pv[0], bx, ax, kx, y2, live);
+                    tz1 = pml_term(pm.psiH[0][1], q, pv[1], bx, ax, kx, z1, live);
                 }
                 if (sy >= 0) {
-                    const long long q = i + (long long)a.I * (sy + (long long)pm.W[1] * k);
-                    tz2 = pml_term_l(pm.psiH[1][0] + q, by, ay, ky, z2, live);
-                    tx1 = pml_term_l(pm.psiH[1][1] + q, by, ay, ky, x1, live);
+                    const Off q = psi_y<Off>(pm, a, i, k, sy);
+                    tz2 = pml_term(pm.psiH[1][0], q, pv[2], by, ay, ky, z2, live);
+                    tx1 = pml_term(pm.psiH[1][1], q, pv[3], by, ay, ky, x1, live);
                 }
                 if (sz >= 0) {
                     const float bz = pm.tab[2][3 * a.K + k], az = pm.tab[2][4 * a.K + k];
-                    const long long q = p + a.P * sz;
-                    tx2 = pml_term_l(pm.psiH[2][0] + q, bz, az, kz, x2, live);
-                    ty1 = pml_term_l(pm.psiH[2][1] + q, bz, az, kz, y1, live);
+                    const Off q = psi_z<Off>(a, p, sz);
+                    tx2 = pml_term(pm.psiH[2][0], q, pv[4], bz, az, kz, x2, live);
+                    ty1 = pml_term(pm.psiH[2][1], q, pv[5], bz, az, kz, y1, live);
                 }
                 x1 = tx1, x2 = tx2, y1 = ty1, y2 = ty2, z1 = tz1, z2 = tz2;
             }
@@ -337,41 +420,45 @@ __device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &
                 hz -= cf[17] * sv[2];
             }
             if (live) {
-                Hx[c] = hx;
-                Hy[c] = hy;
-                Hz[c] = hz;
+                st(Hx, c, hx);
+                st(Hy, c, hy);
+                st(Hz, c, hz);
             }
         };
-        const long long c0 = p + (long long)kb * a.P;
+        const Off c0 = p + (Off)kb * P;
         if constexpr (KC > 0) {
             float ez[KC], ex_kp[KC], ey_kp[KC], ey_ip[KC], ez_ip[KC], ex_jp[KC], ez_jp[KC], hox[KC], hoy[KC], hoz[KC];
+            float pv[KC][6];
             int m[KC];
-            const float ex_k0 = Ex[c0], ey_k0 = Ey[c0];
+            const float ex_k0 = ld(Ex, c0), ey_k0 = ld(Ey, c0);
 #pragma unroll
             for (int t = 0; t < KC; t++) {
                 const int kk = kb + t < ke ? kb + t : ke - 1;
-                const long long c = p + (long long)kk * a.P;
-                ez[t] = Ez[c];
-                ex_kp[t] = Ex[c + a.P], ey_kp[t] = Ey[c + a.P];
-                ey_ip[t] = Ey[c + 1], ez_ip[t] = Ez[c + 1];
-                ex_jp[t] = Ex[c + a.I], ez_jp[t] = Ez[c + a.I];
-                hox[t] = Hx[c], hoy[t] = Hy[c], hoz[t] = Hz[c];
-                m[t] = mat_id<MatT>(a.mat, c);
+                const Off c = p + (Off)kk * P;
+                ez[t] = ld(Ez, c);
+                ex_kp[t] = ld(Ex, c + P), ey_kp[t] = ld(Ey, c + P);
+                ey_ip[t] = ld(Ey, c + 1), ez_ip[t] = ld(Ez, c + 1);
+                ex_jp[t] = ld(Ex, c + I), ez_jp[t] = ld(Ez, c + I);
+                hox[t] = ld(Hx, c), hoy[t] = ld(Hy, c), hoz[t] = ld(Hz, c);
+                m[t] = mat_at<MatT>(a.mat, c);
+                if (PML) psi_load(pm.psiH, pm, a, i, j, p, kk, sx, sy, pv[t]);
             }
 #pragma unroll
             for (int t = 0; t < KC; t++) {
                 const bool live = kb + t < ke;
                 const int kk = live ? kb + t : ke - 1;
-                cell(kk, live, p + (long long)kk * a.P, t == 0 ? ex_k0 : ex_kp[t - 1], t == 0 ? ey_k0 : ey_kp[t - 1],
-                     ez[t], ex_kp[t], ey_kp[t], ey_ip[t], ez_ip[t], ex_jp[t], ez_jp[t], hox[t], hoy[t], hoz[t], m[t]);
+                cell(kk, live, p + (Off)kk * P, t == 0 ? ex_k0 : ex_kp[t - 1], t == 0 ? ey_k0 : ey_kp[t - 1], ez[t],
+                     ex_kp[t], ey_kp[t], ey_ip[t], ez_ip[t], ex_jp[t], ez_jp[t], hox[t], hoy[t], hoz[t], m[t], pv[t]);
             }
         } else {
-            float ex = Ex[c0], ey = Ey[c0];
-            long long c = c0;
-            for (int k = kb; k < ke; k++, c += a.P) {
-                const float ex_kp = Ex[c + a.P], ey_kp = Ey[c + a.P];
-                cell(k, true, c, ex, ey, Ez[c], ex_kp, ey_kp, Ey[c + 1], Ez[c + 1], Ex[c + a.I], Ez[c + a.I], Hx[c],
-                     Hy[c], Hz[c], mat_id<MatT>(a.mat, c));
+            float ex = ld(Ex, c0), ey = ld(Ey, c0);
+            Off c = c0;
+            for (int k = kb; k < ke; k++, c += P) {
+                const float ex_kp = ld(Ex, c + P), ey_kp = ld(Ey, c + P);
+                float pv[6];
+                if (PML) psi_load(pm.psiH, pm, a, i, j, p, k, sx, sy, pv);
+                cell(k, true, c, ex, ey, ld(Ez, c), ex_kp, ey_kp, ld(Ey, c + 1), ld(Ez, c + 1), ld(Ex, c + I),
+                     ld(Ez, c + I), ld(Hx, c), ld(Hy, c), ld(Hz, c), mat_at<MatT>(a.mat, c), pv);
                 ex = ex_kp;
                 ey = ey_kp;
             }
@@ -379,7 +466,6 @@ __device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &
     }
     if (MON) monitor_epilogue(a, kb, ke, pb, p, active);
 }
-
 
 }  // namespace
 }  // namespace fdtd

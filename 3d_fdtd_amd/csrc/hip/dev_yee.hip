@@ -15,23 +15,23 @@
 namespace fdtd {
 namespace {
 
-template <typename MatT, bool PML, int KC>
+template <typename MatT, bool PML, int KC, typename Off>
 __global__ __launch_bounds__(kBlock) void k_update_E(YeeArgs a, PmlArgs pm) {
-    update_E_block<MatT, PML, KC>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
+    update_E_block<MatT, PML, KC, Off>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
 }
 
-template <typename MatT, bool PML, int KC>
+template <typename MatT, bool PML, int KC, typename Off>
 __global__ __launch_bounds__(kBlock) void k_update_H(YeeArgs a, PmlArgs pm) {
-    update_H_block<MatT, PML, false, KC>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
+    update_H_block<MatT, PML, false, KC, Off>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
 }
 
 // The monitor arguments raise the scalar register count of the runtime-chunk variant just past
 // the 8-waves-per-SIMD budget; the attribute keeps the occupancy of the plain update there (a few
 // scalars spill to vector lanes).  The unrolled variants are limited by vector registers anyway.
-template <typename MatT, bool PML, int KC>
+template <typename MatT, bool PML, int KC, typename Off>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KC == 0 ? 8 : 1))) void k_update_H_mon(
     YeeArgs a, PmlArgs pm) {
-    update_H_block<MatT, PML, true, KC>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
+    update_H_block<MatT, PML, true, KC, Off>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
 }
 
 struct FaceArgs {
@@ -100,7 +100,7 @@ __global__ void k_mur_face(FaceArgs fa, int a, int side, int field, int k0, int 
     const int f = a + 3 * side;
     const float *sh0 = fa.shell + fa.shell_off[f] + uv;          // boundary layer
     const float *sh1 = sh0 + 4 * area;                          // inner layer
-    const float *cf = fa.coef + (long long)mat_id<MatT>(fa.mat, cb) * kCoefStride + 18 + 6 * field + 2 * a;
+    const float *cf = fa.coef + (long long)mat_at<MatT>(fa.mat, cb) * kCoefStride + 18 + 6 * field + 2 * a;
     const int comp0 = 3 * field;
     // q = 0 -> component b, q = 1 -> component cc
     {
@@ -206,25 +206,26 @@ PmlArgs pml_args(fdtd_dev *d) {
     return pm;
 }
 
-template <int KC>
+// Off: cell-offset type of the kernels (uint32_t for grids below 2^30 cells, else long long)
+template <int KC, typename Off>
 void launch_split(bool e_field, bool mon, bool u8, bool pml, dim3 grid, hipStream_t st, hipEvent_t t0, hipEvent_t t1,
                   const YeeArgs &a, const PmlArgs &pm) {
 #define FDTD_GO(KER) hipExtLaunchKernelGGL(KER, grid, dim3(kBlock), 0, st, t0, t1, 0, a, pm)
     if (e_field) {
-        if (u8 && pml) FDTD_GO((k_update_E<uint8_t, true, KC>));
-        else if (u8) FDTD_GO((k_update_E<uint8_t, false, KC>));
-        else if (pml) FDTD_GO((k_update_E<uint16_t, true, KC>));
-        else FDTD_GO((k_update_E<uint16_t, false, KC>));
+        if (u8 && pml) FDTD_GO((k_update_E<uint8_t, true, KC, Off>));
+        else if (u8) FDTD_GO((k_update_E<uint8_t, false, KC, Off>));
+        else if (pml) FDTD_GO((k_update_E<uint16_t, true, KC, Off>));
+        else FDTD_GO((k_update_E<uint16_t, false, KC, Off>));
     } else if (!mon) {
-        if (u8 && pml) FDTD_GO((k_update_H<uint8_t, true, KC>));
-        else if (u8) FDTD_GO((k_update_H<uint8_t, false, KC>));
-        else if (pml) FDTD_GO((k_update_H<uint16_t, true, KC>));
-        else FDTD_GO((k_update_H<uint16_t, false, KC>));
+        if (u8 && pml) FDTD_GO((k_update_H<uint8_t, true, KC, Off>));
+        else if (u8) FDTD_GO((k_update_H<uint8_t, false, KC, Off>));
+        else if (pml) FDTD_GO((k_update_H<uint16_t, true, KC, Off>));
+        else FDTD_GO((k_update_H<uint16_t, false, KC, Off>));
     } else {
-        if (u8 && pml) FDTD_GO((k_update_H_mon<uint8_t, true, KC>));
-        else if (u8) FDTD_GO((k_update_H_mon<uint8_t, false, KC>));
-        else if (pml) FDTD_GO((k_update_H_mon<uint16_t, true, KC>));
-        else FDTD_GO((k_update_H_mon<uint16_t, false, KC>));
+        if (u8 && pml) FDTD_GO((k_update_H_mon<uint8_t, true, KC, Off>));
+        else if (u8) FDTD_GO((k_update_H_mon<uint8_t, false, KC, Off>));
+        else if (pml) FDTD_GO((k_update_H_mon<uint16_t, true, KC, Off>));
+        else FDTD_GO((k_update_H_mon<uint16_t, false, KC, Off>));
     }
 #undef FDTD_GO
 }
@@ -272,6 +273,7 @@ int base_args(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool pml, YeeA
     a.nmon = 0;
     a.step = nullptr;
     a.step0 = 0;
+    set_source_range(a);
     *pmp = pml_args(d);
     return 0;
 }
@@ -288,6 +290,7 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
     const int zchunks = (nz + a.kchunk - 1) / a.kchunk;
     a.nrs = d->n_run_src;
     for (int q = 0; q < d->n_run_src; q++) a.rs[q] = d->run_src[q];
+    set_source_range(a);
     // run monitors ride on a full-range H update only
     const bool mon = !e_field && d->n_run_mon > 0 && k0 == 1 && k1 == d->K - 1;
     a.nmon = mon ? d->n_run_mon : 0;
@@ -297,11 +300,15 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
     const bool u8 = d->mat_bytes == 1;
     hipEvent_t t0, t1;
     timing_events(d, e_field ? 0 : 1, &t0, &t1);
-    switch (a.kchunk) {
-        case 2: launch_split<2>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
-        case 3: launch_split<3>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
-        case 4: launch_split<4>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
-        default: launch_split<0>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
+    if (d->N < kSmallGridCells) {
+        switch (a.kchunk) {
+            case 2: launch_split<2, uint32_t>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
+            case 3: launch_split<3, uint32_t>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
+            case 4: launch_split<4, uint32_t>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
+            default: launch_split<0, uint32_t>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
+        }
+    } else {
+        launch_split<0, long long>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm);
     }
     FDTD_TRY(hipGetLastError());
     return 0;
