@@ -192,6 +192,12 @@ SIGNATURES = {
     "fdtd_mlp_loss_grad": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _i, _P(_d), _P(C.c_float)], _i),
     "fdtd_mlp_adam_step": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _i, C.c_float, C.c_float, C.c_float, C.c_float,
                             _P(_d)], _i),
+    "fdtd_mlp_train": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _i, _i, C.c_float, C.c_float, C.c_float, C.c_float,
+                        _P(_d), _P(_d)], _i),
+    "fdtd_mlp_gemm_timing": ([C.c_void_p, _i], _i),
+    "fdtd_mlp_gemm_times": ([C.c_void_p, _P(_i), _P(_d), _P(_d)], _i),
+    "fdtd_mlp_timer_start": ([C.c_void_p], _i),
+    "fdtd_mlp_timer_stop": ([C.c_void_p, _P(_d)], _i),
 }
 
 _lib = None
@@ -470,6 +476,32 @@ class Mlp:
         dev_check(lib().fdtd_mlp_adam_step(self.m, self._fp(x), self._fp(t), x.shape[0], lr, beta1, beta2, eps,
                                            C.byref(loss)), "mlp adam_step")
         return loss.value
+
+    def train(self, x, t, steps, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        """`steps` full-batch Adam steps on device-resident (x, t); returns (loss before, loss after)."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        first, last = C.c_double(), C.c_double()
+        dev_check(lib().fdtd_mlp_train(self.m, self._fp(x), self._fp(t), x.shape[0], steps, lr, beta1, beta2, eps,
+                                       C.byref(first), C.byref(last)), "mlp train")
+        return first.value, last.value
+
+    def gemm_timing(self, on=True):
+        dev_check(lib().fdtd_mlp_gemm_timing(self.m, int(on)), "mlp gemm_timing")
+
+    def gemm_times(self):
+        """(launches, summed ms, flop) of the GEMM launches timed since gemm_timing(True)."""
+        n, ms, fl = C.c_int(), C.c_double(), C.c_double()
+        dev_check(lib().fdtd_mlp_gemm_times(self.m, C.byref(n), C.byref(ms), C.byref(fl)), "mlp gemm_times")
+        return n.value, ms.value, fl.value
+
+    def timer_start(self):
+        dev_check(lib().fdtd_mlp_timer_start(self.m), "mlp timer")
+
+    def timer_stop(self):
+        ms = C.c_double()
+        dev_check(lib().fdtd_mlp_timer_stop(self.m, C.byref(ms)), "mlp timer")
+        return ms.value
 
     def free(self):
         if self.m:

@@ -225,10 +225,7 @@ def fit(family, designs, t, width=256, depth=4, steps=2000, lr=1e-3, seed=0):
     y = to_targets(t)
     dims = (x.shape[1], width, depth, y.shape[1])
     net = fd.Mlp(*dims, params=he_uniform(dims, seed))
-    loss = None
-    for _ in range(steps):
-        loss = net.adam_step(x, y, lr=lr)
-    net.last_loss = loss
+    _, net.last_loss = net.train(x, y, steps, lr=lr)
     return net
 
 

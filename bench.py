@@ -246,6 +246,64 @@ def bench_design(fd, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_surrogate(fd, args):
+    """The transmission surrogate's hand-written BF16 matrix-core kernels: full-batch Adam steps on
+    4096 samples and screening of 10^5 candidates (the design loop's shapes: 9 -> 256 x 4 -> 32,
+    GELU).  value = candidates screened per second (device time on the network's stream, host
+    transfers of inputs and outputs included); roofline = the GEMM launches' arithmetic over their
+    own dispatch-recorded durations, against the dense BF16 matrix-core peak."""
+    design = importlib.import_module("3d_fdtd_amd.design")
+    dims = (9, 256, 4, 32)
+    rng = np.random.default_rng(TAPER_SEED)
+    n_train, n_pool = 4096, 100000
+    x = rng.uniform(-1, 1, (n_train, dims[0])).astype(np.float32)
+    t = np.sin(x @ rng.normal(0, 0.6, (dims[0], dims[3]))).astype(np.float32)
+    pool = rng.uniform(-1, 1, (n_pool, dims[0])).astype(np.float32)
+    net = fd.Mlp(*dims, params=design.he_uniform(dims, 0))
+    net.train(x, t, 20)
+    steps = args.steps if args.steps != N_RUN else 200
+    net.timer_start()
+    loss0, loss1 = net.train(x, t, steps)
+    train_ms = net.timer_stop() / steps
+    net.gemm_timing(True)
+    net.train(x, t, 20)
+    tl, tms, tfl = net.gemm_times()
+    net.forward(pool)
+    reps = 20
+    net.gemm_timing(True)
+    net.timer_start()
+    for _ in range(reps):
+        net.forward(pool)
+    fwd_ms = net.timer_stop() / reps
+    fl, fms, ffl = net.gemm_times()
+    net.gemm_timing(False)
+    net.free()
+    tflops = ffl / (fms / 1e3) / 1e12
+    # algorithmic bytes of one screening forward: each layer reads its bf16 input rows and weights
+    # and writes bf16 activations (FP32 outputs for the last layer); 41.4 GFLOP over ~426 MB is
+    # ~97 FLOP/B, below the bf16 ridge (2.5 PFLOP/s / 8 TB/s = 312 FLOP/B): the GEMMs are HBM-bound
+    kin = (dims[0] + 7) // 8 * 8
+    fins = [kin] + [dims[1]] * dims[2]
+    fouts = [dims[1]] * dims[2] + [dims[3]]
+    fwd_bytes = sum(n_pool * fi * 2 + fi * fo * 2 + n_pool * fo * (4 if q == len(fins) - 1 else 2)
+                    for q, (fi, fo) in enumerate(zip(fins, fouts)))
+    achieved = fwd_bytes / (fms / reps / 1e3) / 1e9
+    out = {"metric": METRIC, "value": n_pool / (fwd_ms / 1e3), "unit": "screened candidates/s", "n_gpus": 1,
+           "steps": steps, "warmup": 20, "ms_per_step": train_ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16 operands, fp32 accumulate", "data": "synthetic",
+           "config": {"workload": "transmission surrogate MLP 9-256x4-32 GELU: Adam on 4096 samples, screening "
+                                  "10^5 candidates", "train_samples": n_train, "screened": n_pool},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "k_gemm_nt (screening forward, 5 launches)",
+                        "algorithmic_bytes_per_forward": fwd_bytes, "gemm_ms_per_forward": round(fms / reps, 4),
+                        "mfma_tflops": round(tflops, 2), "mfma_frac_of_2500": round(tflops / 2500.0, 5),
+                        "train_gemm_tflops": round(tfl / (tms / 1e3) / 1e12, 2)},
+           "surrogate": {"adam_step_ms": round(train_ms, 4), "screen_1e5_ms": round(fwd_ms, 4),
+                         "loss_before": loss0, "loss_after": loss1}}
+    print(json.dumps(out), flush=True)
+
+
 def design_cpu_baseline(fam, loop, runs_per_design, n=3):
     """FP64 oracle forward runs of the first n accepted designs (one core): the CPU cost of the
     plain loop per accepted design, and the FP32 GPU transmission error on those designs."""
@@ -344,9 +402,10 @@ def main():
     ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
     ap.add_argument("--kchunk", type=int, default=None,
                     help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
-    ap.add_argument("--workload", choices=("waveguide", "tapers", "design"), default="waveguide",
+    ap.add_argument("--workload", choices=("waveguide", "tapers", "design", "surrogate"), default="waveguide",
                     help="waveguide: the strip waveguide (default); tapers: the batched taper set; design: "
-                         "verified taper designs/hour, surrogate loop vs plain adjoint loop")
+                         "verified taper designs/hour, surrogate loop vs plain adjoint loop; surrogate: the MLP's "
+                         "matrix-core training and screening kernels")
     ap.add_argument("--train", type=int, default=4096, help="design workload: FDTD training designs")
     ap.add_argument("--fit-steps", type=int, default=1500, help="design workload: surrogate training steps")
     ap.add_argument("--campaign", type=float, default=150.0,
@@ -372,6 +431,9 @@ def main():
         return
     if args.workload == "design":
         bench_design(fd, args)
+        return
+    if args.workload == "surrogate":
+        bench_surrogate(fd, args)
         return
     W, S = args.warmup, args.steps
     g, dt, md, mons, omegas = workload(fd, W + S)

@@ -35,6 +35,19 @@ int fdtd_mlp_loss_grad(fdtd_mlp *m, const float *x, const float *t, int n, doubl
 /* one Adam step on the batch (x, t); *loss is the loss before the step */
 int fdtd_mlp_adam_step(fdtd_mlp *m, const float *x, const float *t, int n, float lr, float beta1, float beta2,
                        float eps, double *loss);
+/* `steps` full-batch Adam steps on (x, t), uploaded once and kept on the device; the loss before
+ * the first step and after the last one (either pointer may be NULL; each read synchronises once) */
+int fdtd_mlp_train(fdtd_mlp *m, const float *x, const float *t, int n, int steps, float lr, float beta1, float beta2,
+                   float eps, double *loss_first, double *loss_last);
+
+/* Measurement.  gemm_timing(on) clears the record and, while on, brackets every matrix-core GEMM
+ * launch with events recorded at its dispatch; gemm_times returns the number of recorded launches,
+ * their summed duration (ms) and their arithmetic (2 x multiply-adds), and clears the record.
+ * timer_start / timer_stop time everything enqueued on the network's stream in between. */
+int fdtd_mlp_gemm_timing(fdtd_mlp *m, int on);
+int fdtd_mlp_gemm_times(fdtd_mlp *m, int *launches, double *ms, double *flop);
+int fdtd_mlp_timer_start(fdtd_mlp *m);
+int fdtd_mlp_timer_stop(fdtd_mlp *m, double *elapsed_ms);
 
 #ifdef __cplusplus
 }

@@ -81,3 +81,22 @@ def test_training_reduces_loss():
     say("training loss %.4e -> %.4e" % (first, last))
     assert last < 0.1 * first
     net.free()
+
+
+def test_resident_training_equals_step_by_step():
+    """fdtd_mlp_train (data uploaded once, loss read back only at the ends) performs exactly the
+    arithmetic of the same number of fdtd_mlp_adam_step calls: parameters equal bit for bit."""
+    x, t = _case(300, 12)
+    a = fd.Mlp(*DIMS, params=M.init_params(*DIMS, seed=8))
+    b = fd.Mlp(*DIMS, params=M.init_params(*DIMS, seed=8))
+    first_a = a.adam_step(x, t, lr=2e-3)
+    for _ in range(14):
+        a.adam_step(x, t, lr=2e-3)
+    first_b, last_b = b.train(x, t, 15, lr=2e-3)
+    pa, pb = a.params(), b.params()
+    say("losses: first %.6e / %.6e, after %.6e; max |dp| %.1e" % (first_a, first_b, last_b, np.abs(pa - pb).max()))
+    assert first_a == first_b
+    assert np.array_equal(pa, pb)
+    assert last_b < first_b
+    a.free()
+    b.free()
