@@ -108,6 +108,19 @@ SIGNATURES = {
     "mode_gaussian": ([_i, _i, _d, Complex, Complex, _d, _d, _i], Mode),
     "mode_power_inner_product": ([Mode, Mode], Complex),
     "mode_power_normalize": ([Mode], None),
+    "mode_H_inner_product": ([Mode, Mode], Complex),
+    "mode_H_normalize": ([Mode], None),
+    "mode_set_background": ([Mode, Complex, Complex], None),
+    "mode_make_rectangle": ([Mode, Complex, Complex, _i, _i, _i, _i], None),
+    "mode_set_pml": ([Mode, _d, _i, _i], None),
+    "mode_apply_bc": ([Mode, _i], None),
+    "mode_apply_beta_squared_operator": ([Mode, _d], Mode),
+    "mode_solve_beta_squared": ([_P(Mode), _d, _d, _P(Mode), _i, _i, _i, _i], Complex),
+    # matrix.h
+    "matrix_new": ([_i, _i], Matrix),
+    "matrix_free": ([Matrix], None),
+    "matrix_diagonalize": ([Matrix, _d, _i], Matrix),
+    "matrix_eigenvector": ([Matrix, Complex], Matrix),
     # complex.h
     "complex_exp": ([Complex], Complex),
     "complex_angle": ([Complex], _d),
@@ -244,6 +257,34 @@ def arrays(tf):
 def mode_fields(md):
     """Interleaved (re, im) view of a mode's 6*i*j complex fields."""
     return _view(md.fields.m, 12 * md.i * md.j, C.c_double, np.float64)
+
+
+def matrix_view(mat, n):
+    """Interleaved (re, im) view of the first n complex entries of a Matrix."""
+    return _view(mat.m, 2 * n, C.c_double, np.float64)
+
+
+def solve_mode(ep, disc, omega, tol=1e-9, size=20, confinement=2, pml=None, lateral_symmetry=0, prev=()):
+    """Fundamental mode of a cross-section by the library's mode solver (mode_solve_beta_squared).
+
+    ep: complex permittivity [j][i][3] (x, y, z components); pml: None or (max_ep_im, order,
+    thickness) for mode_set_pml; prev: modes (Mode) the result is kept orthogonal to.
+    Returns (beta, Mode); the caller frees the Mode (lib().mode_free)."""
+    ep = np.asarray(ep, np.complex128)
+    mj, mi = ep.shape[:2]
+    md = lib().mode_new(mi, mj, disc)
+    e = matrix_view(md.ep, 3 * mi * mj).reshape(-1, 2)
+    e[:, 0] = ep.reshape(-1).real
+    e[:, 1] = ep.reshape(-1).imag
+    m = matrix_view(md.mu, 3 * mi * mj).reshape(-1, 2)
+    m[:, 0] = 1.0
+    m[:, 1] = 0.0
+    if pml is not None:
+        lib().mode_set_pml(md, float(pml[0]), int(pml[1]), int(pml[2]))
+    prev = list(prev)
+    arr = (Mode * max(len(prev), 1))(*prev) if prev else None
+    beta = lib().mode_solve_beta_squared(C.byref(md), omega, tol, arr, len(prev), size, confinement, lateral_symmetry)
+    return complex(beta.re, beta.im), md
 
 
 def mode_from_array(mi, mj, disc, values):

@@ -219,3 +219,199 @@ void matrix_vector_normalize_in_place(matrix A) {
     double n = matrix_vector_norm(A);
     matrix_smul_in_place(1.0 / n, A);
 }
+
+/* ---- eigenvalues of a small dense complex matrix (matrix.c:672-871, 1114-1158) ----
+ * The reference reaches a triangular form with an unshifted QL iteration and stops when every
+ * entry above the diagonal is below tol x the largest diagonal entry.  Here: Householder
+ * reduction to upper Hessenberg form, then single-shift QR steps with Wilkinson shifts and
+ * deflation of negligible sub-diagonal entries (Golub & Van Loan, Matrix Computations, 7.4-7.5),
+ * so the eigenvalues are converged to working precision; tol is kept in the signature and
+ * max_iterations bounds the QR steps per eigenvalue.  Returns the diagonal matrix of the
+ * eigenvalues (upper-left to lower-right in deflation order, last ones first). */
+
+static complex cplx(double re, double im) { return complex_from_doubles(re, im); }
+static double cabs1(complex z) { return fabs(z.re) + fabs(z.im); }
+
+/* Givens rotation G = [[c, s], [-conj(s), c]] (c real) with G (a, b)^T = (r, 0)^T */
+static void givens(complex a, complex b, double *c, complex *s) {
+    const double na = complex_mag(a), nb = complex_mag(b);
+    if (nb == 0.0) {
+        *c = 1.0;
+        *s = complex_zero();
+        return;
+    }
+    if (na == 0.0) {
+        *c = 0.0;
+        *s = complex_conj(complex_smul(1.0 / nb, b));
+        return;
+    }
+    const double r = hypot(na, nb);
+    *c = na / r;
+    *s = complex_smul(1.0 / r, complex_mul(complex_smul(1.0 / na, a), complex_conj(b)));
+}
+
+static void hessenberg(complex *H, int n) {
+    for (int k = 0; k < n - 2; k++) {
+        double alpha = 0.0;
+        for (int r = k + 1; r < n; r++) alpha += complex_mag(H[k + r * n]) * complex_mag(H[k + r * n]);
+        alpha = sqrt(alpha);
+        if (alpha == 0.0) continue;
+        /* v = x + e^{i arg x0} |x| e0, reflector I - 2 v v^H / (v^H v) */
+        const complex x0 = H[k + (k + 1) * n];
+        const double m0 = complex_mag(x0);
+        const complex ph = m0 > 0 ? complex_smul(1.0 / m0, x0) : cplx(1, 0);
+        complex *v = calloc((size_t)n, sizeof(complex));
+        for (int r = k + 1; r < n; r++) v[r] = H[k + r * n];
+        v[k + 1] = complex_add(v[k + 1], complex_smul(alpha, ph));
+        double vv = 0.0;
+        for (int r = k + 1; r < n; r++) vv += complex_mag(v[r]) * complex_mag(v[r]);
+        if (vv == 0.0) {
+            free(v);
+            continue;
+        }
+        for (int c = 0; c < n; c++) {  /* H <- (I - 2 v v^H / vv) H */
+            complex d = complex_zero();
+            for (int r = k + 1; r < n; r++) d = complex_add(d, complex_mul(complex_conj(v[r]), H[c + r * n]));
+            d = complex_smul(2.0 / vv, d);
+            for (int r = k + 1; r < n; r++) H[c + r * n] = complex_sub(H[c + r * n], complex_mul(v[r], d));
+        }
+        for (int r = 0; r < n; r++) {  /* H <- H (I - 2 v v^H / vv) */
+            complex d = complex_zero();
+            for (int c = k + 1; c < n; c++) d = complex_add(d, complex_mul(H[c + r * n], v[c]));
+            d = complex_smul(2.0 / vv, d);
+            for (int c = k + 1; c < n; c++) H[c + r * n] = complex_sub(H[c + r * n], complex_mul(d, complex_conj(v[c])));
+        }
+        for (int r = k + 2; r < n; r++) H[k + r * n] = complex_zero();
+        free(v);
+    }
+}
+
+matrix matrix_diagonalize(matrix A, double tol, int max_iterations) {
+    (void)tol;
+    need(A.i == A.j, "Matrix is not square for QL diagonalization!");
+    const int n = A.i;
+    matrix D = matrix_new(n, n);
+    if (n == 0) return D;
+    complex *H = malloc((size_t)n * n * sizeof(complex));
+    need(H != NULL, "out of memory");
+    memcpy(H, A.m, (size_t)n * n * sizeof(complex));
+    hessenberg(H, n);
+    const double eps = 2.220446049250313e-16;
+    int hi = n - 1, its = 0;
+    const int cap = max_iterations > 30 ? max_iterations : 30;
+    while (hi > 0) {
+        int lo = hi;
+        while (lo > 0) {  /* find the top of the unreduced block ending at hi */
+            const double scale = cabs1(H[lo + lo * n]) + cabs1(H[(lo - 1) + (lo - 1) * n]);
+            if (cabs1(H[(lo - 1) + lo * n]) <= eps * (scale > 0 ? scale : 1.0)) {
+                H[(lo - 1) + lo * n] = complex_zero();
+                break;
+            }
+            lo--;
+        }
+        if (lo == hi) {  /* one eigenvalue deflated */
+            hi--;
+            its = 0;
+            continue;
+        }
+        if (++its > cap) break;
+        /* Wilkinson shift: the eigenvalue of the trailing 2 x 2 block nearer its last entry */
+        const complex a = H[(hi - 1) + (hi - 1) * n], b = H[hi + (hi - 1) * n];
+        const complex c = H[(hi - 1) + hi * n], d = H[hi + hi * n];
+        const complex half_tr = complex_smul(0.5, complex_add(a, d));
+        const complex disc = complex_sqrt(complex_add(complex_mul(complex_smul(0.5, complex_sub(a, d)),
+                                                                  complex_smul(0.5, complex_sub(a, d))),
+                                                      complex_mul(b, c)));
+        const complex l1 = complex_add(half_tr, disc), l2 = complex_sub(half_tr, disc);
+        complex mu = complex_mag(complex_sub(l1, d)) < complex_mag(complex_sub(l2, d)) ? l1 : l2;
+        if (its % 11 == 10) mu = complex_add(d, cplx(cabs1(H[(hi - 1) + hi * n]), 0));  /* exceptional shift */
+        /* implicit single-shift QR step on rows/columns lo..hi (bulge chase with Givens rotations) */
+        complex x = complex_sub(H[lo + lo * n], mu), y = H[lo + (lo + 1) * n];
+        for (int k = lo; k < hi; k++) {
+            double cg;
+            complex sg;
+            givens(x, y, &cg, &sg);
+            for (int col = (k > lo ? k - 1 : lo); col <= hi; col++) {  /* rows k, k+1 <- G (rows) */
+                const complex t1 = H[col + k * n], t2 = H[col + (k + 1) * n];
+                H[col + k * n] = complex_add(complex_smul(cg, t1), complex_mul(sg, t2));
+                H[col + (k + 1) * n] = complex_sub(complex_smul(cg, t2), complex_mul(complex_conj(sg), t1));
+            }
+            const int rmax = k + 2 < hi ? k + 2 : hi;
+            for (int row = lo; row <= rmax; row++) {  /* columns k, k+1 <- (columns) G^H */
+                const complex t1 = H[k + row * n], t2 = H[(k + 1) + row * n];
+                H[k + row * n] = complex_add(complex_smul(cg, t1), complex_mul(complex_conj(sg), t2));
+                H[(k + 1) + row * n] = complex_sub(complex_smul(cg, t2), complex_mul(sg, t1));
+            }
+            if (k < hi - 1) {
+                x = H[k + (k + 1) * n];
+                y = H[k + (k + 2) * n];
+            }
+        }
+    }
+    for (int q = 0; q < n; q++) D.m[q + q * n] = H[q + q * n];
+    free(H);
+    return D;
+}
+
+/* Eigenvector of A for an eigenvalue: inverse iteration on A - lambda I (the shift nudged by a
+ * few ulps so the factorisation stays regular), normalised to unit 2-norm with the phase the
+ * reference's construction gives (sum of the components real and positive, matrix.c:1114-1158). */
+matrix matrix_eigenvector(matrix A, complex eigenvalue) {
+    need(A.i == A.j, "A matrix is not square for eigenvector solve!");
+    const int n = A.i;
+    matrix x = matrix_new(n, 1);
+    if (n == 0) return x;
+    double an = 0.0;
+    for (int q = 0; q < n * n; q++) an = fmax(an, complex_mag(A.m[q]));
+    const double nudge = 1e-13 * (an > 0 ? an : 1.0);
+    const complex lam = complex_add(eigenvalue, cplx(nudge, nudge));
+    /* LU with partial pivoting of B = A - lam I, pivots floored at the nudge */
+    complex *B = malloc((size_t)n * n * sizeof(complex));
+    int *piv = malloc((size_t)n * sizeof(int));
+    need(B && piv, "out of memory");
+    memcpy(B, A.m, (size_t)n * n * sizeof(complex));
+    for (int q = 0; q < n; q++) B[q + q * n] = complex_sub(B[q + q * n], lam);
+    for (int col = 0; col < n; col++) {
+        int p = col;
+        for (int r = col + 1; r < n; r++)
+            if (complex_mag(B[col + r * n]) > complex_mag(B[col + p * n])) p = r;
+        piv[col] = p;
+        if (p != col)
+            for (int c = 0; c < n; c++) {
+                const complex t = B[c + col * n];
+                B[c + col * n] = B[c + p * n];
+                B[c + p * n] = t;
+            }
+        if (complex_mag(B[col + col * n]) < nudge) B[col + col * n] = cplx(nudge, 0);
+        const complex inv = complex_reciprocal(B[col + col * n]);
+        for (int r = col + 1; r < n; r++) {
+            const complex f = complex_mul(B[col + r * n], inv);
+            B[col + r * n] = f;
+            for (int c = col + 1; c < n; c++) B[c + r * n] = complex_sub(B[c + r * n], complex_mul(f, B[c + col * n]));
+        }
+    }
+    for (int q = 0; q < n; q++) x.m[q] = cplx(1, 0);
+    for (int it = 0; it < 3; it++) {
+        for (int col = 0; col < n; col++)
+            if (piv[col] != col) {
+                const complex t = x.m[col];
+                x.m[col] = x.m[piv[col]];
+                x.m[piv[col]] = t;
+            }
+        for (int r = 0; r < n; r++)
+            for (int c = 0; c < r; c++) x.m[r] = complex_sub(x.m[r], complex_mul(B[c + r * n], x.m[c]));
+        for (int r = n - 1; r >= 0; r--) {
+            complex s = x.m[r];
+            for (int c = r + 1; c < n; c++) s = complex_sub(s, complex_mul(B[c + r * n], x.m[c]));
+            x.m[r] = complex_mul(s, complex_reciprocal(B[r + r * n]));
+        }
+        matrix_vector_normalize_in_place(x);
+    }
+    complex sum = complex_zero();
+    for (int q = 0; q < n; q++) sum = complex_add(sum, x.m[q]);
+    const double ms = complex_mag(sum);
+    if (ms > 0) matrix_complex_smul_in_place(complex_smul(1.0 / ms, complex_conj(sum)), x);
+    free(B);
+    free(piv);
+    return x;
+}

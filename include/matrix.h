@@ -47,6 +47,11 @@ matrix matrix_mul(matrix A, matrix B);
 matrix matrix_linear_solve(matrix A, matrix B);        /* Gaussian elimination, partial pivot */
 matrix matrix_inverse(matrix A);
 void matrix_gram_schmidt(matrix A);
+/* Eigenvalues of a square matrix as a diagonal matrix (matrix.c:672: there by a QL iteration
+ * stopped at an off-diagonal ratio tol; here converged to working precision) and the unit
+ * eigenvector for one of them (matrix.c:1114; phase: components sum to a positive real). */
+matrix matrix_diagonalize(matrix A, double tol, int max_iterations);
+matrix matrix_eigenvector(matrix A, complex eigenvalue);
 
 complex matrix_vector_inner_product(matrix A, matrix B);
 double matrix_vector_norm(matrix A);

@@ -7,9 +7,10 @@
  * The FDTD hot path consumes modes through timefield3d_mode_source/_adjoint/_overlap, which
  * power-normalise the caller's mode in place on every call (mode.c:1367).
  *
- * Not provided by this library: the finite-difference eigen-solvers (mode_solve_*), their
- * operators and PML setters, radial/angular variants and the susceptibility integrals
- * (mode.c:1381-2328, 2399-3825, 3827-4527) -- host-side setup outside the FDTD hot path.
+ * The finite-difference mode solver (mode_solve_beta_squared with its operator, boundary
+ * conditions, PML profile and lateral symmetry) is provided; the first-order beta solver, the
+ * one-sided PML setters, radial/angular variants and the susceptibility integrals
+ * (mode.c:1792-2198, 2399-2795, 3146-3349, 3404-3460, 3618-3788, 3827-4493) are not.
  */
 #ifndef _mode_h
 #define _mode_h
@@ -76,6 +77,31 @@ void mode_make_rectangle(mode psi, complex ep, complex mu, int left, int right, 
 
 void mode_apply_H_curl(mode psi, complex beta, double omega);              /* mode.c:2330 */
 void mode_apply_H_div(mode psi, complex beta);                             /* mode.c:3350 */
+
+/* Mode solver: the fundamental (largest Re beta^2) mode of the cross-section in psi (ep, mu, pml
+ * set by the caller), orthogonal in the H inner product to prev_modes[0..prev_num); size is the
+ * Krylov dimension per restart, tol the change of |n_eff| between restarts that ends the
+ * iteration; confinement 1 = slab (zero top/bottom, zero-gradient sides), 2 = zero on every edge.
+ * Returns beta; psi holds all six components, H-normalised.  Starts from rand() values.  As in
+ * the reference only the restart vector is orthogonalised against prev_modes, so the operator
+ * (not normal in the H inner product) regrows their components inside each Krylov space: higher
+ * modes found this way are not reliable. */
+complex mode_solve_beta_squared(mode *psi, double omega, double tol, mode *prev_modes, int prev_num, int size,
+                                int confinement, int lateral_symmetry);        /* mode.c:3461 */
+mode mode_apply_beta_squared_operator(mode psi, double omega);             /* mode.c:2796 */
+void mode_initialize_general(mode psi);                                    /* mode.c:1164 */
+void mode_initialize_rand(mode psi);                                       /* mode.c:1192 */
+void mode_apply_zero_bc(mode psi);                                         /* mode.c:1381 */
+void mode_apply_zero_bc_vertical(mode psi);                                /* mode.c:1429 */
+void mode_apply_zero_bc_horizontal(mode psi);                              /* mode.c:1457 */
+void mode_apply_zero_gradient_bc_vertical(mode psi);                       /* mode.c:1532 */
+void mode_apply_zero_gradient_bc_horizontal(mode psi);                     /* mode.c:1560 */
+void mode_apply_zero_bc_pml(mode psi);                                     /* mode.c:1587 */
+void mode_apply_bc(mode psi, int confinement);                             /* mode.c:1614 */
+void mode_set_pml(mode psi, double max_ep_im, int order, int pml_thickness); /* mode.c:1634 */
+void mode_apply_lateral_symmetry(mode psi);                                /* mode.c:4494 */
+complex mode_true_beta(complex beta);                                      /* mode.c:3789 */
+mode mode_true_fields(mode psi);                                           /* mode.c:3798 */
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,8 @@ def lib():
         "mode_power_normalize": ([Mode], None),
         "mode_power_inner_product": ([Mode, Mode], Complex),
         "mode_smul": ([Complex, Mode], None),
+        "mode_set_pml": ([Mode, d, i, i], None),
+        "mode_apply_beta_squared_operator": ([Mode, d], Mode),
         "complex_exp": ([Complex], Complex),
         "complex_angle": ([Complex], d),
         "complex_sqrt": ([Complex], Complex),

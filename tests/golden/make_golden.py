@@ -1,7 +1,8 @@
 """Write the golden fixtures in tests/golden/ from the compiled reference.
 
 Run in the build container (needs /root/reference to build oracle/_ref):
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py          # every fixture
+    python tests/golden/make_golden.py f11                            # only the named ones
 
 Each fixture holds inputs and the reference's outputs for one hot-path function family; the
 reference is called through oracle/ref_py.py.  Large outputs are stored as a sha256 of the
@@ -321,8 +322,53 @@ def f9(L, manifest):
     manifest["F9_complex"] = {"inputs": "150 U[0,2pi) + 50 U[-20,20) imaginary; 60 U[-3,3)^2 complex"}
 
 
+def f11(L, manifest):
+    """F11: the mode solver's transverse-H operator (mode.c:2796-2943) on a Si strip cross-section
+    (n 3.48 in 1.44, 20 nm cells, 30 x 24) with an order-2 PML of 4 cells (mode.c:1634) and
+    random complex Hx, Hy (all six components set, as the operator must ignore the others)."""
+    rng = np.random.default_rng(1111)
+    mi, mj, disc, omega = 30, 24, DISC, 2 * np.pi / 1.55
+    md = L.mode_new(mi, mj, disc)
+    n = mi * mj
+    ep = np.full((mj, mi, 3), 1.44 ** 2, complex)
+    ep[9:15, 6:24, :] = 3.48 ** 2
+    mu = np.ones((mj, mi, 3), complex)
+    mu[:, :, 2] += 0.05j * rng.uniform(size=(mj, mi))  # exercises the mu_z division
+    fields = rng.normal(size=(n * 6,)) + 1j * rng.normal(size=(n * 6,))
+
+    def put(mat, arr):
+        v = R.view(mat.m, arr.size, np.float64).reshape(-1, 2)
+        v[:, 0] = arr.reshape(-1).real
+        v[:, 1] = arr.reshape(-1).imag
+
+    put(md.ep, ep)
+    put(md.mu, mu)
+    put(md.fields, fields)
+    L.mode_set_pml(md, 0.8, 2, 4)
+    pml = R.view(md.pml.m, 2 * n, np.float64).copy()
+    out = L.mode_apply_beta_squared_operator(md, omega)
+    res = R.view(out.fields.m, 6 * n, np.float64).copy()
+    L.mode_free(out)
+    L.mode_free(md)
+    save("F11_mode_operator.npz", dims=np.array([mi, mj]), disc=disc, omega=omega, ep=ep, mu=mu, fields=fields,
+         pml=pml, out=res)
+    manifest["F11_mode_operator"] = {"grid": [mi, mj], "disc_um": disc, "lambda_um": 1.55,
+                                     "core": "rows 9-14, columns 6-23, eps 3.48^2 in 1.44^2",
+                                     "pml": "mode_set_pml(0.8, order 2, 4 cells)", "seed": 1111}
+
+
 def main():
     L = R.lib()
+    only = sys.argv[1:]  # fixture function names (e.g. f11): regenerate just those
+    if only:
+        with open(os.path.join(HERE, "manifest.json")) as fh:
+            manifest = json.load(fh)
+        for name in only:
+            globals()[name](L, manifest)
+        with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+            json.dump(manifest, fh, indent=1, sort_keys=True)
+        print("fixtures", only, "written to", HERE)
+        return
     manifest = {"reference_sources_sha256": {}, "compiler": "gcc -O3 -fopenmp -fPIC (oracle/Makefile ref)",
                 "disc_um": DISC, "dt": DT}
     for fn in ("complex.c", "matrix.c", "mode.c", "timefield3d.c", "adjoint3d.c"):
@@ -335,6 +381,7 @@ def main():
     f6(L, manifest)
     f7(L, manifest)
     f9(L, manifest)
+    f11(L, manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, sort_keys=True)
     print("fixtures written to", HERE)
