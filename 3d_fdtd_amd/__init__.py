@@ -186,6 +186,8 @@ SIGNATURES = {
     "fdtd_dev_dft_gradient": ([C.c_void_p, _i, C.c_void_p, _i, _i, _d, _d, _P(_d)], _i),
     "fdtd_dev_synchronize": ([C.c_void_p], _i),
     "fdtd_dev_release_cached_memory": ([], _i),
+    "fdtd_dev_kernel_timing": ([C.c_void_p, _i], _i),
+    "fdtd_dev_kernel_times": ([C.c_void_p, _P(_i), _P(_d)], _i),
     "fdtd_dev_timer_start": ([C.c_void_p], _i),
     "fdtd_dev_timer_stop": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_batch_new": ([_P(C.c_void_p), _i], C.c_void_p),
@@ -415,6 +417,24 @@ class DevGrid:
 
     def run(self, dt, n0, n, t_offset=0.0):
         dev_check(lib().fdtd_dev_run(self.d, dt, n0, n, t_offset), "run")
+
+    def kernel_timing(self, every):
+        """Bracket every `every`-th split E / H launch with dispatch-recorded events (0: off)."""
+        dev_check(lib().fdtd_dev_kernel_timing(self.d, every), "kernel_timing")
+
+    def kernel_times(self):
+        """Mean durations (us) of the timed E and H launches and their counts; clears the record."""
+        cnt, ms = (C.c_int * 3)(), (C.c_double * 3)()
+        dev_check(lib().fdtd_dev_kernel_times(self.d, cnt, ms), "kernel_times")
+        return [ms[q] / cnt[q] * 1e3 if cnt[q] else 0.0 for q in range(2)], [cnt[0], cnt[1]]
+
+    def timer_start(self):
+        dev_check(lib().fdtd_dev_timer_start(self.d), "timer_start")
+
+    def timer_stop(self):
+        ms = C.c_double()
+        dev_check(lib().fdtd_dev_timer_stop(self.d, C.byref(ms)), "timer_stop")
+        return ms.value
 
     def overlap(self, mid, nfreq):
         cum = np.zeros(2 * nfreq)

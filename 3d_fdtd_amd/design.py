@@ -143,25 +143,30 @@ class TaperFamily:
         aid = ga.add_dft_box(box, 7, self.omegas)
         ga.run(self.dt, 0, self.steps, 1.0)
         t = a_out / a_in
-        n = np.arange(self.steps)
         grad_eps = np.zeros(nb)
         for f, w in enumerate(self.omegas):
-            E0 = np.sum(self.amp * np.exp(1j * (w - w0) * n * self.dt))
-            u = np.exp(-1j * w * self.dt) - 1.0
-            weight = 2.0 * np.conj(t[f]) * (2.0 * u / (self.dt * E0)) / a_in[f] / nf
+            weight = 2.0 * np.conj(t[f]) * adjoint_scale(self.amp, w, w0, self.dt) / a_in[f] / nf
             grad_eps += g.gradient(bid, ga, aid, f, weight, nb)
         g.free()
         ga.free()
-        grad_eps = grad_eps.reshape(km - ks - 4, self.core, self.I)
+        return float(band_transmission(t)), self.width_gradient(p, K, grad_eps.reshape(km - ks - 4, self.core, self.I),
+                                                                ks + 4)
+
+    def width_gradient(self, p, K, grad_eps, k_lo):
+        """dT / dw_i (per um) from dT / d eps over the core rows of planes k_lo.. (grad_eps
+        [planes][core rows][I]): each edge cell's eps grows by (eps_core - eps_clad) dw / (2 dx)
+        under the fill-fraction raster, and the spline basis carries dT / dw(k) to the control
+        points."""
         wk = self.widths_cells(p, K)
         dT_dw = np.zeros(K)
         deps = (self.eps_core - self.eps_clad) / (2 * self.disc)
-        for q, k in enumerate(range(ks + 4, km)):
+        for q in range(grad_eps.shape[0]):
+            k = k_lo + q
             ir = int(np.floor(self.I / 2 + wk[k] / 2))  # the cells that hold the core edges
             il = int(np.floor(self.I / 2 - wk[k] / 2))
             if 0 <= il < ir < self.I:
                 dT_dw[k] = deps * (grad_eps[q, :, ir].sum() + grad_eps[q, :, il].sum())
-        return float(band_transmission(t)), self.spline_basis(K) @ dT_dw
+        return self.spline_basis(K) @ dT_dw
 
     def simulate(self, designs, batch=64):
         """Complex transmission t [n][nfreq] of each design, by batched FDTD on the GPU."""
@@ -180,6 +185,15 @@ class TaperFamily:
             for g in grids:
                 g.free()
         return np.array(out)
+
+
+def adjoint_scale(amp, w, w0, dt):
+    """2u / (dt E0(w)): the factor that turns sum_m F_m A_m of a forward run (twiddle offset 0)
+    and an adjoint run (offset 1, the forward envelope on the monitor's adjoint source) into
+    d a_out / d eps, with u = e^{-i w dt} - 1 and E0(w) = sum_n A_n e^{i (w - w0) n dt}."""
+    n = np.arange(len(amp))
+    E0 = np.sum(np.asarray(amp) * np.exp(1j * (w - w0) * n * dt))
+    return 2.0 * (np.exp(-1j * w * dt) - 1.0) / (dt * E0)
 
 
 def band_transmission(t):

@@ -257,6 +257,152 @@ def bench_design(fd, args):
     print(json.dumps(out), flush=True)
 
 
+def port_mode(fd, fam, p, k, w0, margin=12):
+    """TE0 of the cross-section at plane k of design p (the mode solver on a window inside the
+    PML), as the power-normalised interleaved array the run API takes, and its window size."""
+    ep, K = fam.permittivity(p)
+    sec = ep.reshape(K, fam.J, fam.I, 3)[k].astype(np.complex128)
+    mi, mj = fam.I - 2 * margin, fam.J - 2 * margin
+    i0, j0 = fam.I // 2 - mi // 2, fam.J // 2 - mj // 2
+    beta, md = fd.solve_mode(sec[j0:j0 + mj, i0:i0 + mi], fam.disc, w0, tol=1e-9, size=20, confinement=2)
+    arr = fd.normalised_mode(mi, mj, fam.disc, fd.mode_fields(md).copy())
+    fd.lib().mode_free(md)
+    return arr, mi, mj, beta / w0
+
+
+def bench_gradient(fd, args):
+    """The 3-D taper with its DFT-domain permittivity gradient: a linear Si taper 0.5 -> 3.0 um
+    over 10 um between strip ports (8 spline control widths), 620 x 200 x 100 cells of 20 nm
+    (library axes 200 x 100 x 620), 10-cell CPML, TE0 of each port from the mode solver.  One
+    forward run (6000 steps, FP64 DFT box over the taper's core rows at 1.55 um) and one adjoint
+    run of the output monitor (6000 steps) give d|t|^2/d eps over the box and, through the edge
+    cells, d|t|^2/d w_i; a central difference of two more forward runs checks one width.
+    value = cell-updates/s over the two gradient runs (device time); roofline = the split kernels
+    in a separate 200-step window with every launch timed."""
+    design = importlib.import_module("3d_fdtd_amd.design")
+    steps = args.steps if args.steps != N_RUN else 6000
+    fam = design.TaperFamily(disc=DISC, I=200, J=100, core_cells=11, npml=10, port=48, steps=steps,
+                             lam=(1.55, 1.55), nfreq=1)
+    p = np.concatenate([np.linspace(0.5, 3.0, 8), [10.0]])
+    ep, K = fam.permittivity(p)
+    I, J = fam.I, fam.J
+    dt, w0 = fam.dt, 2 * math.pi / 1.55
+    ci, cj, ks, km = I // 2, J // 2, fam.npml + 1, K - fam.npml - 2
+    print("bench: solving the port modes", file=sys.stderr, flush=True)
+    m_in, mi_in, mj_in, n_in = port_mode(fd, fam, p, ks, w0)
+    m_out, mi_out, mj_out, n_out = port_mode(fd, fam, p, km, w0)
+    j0 = J // 2 - fam.core // 2
+    klo, khi = fam.npml + fam.port + 2, K - fam.npml - fam.port - 2
+    box = (0, I, j0, j0 + fam.core, klo, khi)
+    nb = I * fam.core * (khi - klo)
+    om = np.array([w0])
+
+    def forward(pp, with_box):
+        e, _ = fam.permittivity(pp)
+        g = fd.DevGrid(I, J, K, fam.disc, e)
+        g.cpml((fam.npml,) * 6, 3, 1e-8, 1.0, 0.05)
+        g.add_source(ci, cj, ks, m_in, mi_in, mj_in, fam.disc, fam.amp, fam.phase)
+        g.mon_in = g.add_overlap(ci, cj, ks + 3, m_in, mi_in, mj_in, fam.disc, om)
+        g.mon_out = g.add_overlap(ci, cj, km, m_out, mi_out, mj_out, fam.disc, om)
+        g.box = g.add_dft_box(box, 7, om) if with_box else None
+        return g
+
+    def transmission(g):
+        a_in, _ = g.overlap(g.mon_in, 1)
+        a_out, _ = g.overlap(g.mon_out, 1)
+        return a_out[0] / a_in[0], a_in[0]
+
+    print(f"bench: forward run ({steps} steps, {I * J * K / 1e6:.1f} M cells)", file=sys.stderr, flush=True)
+    g = forward(p, True)
+    g.timer_start()
+    g.run(dt, 0, steps)
+    t_fwd = g.timer_stop() / 1e3
+    t, a_in = transmission(g)
+    print("bench: adjoint run", file=sys.stderr, flush=True)
+    ga = fd.DevGrid(I, J, K, fam.disc, ep)
+    ga.cpml((fam.npml,) * 6, 3, 1e-8, 1.0, 0.05)
+    ga.add_adjoint_source(ci, cj, km, m_out, mi_out, mj_out, fam.disc, w0 * dt, fam.amp, fam.phase)
+    aid = ga.add_dft_box(box, 7, om)
+    ga.timer_start()
+    ga.run(dt, 0, steps, 1.0)
+    t_adj = ga.timer_stop() / 1e3
+    t0 = time.perf_counter()
+    weight = 2.0 * np.conj(t) * design.adjoint_scale(fam.amp, w0, w0, dt) / a_in
+    grad_eps = g.gradient(g.box, ga, aid, 0, weight, nb)
+    t_grad = time.perf_counter() - t0
+    g.free()
+    ga.free()
+    dT_dw = fam.width_gradient(p, K, grad_eps.reshape(khi - klo, fam.core, I), klo)
+    print("bench: finite-difference check of w_4", file=sys.stderr, flush=True)
+    d = 0.01
+    T_pm = []
+    for sgn in (1, -1):
+        pp = p.copy()
+        pp[3] += sgn * d
+        gf = forward(pp, False)
+        gf.run(dt, 0, steps)
+        T_pm.append(abs(transmission(gf)[0]) ** 2)
+        gf.free()
+    fd_w4 = (T_pm[0] - T_pm[1]) / (2 * d)
+    print("bench: kernel-timing window", file=sys.stderr, flush=True)
+    gk = fd.DevGrid(I, J, K, fam.disc, ep)
+    gk.cpml((fam.npml,) * 6, 3, 1e-8, 1.0, 0.05)
+    gk.add_source(ci, cj, ks, m_in, mi_in, mj_in, fam.disc, fam.amp, fam.phase)
+    gk.run(dt, 0, 800)
+    gk.kernel_timing(1)
+    gk.run(dt, 800, 200)
+    (e_us, h_us), _ = gk.kernel_times()
+    gk.kernel_timing(0)
+    gk.free()
+    n = I * J * K
+    dom = max(e_us, h_us)
+    achieved = SPLIT_KERNEL_BYTES * n / (dom * 1e-6) / 1e9
+    out = {"metric": METRIC, "value": 2 * steps * n / (t_fwd + t_adj), "unit": "cell-updates/s", "n_gpus": 1,
+           "steps": 2 * steps, "warmup": 0, "ms_per_step": (t_fwd + t_adj) * 1e3 / (2 * steps),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "linear Si taper 0.5 -> 3.0 um over 10 um between strip ports, 8 spline control "
+                                  "widths, 620x200x100 cells of 20 nm, 10-cell CPML, TE0 port modes from the mode "
+                                  "solver; forward + adjoint runs with an FP64 DFT box over the taper's core rows at "
+                                  "1.55 um and the DFT-domain eps-gradient",
+                      "grid_lib_axes": [I, J, K], "steps_per_run": steps, "dft_box_cells": nb},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "yee_update_E" if e_us >= h_us else "yee_update_H",
+                        "kernel_avg_us": round(dom, 2), "e_us": round(e_us, 2), "h_us": round(h_us, 2),
+                        "algorithmic_bytes_per_launch": SPLIT_KERNEL_BYTES * n,
+                        "step_frac_48B": round(STEP_BYTES * n / ((e_us + h_us) * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
+           "gradient": {"forward_s": round(t_fwd, 3), "adjoint_s": round(t_adj, 3), "gradient_s": round(t_grad, 4),
+                        "T": abs(t) ** 2, "n_eff_in": round(float(n_in.real), 4), "n_eff_out": round(float(n_out.real), 4),
+                        "dT_dw_per_um": [round(float(v), 6) for v in dT_dw],
+                        "fd_check_w4": {"adjoint": round(float(dT_dw[3]), 6), "central_difference": round(float(fd_w4), 6),
+                                        "delta_um": d}}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = gradient_cpu_baseline(fam, ep, K)
+    print(json.dumps(out), flush=True)
+
+
+def gradient_cpu_baseline(fam, ep, K, target_s=10.0):
+    """FP64 oracle CPML steps on the same grid (host threads), about target_s of them."""
+    from oracle import oracle_py as O
+
+    threads = int(os.environ.get("FDTD3D_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    g = O.Grid(fam.I, fam.J, K, fam.disc)
+    g.ep[:] = ep
+    cp = O.Cpml(g, (fam.npml,) * 6, fam.dt, order=3, R0=1e-8, kappa_max=1.0, alpha_max=0.05)
+    g.f[:] = np.random.default_rng(0).uniform(-1, 1, g.f.size)
+    t0 = time.perf_counter()
+    O.step_cpml(g, cp, fam.dt)
+    steps = max(2, min(1000, int(target_s / max(time.perf_counter() - t0, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.step_cpml(g, cp, fam.dt)
+    el = time.perf_counter() - t0
+    return {"value": fam.I * fam.J * K * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} FP64 oracle CPML steps of the {fam.I}x{fam.J}x{K} grid, {el:.1f} s, "
+                      f"OMP_NUM_THREADS={threads}"}
+
+
 def bench_surrogate(fd, args):
     """The transmission surrogate's hand-written BF16 matrix-core kernels: full-batch Adam steps on
     4096 samples and screening of 10^5 candidates (the design loop's shapes: 9 -> 256 x 4 -> 32,
@@ -413,10 +559,12 @@ def main():
     ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
     ap.add_argument("--kchunk", type=int, default=None,
                     help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
-    ap.add_argument("--workload", choices=("waveguide", "tapers", "design", "surrogate"), default="waveguide",
+    ap.add_argument("--workload", choices=("waveguide", "tapers", "design", "surrogate", "gradient"),
+                    default="waveguide",
                     help="waveguide: the strip waveguide (default); tapers: the batched taper set; design: "
                          "verified taper designs/hour, surrogate loop vs plain adjoint loop; surrogate: the MLP's "
-                         "matrix-core training and screening kernels")
+                         "matrix-core training and screening kernels; gradient: the 620x200x100 taper's forward + "
+                         "adjoint runs and DFT eps-gradient")
     ap.add_argument("--train", type=int, default=4096, help="design workload: FDTD training designs")
     ap.add_argument("--fit-steps", type=int, default=1500, help="design workload: surrogate training steps")
     ap.add_argument("--campaign", type=float, default=150.0,
@@ -445,6 +593,9 @@ def main():
         return
     if args.workload == "surrogate":
         bench_surrogate(fd, args)
+        return
+    if args.workload == "gradient":
+        bench_gradient(fd, args)
         return
     W, S = args.warmup, args.steps
     g, dt, md, mons, omegas = workload(fd, W + S + min(400, max(S // 4, 20)))
