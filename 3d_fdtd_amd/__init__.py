@@ -100,6 +100,9 @@ SIGNATURES = {
     "timefield3d_TM_gaussian_source": ([TimeField, _i, _i, _i, _d, _d, _d, _d], None),
     "timefield3d_mode_overlap": ([TimeField, _i, _i, _i, Mode, _d, _i, _P(Complex)], Complex),
     "timefield3d_file_write_permittivity_y_slice": ([TimeField, C.c_char_p, _i, _i, _i, _i, _i, _d, _d], None),
+    **{f"timefield3d_plot_{q}_{a}_slice{suffix}": ([TimeField, _i], None)
+       for q, suffix in (("Ex", ""), ("Ey", ""), ("Ez", ""), ("Ex", "_fourier"), ("permittivity", ""))
+       for a in "xyz"},
     # mode.h
     "mode_new": ([_i, _i, _d], Mode),
     "mode_free": ([Mode], None),

@@ -197,24 +197,43 @@ static FILE *plot_open(const char *name, int slice) {
     return fopen(path, "w");
 }
 
-/* axis: 0 = x slice (plane i = s, rows k, cols j), 1 = y slice (rows k, cols i), 2 = z slice
- * (rows j, cols i).  what: 0..5 field component, 6 = |fourier Ex|, 7 = ep_x. */
+/* One slice as the gnuplot "nonuniform matrix" data file the reference's plot functions write
+ * before piping it to gnuplot (timefield3d.c:348-1354): a header row with the column count and
+ * the column coordinates, then one row per row coordinate followed by its values, every number
+ * as "%lf\t".  Coordinates are cell centres relative to the grid centre, (n + 0.5 - N/2) disc.
+ * x slice (plane i = s): rows j, columns k; y slice (plane j = s): rows i, columns k; z slice
+ * (plane k = s): rows j, columns i.  what: 0..5 field component, 6 = |fourier Ex|, 7 = ep_x.
+ * The header's first number is the true column count (the reference writes psi.k for every
+ * slice, which misreads z slices whenever psi.i != psi.k). */
 static void plot_slice(timefield3d psi, int axis, int s, int what, const char *name) {
+    const int n_axis = axis == 0 ? psi.i : (axis == 1 ? psi.j : psi.k);
+    if (s < 0 || s >= n_axis) {
+        fprintf(stderr, "fdtd3d: %s slice %d outside 0..%d\n", name, s, n_axis - 1);
+        return;
+    }
     FILE *f = plot_open(name, s);
     if (!f) return;
     if (what < 6) pull_fields(psi);
     if (what == 6) dev_check(fdtd_dev_get_fourier(tf_dev(psi), (double *)psi.fourier_fields), "fourier download");
-    int nr = axis == 2 ? psi.j : psi.k;
-    int nc = axis == 0 ? psi.j : psi.i;
-    for (int r = 0; r < nr; r++) {
-        for (int col = 0; col < nc; col++) {
-            int x = axis == 0 ? s : col, y = axis == 0 ? col : (axis == 1 ? s : r), z = axis == 2 ? s : r;
-            long long q = x + (long long)psi.i * (y + (long long)psi.j * z);
+    const int N[3] = {psi.i, psi.j, psi.k};
+    const int ra = axis == 1 ? 0 : 1;  // row axis
+    const int ca = axis == 2 ? 0 : 2;  // column axis
+    fprintf(f, "%lf\t", (double)N[ca]);
+    for (int c = 0; c < N[ca]; c++) fprintf(f, "%lf\t", (c + 0.5 - N[ca] / 2.0) * psi.disc);
+    fprintf(f, "\n");
+    for (int r = 0; r < N[ra]; r++) {
+        fprintf(f, "%lf\t", (r + 0.5 - N[ra] / 2.0) * psi.disc);
+        for (int c = 0; c < N[ca]; c++) {
+            int x[3];
+            x[axis] = s;
+            x[ra] = r;
+            x[ca] = c;
+            long long q = x[0] + (long long)psi.i * (x[1] + (long long)psi.j * x[2]);
             double v;
             if (what < 6) v = psi.fields[6 * q + what];
             else if (what == 6) v = complex_mag(psi.fourier_fields[6 * q]);
             else v = psi.ep[3 * q];
-            fprintf(f, "%g\t", v);
+            fprintf(f, "%lf\t", v);
         }
         fprintf(f, "\n");
     }

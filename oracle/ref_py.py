@@ -82,6 +82,7 @@ def lib():
         "timefield3d_set_taper": ([TimeField, d, d, i, i, i, i, i, i, i, i], None),
         "timefield3d_set_waveguide": ([TimeField, f, i, i, i, i, i, i], None),
         "timefield3d_set_prism_permittivity": ([TimeField, f, i, i, i, i, i, i], None),
+        "timefield3d_file_write_permittivity_y_slice": ([TimeField, C.c_char_p, i, i, i, i, i, d, d], None),
         "mode_new": ([i, i, d], Mode),
         "mode_free": ([Mode], None),
         "mode_gaussian": ([i, i, d, Complex, Complex, d, d, i], Mode),
