@@ -443,13 +443,17 @@ int fdtd_dev_set_fourier(fdtd_dev *d, const double *in) {
 }
 
 int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk) {
-    if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16 && rows != 32) {
-        set_error("step kernel rows must be 0 (split kernels), 1, 2, 4, 8 (fused register strips) or 16, 32 "
-                  "(fused LDS tiles of 8 / 16 rows)");
+    if (rows != -1 && rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16 && rows != 32) {
+        set_error("step kernel rows must be -1 (fused columns), 0 (split kernels), 1, 2, 4, 8 (fused register "
+                  "strips) or 16, 32 (fused LDS tiles of 8 / 16 rows)");
         return -2;
     }
     if (rows != 0 && kchunk < 1) {
         set_error("fused step z chunk must be positive");
+        return -2;
+    }
+    if (rows == -1 && kchunk != 1 && kchunk != 2 && kchunk != 4) {
+        set_error("fused column step: z planes per thread must be 1, 2 or 4");
         return -2;
     }
     d->fused_rows = rows;

@@ -573,6 +573,37 @@ int fused_sync_alt(fdtd_dev *d) {
     return 0;
 }
 
+// Second field buffer in sync and second PML memory allocated (zeroed once).
+int fused_prepare(fdtd_dev *d) {
+    int rc = fused_sync_alt(d);
+    if (rc) return rc;
+    Cpml &cp = d->cpml;
+    const long long pmem = 4 * (cp.psi_n[0] + cp.psi_n[1] + cp.psi_n[2]);
+    if (cp.on && cp.psi_base && !cp.psi_alt) {
+        FDTD_TRY(dev_malloc(&cp.psi_alt, (size_t)pmem * sizeof(float)));
+        FDTD_TRY(hipMemsetAsync(cp.psi_alt, 0, (size_t)pmem * sizeof(float), d->stream));
+    }
+    return 0;
+}
+
+// After a ping-pong step: the second buffers become the current ones.
+void fused_swap(fdtd_dev *d) {
+    float *t = d->field_base;
+    d->field_base = d->alt_base;
+    d->alt_base = t;
+    for (int m = 0; m < 6; m++) d->F[m] = d->field_base + m * d->Nal;
+    Cpml &cp = d->cpml;
+    if (cp.on && cp.psi_base && cp.psi_alt) {
+        float *old = cp.psi_base;
+        cp.psi_base = cp.psi_alt;
+        cp.psi_alt = old;
+        for (int f = 0; f < 2; f++)
+            for (int ax = 0; ax < 3; ax++)
+                for (int q = 0; q < 2; q++)
+                    if (cp.psi[f][ax][q]) cp.psi[f][ax][q] = cp.psi_base + (cp.psi[f][ax][q] - old);
+    }
+}
+
 bool fused_supported(const fdtd_dev *d) {
     if (!d->cpml.on || d->dense_src || d->src.size() > 2) return false;
     const Cpml &cp = d->cpml;
@@ -584,13 +615,8 @@ int fused_step(fdtd_dev *d, double dt, int ty_rows, int kc_len) {
     int rc = ensure_coefficients(d, dt);
     if (rc) return rc;
     if (d->cpml.on && (rc = cpml_tables(d, dt))) return rc;
-    if ((rc = fused_sync_alt(d))) return rc;
+    if ((rc = fused_prepare(d))) return rc;
     Cpml &cp = d->cpml;
-    const long long pmem = 4 * (cp.psi_n[0] + cp.psi_n[1] + cp.psi_n[2]);
-    if (cp.on && cp.psi_base && !cp.psi_alt) {
-        FDTD_TRY(dev_malloc(&cp.psi_alt, (size_t)pmem * sizeof(float)));
-        FDTD_TRY(hipMemsetAsync(cp.psi_alt, 0, (size_t)pmem * sizeof(float), d->stream));
-    }
     FusedArgs a{};
     a.src = d->field_base;
     a.dst = d->alt_base;
@@ -671,20 +697,7 @@ int fused_step(fdtd_dev *d, double dt, int ty_rows, int kc_len) {
 #undef FDTD_TILED
 #undef FDTD_TILED_ONE
     FDTD_TRY(hipGetLastError());
-    // swap the ping-pong buffers
-    float *t = d->field_base;
-    d->field_base = d->alt_base;
-    d->alt_base = t;
-    for (int m = 0; m < 6; m++) d->F[m] = d->field_base + m * d->Nal;
-    if (pml && cp.psi_base) {
-        float *old = cp.psi_base;
-        cp.psi_base = cp.psi_alt;
-        cp.psi_alt = old;
-        for (int f = 0; f < 2; f++)
-            for (int ax = 0; ax < 3; ax++)
-                for (int q = 0; q < 2; q++)
-                    if (cp.psi[f][ax][q]) cp.psi[f][ax][q] = cp.psi_base + (cp.psi[f][ax][q] - old);
-    }
+    fused_swap(d);
     return 0;
 }
 

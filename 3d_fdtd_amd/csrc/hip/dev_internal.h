@@ -214,6 +214,13 @@ bool fused_supported(const fdtd_dev *d);
 // True when fdtd_dev_step runs the split E/H kernels (the path that can fold run monitors).
 bool step_uses_split(const fdtd_dev *d);
 int fused_sync_alt(fdtd_dev *d);
+int fused_prepare(fdtd_dev *d);
+void fused_swap(fdtd_dev *d);
+// Fused one-kernel step in columns (dev_colstep.hip): kc z planes per thread (1, 2 or 4).
+int col_step(fdtd_dev *d, double dt, int kc);
+bool col_supported(const fdtd_dev *d);
+// The step kernel evaluates run overlap monitors itself (split H update or column step).
+bool step_folds_monitors(const fdtd_dev *d);
 // The device source plane for z = k (created zeroed on first use), or null with *err set.
 SrcPlane *plane_for(fdtd_dev *d, int k, int *err);
 void run_release(fdtd_dev *d);

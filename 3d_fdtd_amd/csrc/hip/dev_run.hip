@@ -588,7 +588,7 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     const double scale = -0.5 * d->disc * d->disc;
     const int ninl = (int)r->src.size() < kMaxRunSrc ? (int)r->src.size() : kMaxRunSrc;
     // overlap monitors accumulate inside the H update when every one of them can
-    bool fold = nmon > 0 && nmon <= kMaxRunMon && step_uses_split(d);
+    bool fold = nmon > 0 && nmon <= kMaxRunMon && step_folds_monitors(d);
     for (auto &o : r->ovl) fold = fold && o.ring_free;
     if (fold && n > r->mstep_cap) {
         FDTD_TRY(hipStreamSynchronize(d->stream));

@@ -331,15 +331,18 @@ __device__ __forceinline__ double2 block_sum2(double re, double im) {
 // Overlap monitors of a MON launch, after the block's z march: for each monitor plane inside the
 // chunk, the block's FP64 partial of Hx mEy - Hy mEx + Ey mHx - Ex mHy (timefield3d.c:3195-3224)
 // over its cells, from E (final since the E update) and H (just stored by these same threads).
+// Ex, Ey, Hx, Hy: the arrays holding the step's final fields (the launch's own, or the second
+// buffer of a ping-pong step).
 template <typename Off>
-__device__ __forceinline__ void monitor_epilogue(const YeeArgs &a, int kb, int ke, int pb, Off p, bool active) {
+__device__ __forceinline__ void monitor_epilogue_on(const YeeArgs &a, const float *Ex, const float *Ey, const float *Hx,
+                                                    const float *Hy, int kb, int ke, int pb, Off p, bool active) {
     for (int q = 0; q < a.nmon; q++) {
         const RunMonArg &M = a.mon[q];
         if (M.k < kb || M.k >= ke) continue;  // uniform over the block
         double re = 0.0, im = 0.0;
         if (active) {
             const Off c = p + (Off)M.k * (Off)a.P;
-            const double ex = ld(a.Ex, c), ey = ld(a.Ey, c), hx = ld(a.Hx, c), hy = ld(a.Hy, c);
+            const double ex = ld(Ex, c), ey = ld(Ey, c), hx = ld(Hx, c), hy = ld(Hy, c);
             double w[8];
             for (int r = 0; r < 8; r++) w[r] = ld(M.W + (long long)r * a.P, p);
             re = (hx * w[2] - hy * w[0]) + (ey * w[4] - ex * w[6]);
@@ -349,6 +352,11 @@ __device__ __forceinline__ void monitor_epilogue(const YeeArgs &a, int kb, int k
         double2 *part = a.step ? M.part + (long long)(*a.step - a.step0) * M.stride : M.part;
         if (threadIdx.x == 0) part[pb] = v;
     }
+}
+
+template <typename Off>
+__device__ __forceinline__ void monitor_epilogue(const YeeArgs &a, int kb, int ke, int pb, Off p, bool active) {
+    monitor_epilogue_on(a, a.Ex, a.Ey, a.Hx, a.Hy, kb, ke, pb, p, active);
 }
 
 // H <- caH*H - cbH*(forward-difference curl of E) - cjH*M   (timefield3d.c:1679-1757)

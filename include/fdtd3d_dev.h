@@ -58,10 +58,12 @@ int fdtd_dev_cpml_off(fdtd_dev *d);
 int fdtd_dev_update_E_cpml(fdtd_dev *d, double dt);
 int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt);
 /* Kernel used by fdtd_dev_step under the PML boundary: rows = 0 selects the split E / H pair
- * (kchunk > 0 fixes their z planes per thread, 0 lets each launch choose), rows = 1, 2, 4 or 8
- * the fused single-kernel step in register strips with that many y rows per lane and kchunk z
- * planes per wavefront, rows = 16 or 32 the fused step in LDS tiles of 8 or 16 rows and kchunk
- * z planes per block.  Default: the split pair with automatic chunking. */
+ * (kchunk > 0 fixes their z planes per thread, 0 lets each launch choose), rows = -1 the fused
+ * single-kernel step in columns (one thread per (i, j) column, kchunk = 1, 2 or 4 z planes, the
+ * neighbours' new E recomputed; evaluates run monitors itself), rows = 1, 2, 4 or 8 the fused
+ * step in register strips with that many y rows per lane and kchunk z planes per wavefront,
+ * rows = 16 or 32 the fused step in LDS tiles of 8 or 16 rows and kchunk z planes per block.
+ * Default: the split pair with automatic chunking. */
 int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk);
 
 /* Device-resident run (extension).  A run holds mode sources whose per-step amplitude and

@@ -31,9 +31,10 @@ int fdtd3d_ext_synchronize(timefield3d psi);
 int fdtd3d_ext_kernel_timing(timefield3d psi, int every);
 int fdtd3d_ext_kernel_times(timefield3d psi, int counts[3], double ms[3]);
 /* Under the PML boundary: rows = 0 selects the split E/H kernels (kchunk > 0: z planes per
- * thread, 0: automatic), rows = 1, 2, 4 or 8 the fused one-kernel step in register strips (rows
- * per lane, kchunk z planes per wavefront), rows = 16 or 32 the fused step in LDS tiles of 8 or 16
- * rows (kchunk z planes per block).  Default: split kernels, automatic chunking. */
+ * thread, 0: automatic), rows = -1 the fused one-kernel step in columns (kchunk = 1, 2 or 4 z
+ * planes per thread), rows = 1, 2, 4 or 8 the fused step in register strips (rows per lane,
+ * kchunk z planes per wavefront), rows = 16 or 32 the fused step in LDS tiles of 8 or 16 rows
+ * (kchunk z planes per block).  Default: split kernels, automatic chunking. */
 int fdtd3d_ext_set_step_kernel(timefield3d psi, int rows, int kchunk);
 
 /* Run n reference steps (timefield3d_apply_timestep_omp_full semantics) without host work in

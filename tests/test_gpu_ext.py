@@ -66,12 +66,14 @@ def test_cpml_step_vs_oracle(npml):
     g.free()
 
 
-@pytest.mark.parametrize("rows,kchunk", [(2, 3), (4, 16), (8, 5), (16, 3), (16, 7), (32, 4)])
+@pytest.mark.parametrize("rows,kchunk", [(-1, 1), (-1, 2), (-1, 4), (2, 3), (4, 16), (8, 5), (16, 3), (16, 7),
+                                         (32, 4)])
 def test_fused_step_vs_split_and_oracle(rows, kchunk):
-    """Fused one-kernel step (ping-pong buffers) vs the split E/H kernels and the FP64 oracle, on
-    an odd-sized grid (partial strips, tiles and chunks) with conductivity, a plane source and
-    PML on some faces; switches kernels mid-run to exercise the buffer resynchronisation."""
-    rng = np.random.default_rng(rows * 100 + kchunk)
+    """Fused one-kernel step (ping-pong buffers: columns, register strips or LDS tiles) vs the
+    split E/H kernels and the FP64 oracle, on an odd-sized grid (partial blocks, strips, tiles and
+    chunks) with conductivity, a plane source and PML on some faces; switches kernels mid-run to
+    exercise the buffer resynchronisation."""
+    rng = np.random.default_rng((rows + 2) * 100 + kchunk)
     I, J, K, disc = 70, 23, 29, 0.02
     dt = 0.99 * disc / np.sqrt(3)
     npml = (5, 4, 0, 3, 6, 0)
@@ -205,11 +207,11 @@ def test_run_monitor_fold_matches_separate_kernels(golden_dir):
     assert err <= 1e-10
 
 
-@pytest.mark.parametrize("rows", [8, 16])
+@pytest.mark.parametrize("rows", [-1, 8, 16])
 def test_run_api_fused_matches_split(golden_dir, rows):
     """Device-resident run under the PML with a fused step kernel (run sources evaluated inside
-    the fused kernel, monitors by the separate kernels) vs the split kernels (monitors inside the
-    H update): overlaps agree to relative 1e-5."""
+    the fused kernel; monitors inside the column kernel, by the separate kernels for strips and
+    tiles) vs the split kernels (monitors inside the H update): overlaps agree to relative 1e-5."""
     z = np.load(os.path.join(golden_dir, "F4_mode_source_overlap.npz"))
     I, J, K = (int(v) for v in z["dims"])
     dt, om = float(z["dt"]), float(z["omega"])
@@ -227,7 +229,7 @@ def test_run_api_fused_matches_split(golden_dir, rows):
         g.push()
         npml = (C.c_int * 6)(6, 6, 6, 6, 6, 6)
         fd.check(L().fdtd3d_ext_set_cpml(g.tf, npml, 3, 1e-6, 1.0, 0.05), "cpml")
-        fd.check(L().fdtd3d_ext_set_step_kernel(g.tf, kernel, 0 if kernel == 0 else 5), "kernel")
+        fd.check(L().fdtd3d_ext_set_step_kernel(g.tf, kernel, {0: 0, -1: 2}.get(kernel, 5)), "kernel")
         m1 = fd.mode_from_array(30, 30, float(z["disc"]), z["mode1"])
         assert L().fdtd3d_ext_run_add_mode_source(g.tf, I // 2, J // 2, 10, m1, 0, 0, n, amp.ctypes.data_as(DP),
                                                   phase.ctypes.data_as(DP)) >= 0
