@@ -176,6 +176,7 @@ SIGNATURES = {
     "fdtd_dev_destroy": ([C.c_void_p], None),
     "fdtd_dev_set_materials": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float)], _i),
     "fdtd_dev_get_fields": ([C.c_void_p, _P(_d)], _i),
+    "fdtd_dev_set_fields": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_cpml_on": ([C.c_void_p, _P(_i), _i, _d, _d, _d], _i),
     "fdtd_dev_set_step_kernel": ([C.c_void_p, _i, _i], _i),
     "fdtd_dev_run_add_source": ([C.c_void_p, _i, _i, _i, _P(_d), _i, _i, _d, _i, _i, _i, _P(_d), _P(_d)], _i),
@@ -451,6 +452,13 @@ class DevGrid:
         out = np.zeros(6 * self.I * self.J * self.K)
         dev_check(lib().fdtd_dev_get_fields(self.d, out.ctypes.data_as(C.POINTER(C.c_double))), "get_fields")
         return out
+
+    def set_fields(self, f):
+        """Fields [k][j][i][6] (Ex, Ey, Ez, Hx, Hy, Hz), stored as FP32 on the device."""
+        f = np.ascontiguousarray(f, np.float64)
+        if f.size != 6 * self.I * self.J * self.K:
+            raise ValueError(f"{f.size} field values for {6 * self.I * self.J * self.K}")
+        dev_check(lib().fdtd_dev_set_fields(self.d, f.ctypes.data_as(C.POINTER(C.c_double))), "set_fields")
 
     def free(self):
         if self.d:
