@@ -158,6 +158,8 @@ SIGNATURES = {
     "fdtd3d_ext_kernel_timing": ([TimeField, _i], _i),
     "fdtd3d_ext_kernel_times": ([TimeField, _P(_i), _P(_d)], _i),
     "fdtd3d_ext_set_step_kernel": ([TimeField, _i, _i], _i),
+    "fdtd3d_ext_count_nonfinite": ([TimeField, _P(C.c_longlong)], _i),
+    "fdtd3d_ext_set_finite_check": ([TimeField, _i], _i),
     "fdtd3d_ext_set_cpml": ([TimeField, _P(_i), _i, _d, _d, _d], _i),
     "fdtd3d_ext_set_mur": ([TimeField], _i),
     "fdtd3d_ext_run_clear": ([TimeField], _i),
@@ -177,6 +179,8 @@ SIGNATURES = {
     "fdtd_dev_set_materials": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float)], _i),
     "fdtd_dev_get_fields": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_set_fields": ([C.c_void_p, _P(_d)], _i),
+    "fdtd_dev_count_nonfinite": ([C.c_void_p, _P(C.c_longlong)], _i),
+    "fdtd_dev_set_finite_check": ([C.c_void_p, _i], _i),
     "fdtd_dev_cpml_on": ([C.c_void_p, _P(_i), _i, _d, _d, _d], _i),
     "fdtd_dev_set_step_kernel": ([C.c_void_p, _i, _i], _i),
     "fdtd_dev_run_add_source": ([C.c_void_p, _i, _i, _i, _P(_d), _i, _i, _d, _i, _i, _i, _P(_d), _P(_d)], _i),

@@ -192,6 +192,8 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->fused_rows = 0;  // split E/H kernels: measured faster than the fused step on MI355X
     d->fused_kchunk = 8;
     d->split_kchunk = 0;
+    d->finite_every = 0;
+    d->finite_buf = nullptr;
     hipError_t e = dev_stream_acquire(&d->stream);
     // fields: 6 arrays, each N floats, each starting on a 256-byte boundary
     d->Nal = (d->N + 63) / 64 * 64;
@@ -232,6 +234,7 @@ void fdtd_dev_destroy(fdtd_dev *d) {
     fdtd_dev_cpml_off(d);
     run_release(d);
     (void)dev_free(d->fourier);
+    (void)dev_free(d->finite_buf);
     (void)dev_free(d->partials);
     (void)dev_free(d->red_out);
     for (auto &p : d->src) (void)dev_free(p.J);

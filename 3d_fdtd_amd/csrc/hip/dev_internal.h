@@ -193,6 +193,12 @@ struct fdtd_dev {
     fdtd::RunMonArg run_mon[fdtd::kMaxRunMon];
     fdtd::KernelTiming ktime;
 
+    // non-finite field watch of device-resident runs (dev_health.hip): every `finite_every` steps
+    // a scan adds the count of NaN / Inf field values and the first step that showed any to
+    // finite_buf (unsigned long long count, int first step)
+    int finite_every;
+    void *finite_buf;
+
     hipEvent_t ev0, ev1;
 };
 
@@ -218,6 +224,11 @@ int fused_prepare(fdtd_dev *d);
 void fused_swap(fdtd_dev *d);
 // Fused one-kernel step in columns (dev_colstep.hip): kc z planes per thread (1, 2 or 4).
 int col_step(fdtd_dev *d, double dt, int kc);
+// Non-finite watch of runs (dev_health.hip): reset before a run call, scan after step `step`,
+// and the verdict after the call (0, or -4 with the count and first step in the message).
+int finite_watch_reset(fdtd_dev *d);
+int finite_watch_scan(fdtd_dev *d, int step);
+int finite_watch_verdict(fdtd_dev *d);
 bool col_supported(const fdtd_dev *d);
 // The step kernel evaluates run overlap monitors itself (split H update or column step).
 bool step_folds_monitors(const fdtd_dev *d);

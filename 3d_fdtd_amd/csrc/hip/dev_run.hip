@@ -644,6 +644,7 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
                                (const double2 *)(r->tw + (size_t)s * nf + q.freq_off));
             FDTD_TRY(hipGetLastError());
         }
+        if (int frc = finite_watch_scan(d, step)) return frc;
     }
     if (fold) {
         hipLaunchKernelGGL(k_run_mon_steps, dim3((n + kBlock - 1) / kBlock, nmon), dim3(kBlock), 0, d->stream,
@@ -796,11 +797,13 @@ namespace {
 extern "C" {
 
 int fdtd_dev_run(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
+    int rc = finite_watch_reset(d);
+    if (rc) return rc;
     for (int s = 0; s < n; s += kRunChunk) {
-        const int rc = run_chunk(d, dt, n0 + s, n - s < kRunChunk ? n - s : kRunChunk, t_offset);
+        rc = run_chunk(d, dt, n0 + s, n - s < kRunChunk ? n - s : kRunChunk, t_offset);
         if (rc) return rc;
     }
-    return 0;
+    return finite_watch_verdict(d);
 }
 
 int fdtd_dev_run_overlap_result(fdtd_dev *d, int id, double *cumulative, double *last_inst) {

@@ -56,6 +56,12 @@ int fdtd3d_ext_kernel_times(timefield3d psi, int counts[3], double ms[3]) {
     return fdtd_dev_kernel_times(tf_dev(psi), counts, ms);
 }
 
+int fdtd3d_ext_count_nonfinite(timefield3d psi, long long *count) {
+    return fdtd_dev_count_nonfinite(tf_dev(psi), count);
+}
+
+int fdtd3d_ext_set_finite_check(timefield3d psi, int every) { return fdtd_dev_set_finite_check(tf_dev(psi), every); }
+
 int fdtd3d_ext_set_step_kernel(timefield3d psi, int rows, int kchunk) {
     return fdtd_dev_set_step_kernel(tf_dev(psi), rows, kchunk);
 }

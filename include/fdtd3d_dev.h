@@ -154,6 +154,14 @@ void fdtd_dev_block_release(const void *key);
 int fdtd_dev_kernel_timing(fdtd_dev *d, int every);
 int fdtd_dev_kernel_times(fdtd_dev *d, int counts[3], double ms[3]);
 
+/* Non-finite field detection (no counterpart in the reference, whose loops run on through NaN
+ * and Inf).  count_nonfinite scans the six field arrays once (synchronous) and returns the number
+ * of NaN / Inf values.  With set_finite_check(d, every > 0), fdtd_dev_run scans the fields after
+ * every step whose index s has (s + 1) % every == 0 and, after finishing the call, returns -4
+ * with the count and the first step that showed them in fdtd_dev_last_error() (0 = off). */
+int fdtd_dev_count_nonfinite(fdtd_dev *d, long long *count);
+int fdtd_dev_set_finite_check(fdtd_dev *d, int every);
+
 /* Timing of the work enqueued on the grid's stream (HIP events on that stream). */
 int fdtd_dev_timer_start(fdtd_dev *d);
 int fdtd_dev_timer_stop(fdtd_dev *d, double *elapsed_ms);

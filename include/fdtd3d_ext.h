@@ -30,6 +30,11 @@ int fdtd3d_ext_synchronize(timefield3d psi);
  * of sampled launches and their summed durations in ms. */
 int fdtd3d_ext_kernel_timing(timefield3d psi, int every);
 int fdtd3d_ext_kernel_times(timefield3d psi, int counts[3], double ms[3]);
+/* Non-finite field detection (no counterpart in the reference): count the NaN / Inf field values
+ * now (synchronous), or let fdtd3d_ext_run scan every `every` steps and fail with -4 (0 = off);
+ * see fdtd_dev_count_nonfinite / fdtd_dev_set_finite_check. */
+int fdtd3d_ext_count_nonfinite(timefield3d psi, long long *count);
+int fdtd3d_ext_set_finite_check(timefield3d psi, int every);
 /* Under the PML boundary: rows = 0 selects the split E/H kernels (kchunk > 0: z planes per
  * thread, 0: automatic), rows = -1 the fused one-kernel step in columns (kchunk = 1, 2 or 4 z
  * planes per thread), rows = 1, 2, 4 or 8 the fused step in register strips (rows per lane,

@@ -331,3 +331,35 @@ def test_run_dft_plane_vs_oracle(golden_dir):
     say("run DFT plane rel L2", err)
     assert err <= REL
     g.free()
+
+
+def test_nonfinite_fields_are_detected():
+    """A NaN planted in the fields from the host (failure injection; every kernel still runs to
+    completion on NaN data) is counted by the one-shot scan and fails a watched run with -4,
+    naming the first scanned step that showed it; clean fields pass both."""
+    say("non-finite watch")
+    I, J, K, disc = 40, 30, 36, 0.02
+    dt = 0.99 * disc / np.sqrt(3)
+    ep = np.full(3 * I * J * K, 2.25, np.float32)
+    g = fd.DevGrid(I, J, K, disc, ep)
+    g.cpml((4,) * 6, 3, 1e-8, 1.0, 0.05)
+    f0 = np.random.default_rng(3).uniform(-1, 1, 6 * I * J * K)
+    g.set_fields(f0)
+    n = C.c_longlong(-1)
+    fd.dev_check(L().fdtd_dev_count_nonfinite(g.d, C.byref(n)), "count")
+    assert n.value == 0
+    fd.dev_check(L().fdtd_dev_set_finite_check(g.d, 10), "watch")
+    assert L().fdtd_dev_run(g.d, dt, 0, 30, 0.0) == 0
+    bad = g.fields()
+    c = 6 * ((K // 2 * J + J // 2) * I + I // 2) + 2  # Ez at the centre
+    bad[c] = np.nan
+    g.set_fields(bad)
+    fd.dev_check(L().fdtd_dev_count_nonfinite(g.d, C.byref(n)), "count")
+    assert n.value == 1
+    rc = L().fdtd_dev_run(g.d, dt, 30, 30, 0.0)
+    msg = L().fdtd_dev_last_error().decode()
+    say("watched run:", rc, msg)
+    assert rc == -4 and "after step 39" in msg
+    fd.dev_check(L().fdtd_dev_count_nonfinite(g.d, C.byref(n)), "count")
+    assert n.value > 100  # the NaN has spread through the stencil
+    g.free()
