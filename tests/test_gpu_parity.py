@@ -194,6 +194,49 @@ def test_dipole_mur_64(golden_dir):
     assert err <= REL_L2
 
 
+def test_dipole_cpml_64():
+    """The CPU-runnable configuration as specified, with its 8-cell CPML on all faces: 64^3
+    vacuum, 20 nm cells, z-polarised point dipole at the centre driven by the Gaussian-modulated
+    sine (lambda0 1.55 um, bandwidth 0.2 um), Courant 0.99/sqrt(3), 500 steps.  Full Ez at the
+    last step and the running DFT of Ez on the plane z = 48 at 5 wavelengths 1.50-1.60 um (the
+    drop-in per-frequency accumulator, one run per wavelength) against the FP64 oracle."""
+    N, disc, steps = 64, 0.02, 500
+    dt = 0.99 * disc / np.sqrt(3)
+    npml = (8,) * 6
+    oms = 2 * np.pi / np.linspace(1.50, 1.60, 5)
+    J = dipole_J(np.arange(steps), dt)
+    c0 = 32 + N * (32 + N * 32)
+    say("64^3 dipole under the CPML, 500 steps x 5 wavelengths")
+    dft_gpu, ez_gpu = [], None
+    for q, w in enumerate(oms):
+        g = fd.Grid(N, N, N, disc)
+        fd.check(L().fdtd3d_ext_set_cpml(g.tf, (C.c_int * 6)(*npml), 3, 1e-8, 1.0, 0.05), "cpml")
+        for n in range(steps):
+            g.a["sources"][6 * c0 + 2] = J[n]
+            fd.check(L().fdtd3d_ext_upload_source_plane(g.tf, 32), "source plane")
+            g.step(dt)
+            L().timefield3d_update_fourier_fields_zsection(g.tf, n * dt * float(w), 48, 49)
+        g.pull()
+        if q == 0:
+            ez_gpu = g.a["fields"].reshape(N, N, N, 6)[..., 2].copy()
+        f = g.a["fourier"].reshape(N, N, N, 6, 2)[48, :, :, 2, :]
+        dft_gpu.append(f[..., 0] + 1j * f[..., 1])
+        g.free()
+    o = O.Grid(N, N, N, disc)
+    cp = O.Cpml(o, npml, dt, order=3, R0=1e-8, kappa_max=1.0, alpha_max=0.05)
+    dft_ref = np.zeros((len(oms), N, N), complex)
+    for n in range(steps):
+        o.src[6 * c0 + 2] = J[n]
+        O.step_cpml(o, cp, dt)
+        dft_ref += np.exp(1j * oms * n * dt)[:, None, None] * o.comp(2)[48][None]
+    e_ez = rel_l2(ez_gpu, o.comp(2))
+    e_dft = rel_l2(np.stack(dft_gpu), dft_ref)
+    say("CPML dipole: Ez rel L2", e_ez, "DFT(z=48, 5 wavelengths) rel L2", e_dft)
+    assert np.abs(o.comp(2)).max() > 0 and np.abs(dft_ref).max() > 0
+    assert e_ez <= REL_L2
+    assert e_dft <= REL_L2
+
+
 def test_mode_source_overlap_adjoint(golden_dir):
     """F4: mode source + y-slice DFT + three overlap monitors, then the CW adjoint run."""
     z = load(golden_dir, "F4_mode_source_overlap.npz")
