@@ -205,6 +205,8 @@ SIGNATURES = {
     "fdtd_dev_batch_info": ([C.c_void_p, _P(_i), _P(C.c_longlong), _P(_i), _P(_i), _P(_d)], _i),
     "fdtd_dev_batch_timer_start": ([C.c_void_p], _i),
     "fdtd_dev_batch_timer_stop": ([C.c_void_p, _P(_d)], _i),
+    "fdtd_dev_batch_kernel_timing": ([C.c_void_p, _i], _i),
+    "fdtd_dev_batch_kernel_times": ([C.c_void_p, _P(_i), _P(_d)], _i),
     # transmission surrogate (include/fdtd3d_mlp.h)
     "fdtd_mlp_new": ([_i, _i, _i, _i], C.c_void_p),
     "fdtd_mlp_free": ([C.c_void_p], None),

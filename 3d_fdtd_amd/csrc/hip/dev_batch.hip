@@ -81,6 +81,10 @@ struct fdtd_dev_batch {
     bool exec_unroll = false, exec_persistent = false, exec_mon = false;
     long long exec_total = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // per-kernel timing: while on, steps run without the captured graph and every E / H launch
+    // carries a start/stop event pair recorded by its own dispatch
+    bool ktime = false;
+    std::vector<hipEvent_t> kev[2];
     static constexpr int kMaxTuned = 16;
     double tuned_ms[kMaxTuned] = {};
     int tuned_code[kMaxTuned] = {};  // kc + 16 * unroll + 32 * persistent
@@ -123,28 +127,49 @@ int upload_layout(fdtd_dev_batch *b, int kc, bool persistent) {
     return 0;
 }
 
-template <int KC>
-void launch_step_kc(fdtd_dev_batch *b) {
+// A start/stop event pair for the next timed launch of kernel slot `which` (0 E, 1 H), or nulls.
+void batch_events(fdtd_dev_batch *b, int which, hipEvent_t *t0, hipEvent_t *t1) {
+    *t0 = *t1 = nullptr;
+    if (!b->ktime) return;
+    hipEvent_t x = nullptr, y = nullptr;
+    if (hipEventCreate(&x) != hipSuccess) return;
+    if (hipEventCreate(&y) != hipSuccess) {
+        (void)hipEventDestroy(x);
+        return;
+    }
+    b->kev[which].push_back(x);
+    b->kev[which].push_back(y);
+    *t0 = x;
+    *t1 = y;
+}
+
+template <typename MatT, int KC>
+void launch_step_typed(fdtd_dev_batch *b) {
     const int G = (int)b->host.size();
     const dim3 grid(b->grid_blocks), block(kBlock);
-    if (b->u8) {
-        hipLaunchKernelGGL((k_batch_E<uint8_t, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo, G, b->total);
+    const BatchGeom *geo = b->geo;
+    if (b->ktime) {
+        hipEvent_t e0, e1, h0, h1;
+        batch_events(b, 0, &e0, &e1);
+        batch_events(b, 1, &h0, &h1);
+        hipExtLaunchKernelGGL((k_batch_E<MatT, KC>), grid, block, 0, b->stream, e0, e1, 0, geo, G, b->total);
         if (b->mon)
-            hipLaunchKernelGGL((k_batch_H<uint8_t, true, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo, G,
-                               b->total);
+            hipExtLaunchKernelGGL((k_batch_H<MatT, true, KC>), grid, block, 0, b->stream, h0, h1, 0, geo, G, b->total);
         else
-            hipLaunchKernelGGL((k_batch_H<uint8_t, false, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo, G,
-                               b->total);
+            hipExtLaunchKernelGGL((k_batch_H<MatT, false, KC>), grid, block, 0, b->stream, h0, h1, 0, geo, G, b->total);
     } else {
-        hipLaunchKernelGGL((k_batch_E<uint16_t, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo, G,
-                           b->total);
+        hipLaunchKernelGGL((k_batch_E<MatT, KC>), grid, block, 0, b->stream, geo, G, b->total);
         if (b->mon)
-            hipLaunchKernelGGL((k_batch_H<uint16_t, true, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo,
-                               G, b->total);
+            hipLaunchKernelGGL((k_batch_H<MatT, true, KC>), grid, block, 0, b->stream, geo, G, b->total);
         else
-            hipLaunchKernelGGL((k_batch_H<uint16_t, false, KC>), grid, block, 0, b->stream, (const BatchGeom *)b->geo,
-                               G, b->total);
+            hipLaunchKernelGGL((k_batch_H<MatT, false, KC>), grid, block, 0, b->stream, geo, G, b->total);
     }
+}
+
+template <int KC>
+void launch_step_kc(fdtd_dev_batch *b) {
+    if (b->u8) launch_step_typed<uint8_t, KC>(b);
+    else launch_step_typed<uint16_t, KC>(b);
     hipLaunchKernelGGL(k_step_tick, dim3(1), dim3(64), 0, b->stream, b->step);
 }
 
@@ -238,7 +263,7 @@ int batch_chunk(fdtd_dev_batch *b, double dt, int n0, int n) {
     }
     if ((rc = upload_layout(b, b->kc, b->persistent))) return rc;
     const int left = n - done;
-    const int segs = left / kSeg;
+    const int segs = b->ktime ? 0 : left / kSeg;
     if (segs > 0) {
         if ((rc = segment_graph(b))) return rc;
         for (int s = 0; s < segs; s++) FDTD_TRY(hipGraphLaunch(b->exec, b->stream));
@@ -292,6 +317,8 @@ void fdtd_dev_batch_free(fdtd_dev_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    for (auto &v : b->kev)
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
     (void)dev_free(b->step);
     (void)dev_free(b->geo);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -349,6 +376,33 @@ int fdtd_dev_batch_timer_stop(fdtd_dev_batch *b, double *elapsed_ms) {
     FDTD_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     *elapsed_ms = ms;
     return 0;
+}
+
+int fdtd_dev_batch_kernel_timing(fdtd_dev_batch *b, int on) {
+    b->ktime = on != 0;
+    return 0;
+}
+
+int fdtd_dev_batch_kernel_times(fdtd_dev_batch *b, int counts[2], double ms[2]) {
+    FDTD_TRY(hipStreamSynchronize(b->stream));
+    int rc = 0;
+    for (int w = 0; w < 2; w++) {
+        counts[w] = 0;
+        ms[w] = 0.0;
+        auto &v = b->kev[w];
+        for (size_t q = 0; q + 1 < v.size(); q += 2) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, v[q], v[q + 1]) == hipSuccess) {
+                counts[w]++;
+                ms[w] += t;
+            } else {
+                rc = fail("batch kernel times", hipErrorInvalidValue);
+            }
+        }
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+        v.clear();
+    }
+    return rc;
 }
 
 }  // extern "C"

@@ -148,7 +148,8 @@ def bench_tapers(fd, args):
     """Batched taper workload: whole-job cell-updates/s over every grid (one JSON line)."""
     W, S = args.warmup, args.steps
     print(f"bench: building {args.tapers} taper grids", file=sys.stderr, flush=True)
-    grids, dt, omegas, lengths = tapers_workload(fd, args.tapers, W + S)
+    KW = 10  # kernel-timing window (every E / H launch event-timed, no graph)
+    grids, dt, omegas, lengths = tapers_workload(fd, args.tapers, W + S + KW)
     b = fd.Batch(grids)
     if args.kchunk:
         b.set_shape(args.kchunk, args.kchunk <= 4, True)
@@ -163,9 +164,19 @@ def bench_tapers(fd, args):
     fd.dev_check(fd.lib().fdtd_dev_batch_timer_stop(b.b, C.byref(ms)), "timer")
     wall = time.perf_counter() - t0
     elapsed_s = max(ms.value / 1e3, wall)
+    # kernel durations: a separate window in which every E / H launch carries its own event pair
+    # (graph replays carry none)
+    L = fd.lib()
+    fd.dev_check(L.fdtd_dev_batch_kernel_timing(b.b, 1), "kernel timing")
+    b.run(dt, W + S, KW)
+    cnt, kms = (C.c_int * 2)(), (C.c_double * 2)()
+    fd.dev_check(L.fdtd_dev_batch_kernel_times(b.b, cnt, kms), "kernel times")
+    fd.dev_check(L.fdtd_dev_batch_kernel_timing(b.b, 0), "kernel timing off")
+    e_us, h_us = (kms[w] / cnt[w] * 1e3 if cnt[w] else 0.0 for w in range(2))
     cells = info["cells"]
     value = cells * S / elapsed_s
-    achieved = STEP_BYTES * cells / (elapsed_s / S) / 1e9
+    dom = max(e_us, h_us)
+    achieved = SPLIT_KERNEL_BYTES * cells / (dom * 1e-6) / 1e9 if dom > 0 else 0.0
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": S, "warmup": W,
            "ms_per_step": elapsed_s * 1e3 / S, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
@@ -177,8 +188,11 @@ def bench_tapers(fd, args):
                       "lengths_cells": [int(round(x / DISC)) + 20 for x in lengths]},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                        "kernel": "batched step (E + H launches, graph replays)",
-                        "algorithmic_bytes_per_launch": STEP_BYTES * cells,
+                        "kernel": "batch_update_E" if e_us >= h_us else "batch_update_H",
+                        "kernel_avg_us": round(dom, 1), "e_us": round(e_us, 1), "h_us": round(h_us, 1),
+                        "timed_launches": int(min(cnt[0], cnt[1])), "kernel_timing_window_steps": KW,
+                        "algorithmic_bytes_per_launch": SPLIT_KERNEL_BYTES * cells,
+                        "step_frac_48B": round(STEP_BYTES * cells / (elapsed_s / S) / 1e9 / HBM_PEAK_GBS, 4),
                         "autotuned_shape": info["shape"], "tuned_ms_per_step": info["tuned"]}}
     b.free()
     for g in grids:

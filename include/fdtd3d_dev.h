@@ -117,6 +117,11 @@ int fdtd_dev_batch_info(fdtd_dev_batch *b, int shape[3], long long *cells, int *
                         double tuned_ms[16]);
 int fdtd_dev_batch_timer_start(fdtd_dev_batch *b);
 int fdtd_dev_batch_timer_stop(fdtd_dev_batch *b, double *elapsed_ms);
+/* Per-kernel timing of a batch: while on, steps run without the captured graph and every E / H
+ * launch is bracketed by events its own dispatch records; kernel_times returns per slot (0 E,
+ * 1 H) the number of timed launches and their summed durations in ms, and clears the record. */
+int fdtd_dev_batch_kernel_timing(fdtd_dev_batch *b, int on);
+int fdtd_dev_batch_kernel_times(fdtd_dev_batch *b, int counts[2], double ms[2]);
 
 /* Soft sources on plane k = cz.  mode is interleaved complex double[6*mi*mj] (already
  * power-normalised by the caller).  adjoint selects the CW adjoint form (timefield3d.c:3028). */
