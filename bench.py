@@ -374,6 +374,8 @@ def bench_gradient(fd, args):
     n = I * J * K
     dom = max(e_us, h_us)
     achieved = SPLIT_KERNEL_BYTES * n / (dom * 1e-6) / 1e9
+    dom_name = "yee_update_E" if e_us >= h_us else "yee_update_H"
+    traffic = read_traffic("yee_traffic_cfg3_taper.json")
     out = {"metric": METRIC, "value": 2 * steps * n / (t_fwd + t_adj), "unit": "cell-updates/s", "n_gpus": 1,
            "steps": 2 * steps, "warmup": 0, "ms_per_step": (t_fwd + t_adj) * 1e3 / (2 * steps),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -383,8 +385,9 @@ def bench_gradient(fd, args):
                                   "1.55 um and the DFT-domain eps-gradient",
                       "grid_lib_axes": [I, J, K], "steps_per_run": steps, "dft_box_cells": nb},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                        "kernel": "yee_update_E" if e_us >= h_us else "yee_update_H",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": traffic.get(dom_name) if traffic else None,
+                        "kernel": dom_name,
                         "kernel_avg_us": round(dom, 2), "e_us": round(e_us, 2), "h_us": round(h_us, 2),
                         "algorithmic_bytes_per_launch": SPLIT_KERNEL_BYTES * n,
                         "step_frac_48B": round(STEP_BYTES * n / ((e_us + h_us) * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
@@ -559,8 +562,8 @@ def calibrate_copy(mib, reps=8):
                       "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4}), flush=True)
 
 
-def read_traffic():
-    p = os.path.join(ROOT, "profiles", "yee_traffic_cfg2.json")
+def read_traffic(name="yee_traffic_cfg2.json"):
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     with open(p) as fh:
