@@ -180,6 +180,9 @@ SIGNATURES = {
     "fdtd_dev_get_fields": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_set_fields": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_count_nonfinite": ([C.c_void_p, _P(C.c_longlong)], _i),
+    "fdtd_dev_copy_planes": ([C.c_void_p, _i, C.c_void_p, _i, C.c_uint], _i),
+    "fdtd_dev_update_E_cpml": ([C.c_void_p, _d], _i),
+    "fdtd_dev_update_H_cpml": ([C.c_void_p, _d], _i),
     "fdtd_dev_set_finite_check": ([C.c_void_p, _i], _i),
     "fdtd_dev_cpml_on": ([C.c_void_p, _P(_i), _i, _d, _d, _d], _i),
     "fdtd_dev_set_step_kernel": ([C.c_void_p, _i, _i], _i),

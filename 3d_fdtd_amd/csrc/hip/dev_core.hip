@@ -194,6 +194,7 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->split_kchunk = 0;
     d->finite_every = 0;
     d->finite_buf = nullptr;
+    d->ev0 = d->ev1 = d->xev = nullptr;
     hipError_t e = dev_stream_acquire(&d->stream);
     // fields: 6 arrays, each N floats, each starting on a 256-byte boundary
     d->Nal = (d->N + 63) / 64 * 64;
@@ -212,6 +213,7 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     if (e == hipSuccess) e = dev_malloc(&d->red_out, sizeof(double2) * 4);
     if (e == hipSuccess) e = hipEventCreate(&d->ev0);
     if (e == hipSuccess) e = hipEventCreate(&d->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&d->xev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
     if (e != hipSuccess) {
         fail("fdtd_dev_create", e);
@@ -240,6 +242,7 @@ void fdtd_dev_destroy(fdtd_dev *d) {
     for (auto &p : d->src) (void)dev_free(p.J);
     for (auto &m : d->modes) (void)dev_free(m.dev);
     if (d->ev0) (void)hipEventDestroy(d->ev0);
+    if (d->xev) (void)hipEventDestroy(d->xev);
     if (d->ev1) (void)hipEventDestroy(d->ev1);
     if (d->stream) (void)dev_stream_release(d->stream);
     delete d;

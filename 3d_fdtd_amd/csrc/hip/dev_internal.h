@@ -200,6 +200,7 @@ struct fdtd_dev {
     void *finite_buf;
 
     hipEvent_t ev0, ev1;
+    hipEvent_t xev;  // ordering event of halo-plane copies between grids (no timing)
 };
 
 namespace fdtd {

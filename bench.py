@@ -423,6 +423,93 @@ def gradient_cpu_baseline(fam, ep, K, target_s=10.0):
                       f"OMP_NUM_THREADS={threads}"}
 
 
+def bench_shard(fd, args):
+    """One GPU's share of the 620x200x100 taper when the grid is split into 8 z slabs (library
+    axes 200 x 100 x 620): a middle shard of 78 owned planes plus its two ghost planes (200 x 100 x
+    80), CPML on the lateral faces only, paired with its upper neighbour shard on the same GPU for
+    the halo planes (fdtd_dev_copy_planes: Ex, Ey down after the E update, Hx, Hy up after the H
+    update, 2 x 2 x 80 KB per step).  value = the shard's owned cell-updates/s from its own E and H
+    kernel durations (every launch event-timed); the halo copies are timed separately.  Across
+    GPUs the planes would travel over xGMI (RCCL send/recv or peer copies); the device-to-device
+    copy stands in for them here."""
+    design = importlib.import_module("3d_fdtd_amd.design")
+    fam = design.TaperFamily(disc=DISC, I=200, J=100, core_cells=11, npml=10, port=48, steps=10,
+                             lam=(1.55, 1.55), nfreq=1)
+    ep, K = fam.permittivity(np.concatenate([np.linspace(0.5, 3.0, 8), [10.0]]))
+    I, J = fam.I, fam.J
+    ep = ep.reshape(K, J, I, 3)
+    owned = 78
+    k0 = 3 * owned + 1  # shard 3 of 8: global planes [k0, k0 + 78), ghosts k0 - 1 and k0 + 78
+    dt = fam.dt
+    rng = np.random.default_rng(5)
+    npml = (fam.npml, fam.npml, fam.npml, fam.npml, 0, 0)
+    shards = []
+    for lo in (k0 - 1, k0 + owned - 1):
+        g = fd.DevGrid(I, J, owned + 2, DISC, ep[lo:lo + owned + 2].reshape(-1))
+        g.cpml(npml, 3, 1e-8, 1.0, 0.05)
+        g.set_fields(rng.uniform(-1, 1, 6 * I * J * (owned + 2)))
+        shards.append(g)
+    a, b = shards
+    L = fd.lib()
+    EXY, HXY = 0b000011, 0b011000
+
+    def step():
+        for g in shards:
+            fd.dev_check(L.fdtd_dev_update_E_cpml(g.d, dt), "E")
+        fd.dev_check(L.fdtd_dev_copy_planes(a.d, owned + 1, b.d, 1, EXY), "halo E")
+        for g in shards:
+            fd.dev_check(L.fdtd_dev_update_H_cpml(g.d, dt), "H")
+        fd.dev_check(L.fdtd_dev_copy_planes(b.d, 0, a.d, owned, HXY), "halo H")
+
+    S = args.steps if args.steps != N_RUN else 400
+    for _ in range(50):
+        step()
+    for g in shards:
+        fd.dev_check(L.fdtd_dev_synchronize(g.d), "sync")
+    t0 = time.perf_counter()
+    for _ in range(S):
+        step()
+    for g in shards:
+        fd.dev_check(L.fdtd_dev_synchronize(g.d), "sync")
+    pair_s = (time.perf_counter() - t0) / S
+    # kernel durations of the shard's own launches, without the copies between them (a launch
+    # queued behind a cross-stream wait takes its start stamp early)
+    a.kernel_timing(1)
+    for _ in range(min(S, 200)):
+        fd.dev_check(L.fdtd_dev_update_E_cpml(a.d, dt), "E")
+        fd.dev_check(L.fdtd_dev_update_H_cpml(a.d, dt), "H")
+    (e_us, h_us), cnt = a.kernel_times()
+    a.kernel_timing(0)
+    t0 = time.perf_counter()
+    for _ in range(S):
+        fd.dev_check(L.fdtd_dev_copy_planes(a.d, owned + 1, b.d, 1, EXY), "halo E")
+        fd.dev_check(L.fdtd_dev_copy_planes(b.d, 0, a.d, owned, HXY), "halo H")
+    for g in shards:
+        fd.dev_check(L.fdtd_dev_synchronize(g.d), "sync")
+    copy_us = (time.perf_counter() - t0) / S * 1e6
+    for g in shards:
+        g.free()
+    cells = I * J * owned
+    dom = max(e_us, h_us)
+    achieved = SPLIT_KERNEL_BYTES * cells / (dom * 1e-6) / 1e9
+    out = {"metric": METRIC, "value": cells / ((e_us + h_us) * 1e-6), "unit": "cell-updates/s", "n_gpus": 1,
+           "steps": S, "warmup": 50, "ms_per_step": (e_us + h_us) / 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "one of 8 z-slab shards of the 620x200x100 taper: 78 owned planes + 2 ghost "
+                                  "planes (library 200x100x80), lateral 10-cell CPML, halo planes exchanged with "
+                                  "a neighbour shard on the same GPU",
+                      "owned_cells": cells, "halo_bytes_per_step": 2 * 2 * I * J * 4},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "yee_update_E" if e_us >= h_us else "yee_update_H",
+                        "kernel_avg_us": round(dom, 2), "e_us": round(e_us, 2), "h_us": round(h_us, 2),
+                        "timed_launches": int(min(cnt)), "algorithmic_bytes_per_launch": SPLIT_KERNEL_BYTES * cells},
+           "shard": {"pair_step_us_two_shards_one_gpu": round(pair_s * 1e6, 1),
+                     "halo_copy_pair_us_device_to_device": round(copy_us, 1),
+                     "full_grid_cells_per_shard_fraction": round(owned / K, 4)}}
+    print(json.dumps(out), flush=True)
+
+
 def bench_surrogate(fd, args):
     """The transmission surrogate's hand-written BF16 matrix-core kernels: full-batch Adam steps on
     4096 samples and screening of 10^5 candidates (the design loop's shapes: 9 -> 256 x 4 -> 32,
@@ -579,12 +666,13 @@ def main():
     ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
     ap.add_argument("--kchunk", type=int, default=None,
                     help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
-    ap.add_argument("--workload", choices=("waveguide", "tapers", "design", "surrogate", "gradient"),
+    ap.add_argument("--workload", choices=("waveguide", "tapers", "design", "surrogate", "gradient", "shard"),
                     default="waveguide",
                     help="waveguide: the strip waveguide (default); tapers: the batched taper set; design: "
                          "verified taper designs/hour, surrogate loop vs plain adjoint loop; surrogate: the MLP's "
                          "matrix-core training and screening kernels; gradient: the 620x200x100 taper's forward + "
-                         "adjoint runs and DFT eps-gradient")
+                         "adjoint runs and DFT eps-gradient; shard: one GPU's z-slab share of that taper with "
+                         "its halo planes")
     ap.add_argument("--train", type=int, default=4096, help="design workload: FDTD training designs")
     ap.add_argument("--fit-steps", type=int, default=1500, help="design workload: surrogate training steps")
     ap.add_argument("--campaign", type=float, default=150.0,
@@ -616,6 +704,9 @@ def main():
         return
     if args.workload == "gradient":
         bench_gradient(fd, args)
+        return
+    if args.workload == "shard":
+        bench_shard(fd, args)
         return
     W, S = args.warmup, args.steps
     g, dt, md, mons, omegas = workload(fd, W + S + min(400, max(S // 4, 20)))

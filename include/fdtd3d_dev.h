@@ -167,6 +167,12 @@ int fdtd_dev_kernel_times(fdtd_dev *d, int counts[3], double ms[3]);
 int fdtd_dev_count_nonfinite(fdtd_dev *d, long long *count);
 int fdtd_dev_set_finite_check(fdtd_dev *d, int every);
 
+/* Halo planes of z-slab shards (no counterpart in the reference, which runs one grid): copy the
+ * components in comp_mask (bit m = component m: Ex, Ey, Ez, Hx, Hy, Hz) of plane ks of src into
+ * plane kd of dst (same I, J), ordered after src's queued work and before either grid's later
+ * work.  A shard keeps its neighbours' planes in its boundary planes, which no kernel updates. */
+int fdtd_dev_copy_planes(fdtd_dev *dst, int kd, fdtd_dev *src, int ks, unsigned comp_mask);
+
 /* Timing of the work enqueued on the grid's stream (HIP events on that stream). */
 int fdtd_dev_timer_start(fdtd_dev *d);
 int fdtd_dev_timer_stop(fdtd_dev *d, double *elapsed_ms);
