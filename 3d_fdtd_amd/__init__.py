@@ -181,6 +181,8 @@ SIGNATURES = {
     "fdtd_dev_set_fields": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_count_nonfinite": ([C.c_void_p, _P(C.c_longlong)], _i),
     "fdtd_dev_copy_planes": ([C.c_void_p, _i, C.c_void_p, _i, C.c_uint], _i),
+    "fdtd_dev_set_materials_extruded": ([C.c_void_p, _i, _P(_f), _P(_f), _P(_f), _P(C.c_uint16), _i, _i, _i],
+                                        _i),
     "fdtd_dev_update_E_cpml": ([C.c_void_p, _d], _i),
     "fdtd_dev_update_H_cpml": ([C.c_void_p, _d], _i),
     "fdtd_dev_set_finite_check": ([C.c_void_p, _i], _i),
@@ -357,7 +359,9 @@ def dev_check(rc, what="fdtd_dev call"):
 class DevGrid:
     """A device-only grid (fdtd_dev, no host mirrors): the unit of batched runs."""
 
-    def __init__(self, I, J, K, disc, ep, mu=None, sig=None):
+    def __init__(self, I, J, K, disc, ep, mu=None, sig=None, extruded=None):
+        """ep/mu/sig: host AoS materials (set_materials); or extruded = (table [nmat][3] of ep,
+        ids [K][I], j0, j1, background id) for a geometry rasterised on the device."""
         n = I * J * K
         self.I, self.J, self.K, self.disc = I, J, K, disc
         self.d = lib().fdtd_dev_create(I, J, K, disc)
@@ -373,12 +377,26 @@ class DevGrid:
                 raise ValueError(f"material array of {a.size} values for {size}")
             return a, a.ctypes.data_as(fp)
 
-        if ep is None:
-            raise ValueError("DevGrid needs a permittivity array")
-        ep, ep_p = arr(ep, 3 * n)
-        mu, mu_p = arr(mu, 3 * n)
-        sig, sig_p = arr(sig, 6 * n)
-        dev_check(lib().fdtd_dev_set_materials(self.d, ep_p, mu_p, sig_p), "set_materials")
+        try:
+            if extruded is not None:  # (table_ep [nmat][3], ids [K][I], j0, j1, background)
+                table, ids, j0, j1, bg = extruded
+                table = np.ascontiguousarray(table, np.float32)
+                ids = np.ascontiguousarray(ids, np.uint16)
+                if ids.size != K * I:
+                    raise ValueError(f"{ids.size} extruded ids for {K * I}")
+                dev_check(lib().fdtd_dev_set_materials_extruded(
+                    self.d, table.shape[0], table.ctypes.data_as(fp), None, None,
+                    ids.ctypes.data_as(C.POINTER(C.c_uint16)), j0, j1, bg), "set_materials_extruded")
+                return
+            if ep is None:
+                raise ValueError("DevGrid needs a permittivity array")
+            ep, ep_p = arr(ep, 3 * n)
+            mu, mu_p = arr(mu, 3 * n)
+            sig, sig_p = arr(sig, 6 * n)
+            dev_check(lib().fdtd_dev_set_materials(self.d, ep_p, mu_p, sig_p), "set_materials")
+        except Exception:
+            self.free()
+            raise
 
     def cpml(self, npml, order=3, R0=1e-8, kappa_max=1.0, alpha_max=0.05):
         dev_check(lib().fdtd_dev_cpml_on(self.d, (C.c_int * 6)(*npml), order, R0, kappa_max, alpha_max), "cpml_on")

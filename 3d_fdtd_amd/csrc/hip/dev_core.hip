@@ -35,6 +35,21 @@ struct MatKeyHash {
     }
 };
 
+// Material ids of an extruded geometry: rows j0 <= j < j1 take ids[k][i], every other cell the
+// background id (the structure of the reference's waveguide and taper setters,
+// timefield3d.c:1458-1535, and of adjoint3d_taper_insert).
+template <typename T>
+__global__ void k_extrude_ids(T *__restrict__ mat, const uint16_t *__restrict__ ids, int I, int J, long long N,
+                              int j0, int j1, int background) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const long long P = (long long)I * J;
+    const int k = (int)(c / P);
+    const long long p = c - (long long)k * P;
+    const int j = (int)(p / I), i = (int)(p - (long long)j * I);
+    mat[c] = (T)(j >= j0 && j < j1 ? ids[(long long)k * I + i] : background);
+}
+
 // AoS double[6N] -> six SoA float arrays.
 __global__ void k_aos_to_soa(const double *__restrict__ aos, float *__restrict__ base, long long N, long long Nal) {
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,6 +317,69 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
     } else {
         FDTD_TRY(hipMemcpy(d->mat, cell_id.data(), (size_t)d->N * 2, hipMemcpyHostToDevice));
     }
+    if (nmat != d->nmat || !d->d_props) {
+        (void)dev_free(d->d_props);
+        (void)dev_free(d->d_coef);
+        d->d_props = nullptr;
+        d->d_coef = nullptr;
+        FDTD_TRY(dev_malloc(&d->d_props, (size_t)nmat * kPropStride * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->d_coef, (size_t)2 * nmat * kCoefStride * sizeof(float)));
+    }
+    FDTD_TRY(hipMemcpy(d->d_props, props.data(), props.size() * sizeof(float), hipMemcpyHostToDevice));
+    d->props = std::move(props);
+    d->nmat = nmat;
+    d->mat_bytes = bytes;
+    d->coef_dt = NAN;
+    d->have_mats = true;
+    return 0;
+}
+
+int fdtd_dev_set_materials_extruded(fdtd_dev *d, int nmat, const float *ep, const float *mu, const float *sig,
+                                    const uint16_t *ids, int j0, int j1, int background) {
+    if (nmat < 1 || nmat > 65536 || !ep || !ids || background < 0 || background >= nmat || j0 < 0 || j1 > d->J ||
+        j0 > j1) {
+        set_error("set_materials_extruded: bad table size, id, row range or null array");
+        return -2;
+    }
+    const long long nid = (long long)d->K * d->I;
+    for (long long q = 0; q < nid; q++)
+        if (ids[q] >= nmat) {
+            set_error("set_materials_extruded: material id outside the table");
+            return -2;
+        }
+    // kernels already enqueued may still read the id volume and the tables
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    std::vector<float> props((size_t)nmat * kPropStride, 0.0f);
+    for (int m = 0; m < nmat; m++) {
+        float *pr = &props[(size_t)m * kPropStride];
+        for (int q = 0; q < 3; q++) {
+            pr[q] = ep[3 * m + q];
+            pr[3 + q] = mu ? mu[3 * m + q] : 1.0f;
+        }
+        for (int q = 0; q < 6; q++) pr[6 + q] = sig ? sig[6 * m + q] : 0.0f;
+    }
+    const int bytes = nmat <= 256 ? 1 : 2;
+    if (d->mat && bytes != d->mat_bytes) {
+        FDTD_TRY(dev_free(d->mat));
+        d->mat = nullptr;
+    }
+    if (!d->mat) FDTD_TRY(dev_malloc(&d->mat, (size_t)d->N * bytes));
+    uint16_t *dids = nullptr;
+    FDTD_TRY(dev_malloc(&dids, (size_t)nid * sizeof(uint16_t)));
+    hipError_t e = hipMemcpyAsync(dids, ids, (size_t)nid * sizeof(uint16_t), hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) {
+        const unsigned nb = (unsigned)((d->N + kBlock - 1) / kBlock);
+        if (bytes == 1)
+            hipLaunchKernelGGL(k_extrude_ids<uint8_t>, dim3(nb), dim3(kBlock), 0, d->stream,
+                               static_cast<uint8_t *>(d->mat), dids, d->I, d->J, d->N, j0, j1, background);
+        else
+            hipLaunchKernelGGL(k_extrude_ids<uint16_t>, dim3(nb), dim3(kBlock), 0, d->stream,
+                               static_cast<uint16_t *>(d->mat), dids, d->I, d->J, d->N, j0, j1, background);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);  // the host id array may go right after
+    (void)dev_free(dids);
+    if (e != hipSuccess) return fail("fdtd_dev_set_materials_extruded", e);
     if (nmat != d->nmat || !d->d_props) {
         (void)dev_free(d->d_props);
         (void)dev_free(d->d_coef);

@@ -81,6 +81,21 @@ class TaperFamily:
         ep[:, j0:j0 + self.core] = (self.eps_clad + frac * (self.eps_core - self.eps_clad))[:, None, :, None]
         return ep.reshape(-1), K
 
+    def extruded(self, p):
+        """The permittivity of permittivity(p) as the device rasterises it: a table of the distinct
+        values ([nmat][3], float32) and one id per (k, i) for the core rows, every other cell the
+        cladding (fdtd_dev_set_materials_extruded).  The host work is O(K I) instead of the
+        O(K J I) volume and its upload; the values are the same float32 numbers."""
+        K = self.length_cells(p)
+        w = self.widths_cells(p, K)
+        x = np.arange(self.I) - self.I / 2
+        frac = np.clip(np.minimum(x[None, :] + 1, w[:, None] / 2) - np.maximum(x[None, :], -w[:, None] / 2), 0, 1)
+        rows = (self.eps_clad + frac * (self.eps_core - self.eps_clad)).astype(np.float32)
+        values, ids = np.unique(np.concatenate([[np.float32(self.eps_clad)], rows.reshape(-1)]), return_inverse=True)
+        table = np.repeat(values[:, None], 3, axis=1)
+        j0 = self.J // 2 - self.core // 2
+        return (table, ids[1:].reshape(K, self.I), j0, j0 + self.core, int(ids[0])), K
+
     def run_inputs(self, p):
         """Everything that defines the FDTD run of design p, as plain arrays and numbers: the
         permittivity, grid, time step, CPML depth, mode window, source schedule, wavelengths,
@@ -92,8 +107,8 @@ class TaperFamily:
 
     def grid(self, p):
         """A device grid for design p with its source and the launched / transmitted monitors."""
-        ep, K = self.permittivity(p)
-        g = fd.DevGrid(self.I, self.J, K, self.disc, ep)
+        ex, K = self.extruded(p)
+        g = fd.DevGrid(self.I, self.J, K, self.disc, None, extruded=ex)
         g.cpml((self.npml,) * 6, 3, 1e-8, 1.0, 0.05)
         ci, cj = self.I // 2, self.J // 2
         ks = self.npml + 1
@@ -120,14 +135,14 @@ class TaperFamily:
         Widening the core by dw moves each edge by dw / 2, so eps of the fractional cell that
         holds each edge grows by (eps_core - eps_clad) dw / (2 dx) (the raster is the fill-fraction
         one of permittivity()); the spline basis carries d T / d w(k) to the control points."""
-        ep, K = self.permittivity(p)
+        ex, K = self.extruded(p)
         ci, cj = self.I // 2, self.J // 2
         ks, km = self.npml + 1, K - self.npml - 2
         j0 = self.J // 2 - self.core // 2
         box = (0, self.I, j0, j0 + self.core, ks + 4, km)
         nb = self.I * self.core * (km - ks - 4)
         nf = len(self.omegas)
-        g = fd.DevGrid(self.I, self.J, K, self.disc, ep)
+        g = fd.DevGrid(self.I, self.J, K, self.disc, None, extruded=ex)
         g.cpml((self.npml,) * 6, 3, 1e-8, 1.0, 0.05)
         g.add_source(ci, cj, ks, self.mode, self.mi, self.mj, self.disc, self.amp, self.phase)
         mon_in = g.add_overlap(ci, cj, ks + 3, self.mode, self.mi, self.mj, self.disc, self.omegas)
@@ -136,7 +151,7 @@ class TaperFamily:
         g.run(self.dt, 0, self.steps)
         a_in, _ = g.overlap(mon_in, nf)
         a_out, _ = g.overlap(mon_out, nf)
-        ga = fd.DevGrid(self.I, self.J, K, self.disc, ep)
+        ga = fd.DevGrid(self.I, self.J, K, self.disc, None, extruded=ex)
         ga.cpml((self.npml,) * 6, 3, 1e-8, 1.0, 0.05)
         w0 = self.phase[1] / self.dt
         ga.add_adjoint_source(ci, cj, km, self.mode, self.mi, self.mj, self.disc, w0 * self.dt, self.amp, self.phase)

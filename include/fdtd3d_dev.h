@@ -12,6 +12,8 @@
 #ifndef FDTD3D_DEV_H
 #define FDTD3D_DEV_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -28,6 +30,15 @@ void fdtd_dev_destroy(fdtd_dev *d);                                           /*
  * each cell stores a 1-byte id (2-byte when more than 256 tuples).  mu or sig may be NULL:
  * mu = 1 and sig = 0 everywhere. */
 int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const float *sig);
+
+/* Extruded geometry rasterised on the device (no reference counterpart; it serves the design
+ * loops, which rebuild a grid per candidate): a table of nmat materials (ep 3 per entry, mu 3 and
+ * sig 6 per entry or NULL for 1 / 0) and, for the rows j0 <= j < j1, one id per (k, i) in ids[k * I
+ * + i]; every other cell takes `background`.  This is the structure of the reference's waveguide
+ * and taper setters (timefield3d.c:1458-1535) and of adjoint3d_taper_insert: the host uploads
+ * K * I ids instead of 3 * N permittivities. */
+int fdtd_dev_set_materials_extruded(fdtd_dev *d, int nmat, const float *ep, const float *mu, const float *sig,
+                                    const uint16_t *ids, int j0, int j1, int background);
 
 /* Fields / sources / shells / DFT accumulators between host AoS arrays and the device. */
 int fdtd_dev_set_fields(fdtd_dev *d, const double *fields);

@@ -11,6 +11,9 @@
    launch monitor left out of the gradient box, which touches w_1 alone) bound the agreement:
    cosine >= 0.95 over all eight widths and relative L2 <= 0.2 over w_2..w_8.
 3. The plain method (adjoint gradient ascent) lifts a seeded design to the spec |t|^2 >= 0.7.
+4. The device-side raster of a design (fdtd_dev_set_materials_extruded: K x I ids and a table)
+   gives the same run as the host raster (set_materials over the full volume): identical fields
+   and monitor spectra.
 """
 import importlib
 import sys
@@ -84,3 +87,31 @@ def test_plain_adjoint_loop_reaches_spec(fam):
     say("band |t|^2 history", np.round(hist, 4), "final (batched FDTD) %.4f" % T)
     assert hist[-1] >= 0.7 and hist[0] < 0.7
     assert abs(T - hist[-1]) <= 1e-5  # the single-grid and batched runs agree on the final design
+
+
+def test_device_raster_equals_host_raster(fam):
+    fd = design.fd
+    rng = np.random.default_rng(31)
+    for p in fam.random(rng, 3):
+        say("device vs host raster, design", np.round(p, 3))
+        ep, K = fam.permittivity(p)
+        ex, _ = fam.extruded(p)
+        runs = []
+        for g in (fd.DevGrid(fam.I, fam.J, K, fam.disc, ep), fd.DevGrid(fam.I, fam.J, K, fam.disc, None, extruded=ex)):
+            g.cpml((fam.npml,) * 6, 3, 1e-8, 1.0, 0.05)
+            ci, cj, ks = fam.I // 2, fam.J // 2, fam.npml + 1
+            g.add_source(ci, cj, ks, fam.mode, fam.mi, fam.mj, fam.disc, fam.amp, fam.phase)
+            mon = g.add_overlap(ci, cj, K - fam.npml - 2, fam.mode, fam.mi, fam.mj, fam.disc, fam.omegas)
+            g.run(fam.dt, 0, 400)
+            runs.append((g.fields(), g.overlap(mon, len(fam.omegas))[0]))
+            g.free()
+        assert np.abs(runs[0][0]).max() > 0
+        assert np.array_equal(runs[0][0], runs[1][0])
+        assert np.array_equal(runs[0][1], runs[1][1])
+    # a table id outside the table is refused before any device work
+    ex, K = fam.extruded(fam.random(rng, 1)[0])
+    table, ids, j0, j1, bg = ex
+    bad = ids.copy()
+    bad[0, 0] = table.shape[0]
+    with pytest.raises(RuntimeError, match="outside the table"):
+        fd.DevGrid(fam.I, fam.J, K, fam.disc, None, extruded=(table, bad, j0, j1, bg))
