@@ -119,9 +119,10 @@ int upload_layout(fdtd_dev_batch *b, int kc, bool persistent) {
     FDTD_TRY(hipMemcpyAsync(b->geo, b->host.data(), b->host.size() * sizeof(BatchGeom), hipMemcpyHostToDevice,
                             b->stream));
     FDTD_TRY(hipStreamSynchronize(b->stream));  // the host table is rewritten by the next layout
-    // persistent: up to 8 blocks of 256 threads per CU on 256 CUs; else one block per virtual
-    // block; either way a multiple of 8 (XCDs) so that a grid-stride step keeps the XCD
-    long long g = persistent && b->total > 2048 ? 2048 : b->total;
+    // persistent: 2048 threads per CU on 256 CUs; else one block per virtual block; either way
+    // a multiple of 8 (XCDs) so that a grid-stride step keeps the XCD
+    const long long cap = 256LL * 2048 / kBlock;
+    long long g = persistent && b->total > cap ? cap : b->total;
     g = (g + 7) / 8 * 8;
     b->grid_blocks = (unsigned)(g < 8 ? 8 : g);
     return 0;

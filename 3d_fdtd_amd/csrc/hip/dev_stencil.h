@@ -4,7 +4,7 @@
 // 3176-3239 (overlap); the PML is the extension whose formulas oracle/fdtd_oracle.c states.
 //
 // Layout: six FP32 component arrays, cell c = i + I*(j + J*k).  A block body handles one logical
-// block: 256 (i, j) columns of one z chunk; each thread marches its column upward, keeping the k-1
+// block: kBlock (i, j) columns of one z chunk; each thread marches its column upward, keeping the k-1
 // (E update) or k+1 (H update) neighbours it needs in registers; the i-1/j-1 (i+1/j+1) neighbours
 // come through L1/L2.
 #pragma once
