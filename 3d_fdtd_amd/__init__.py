@@ -181,6 +181,7 @@ SIGNATURES = {
     "fdtd_dev_set_fields": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_count_nonfinite": ([C.c_void_p, _P(C.c_longlong)], _i),
     "fdtd_dev_copy_planes": ([C.c_void_p, _i, C.c_void_p, _i, C.c_uint], _i),
+    "fdtd_dev_set_streaming_loads": ([C.c_void_p, _i], _i),
     "fdtd_dev_set_materials_extruded": ([C.c_void_p, _i, _P(_f), _P(_f), _P(_f), _P(C.c_uint16), _i, _i, _i],
                                         _i),
     "fdtd_dev_update_E_cpml": ([C.c_void_p, _d], _i),

@@ -209,6 +209,7 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->split_kchunk = 0;
     d->finite_every = 0;
     d->finite_buf = nullptr;
+    d->stream_mode = -1;
     d->ev0 = d->ev1 = d->xev = nullptr;
     hipError_t e = dev_stream_acquire(&d->stream);
     // fields: 6 arrays, each N floats, each starting on a 256-byte boundary
@@ -546,6 +547,15 @@ int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk) {
     } else {
         d->fused_kchunk = kchunk;
     }
+    return 0;
+}
+
+int fdtd_dev_set_streaming_loads(fdtd_dev *d, int mode) {
+    if (mode < -1 || mode > 1) {
+        set_error("streaming loads: mode must be -1 (automatic), 0 (off) or 1 (on)");
+        return -2;
+    }
+    d->stream_mode = mode;
     return 0;
 }
 

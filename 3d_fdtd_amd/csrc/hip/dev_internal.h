@@ -38,6 +38,14 @@ constexpr int kBlock = 128;
 // Grids with fewer cells than this run the split and batched kernels with 32-bit cell offsets
 // (every per-grid array is then below 4 GiB).
 constexpr long long kSmallGridCells = 1LL << 30;
+// Streaming (non-temporal) loads of the own-cell fields in the split updates pay off once the
+// six field arrays no longer fit the 256 MB last-level cache and cost grids that fit it (200 x
+// 100 cross-section, E + H per step: 3 M cells 0.87x, 6 M 0.83x, 9 M 0.99x, 12.4 M 1.14x, 18 M
+// 1.06x).  mode: -1 automatic, 0 off, 1 on.
+constexpr long long kStreamFieldBytes = 240LL << 20;
+inline bool streaming_loads(long long cells, int mode) {
+    return mode > 0 || (mode < 0 && 24LL * cells > kStreamFieldBytes);
+}
 
 // Per-material coefficient record (floats), one for each conductivity scheme:
 //   [0..2] caE, [3..5] cbE, [6..8] cjE, [9..11] caH, [12..14] cbH, [15..17] cjH,
@@ -201,6 +209,7 @@ struct fdtd_dev {
     // finite_buf (unsigned long long count, int first step)
     int finite_every;
     void *finite_buf;
+    int stream_mode;  // streaming own-cell loads in the split updates: -1 automatic, 0 off, 1 on
 
     hipEvent_t ev0, ev1;
     hipEvent_t xev;  // ordering event of halo-plane copies between grids (no timing)

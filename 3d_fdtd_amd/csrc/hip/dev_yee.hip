@@ -15,23 +15,24 @@
 namespace fdtd {
 namespace {
 
-template <typename MatT, bool PML, int KC, typename Off>
+// NT: the own-cell field loads carry the streaming hint (grids beyond the last-level cache).
+template <typename MatT, bool PML, int KC, typename Off, bool NT>
 __global__ __launch_bounds__(kBlock) void k_update_E(YeeArgs a, PmlArgs pm) {
-    update_E_block<MatT, PML, KC, Off>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
+    update_E_block<MatT, PML, KC, Off, NT>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
 }
 
-template <typename MatT, bool PML, int KC, typename Off>
+template <typename MatT, bool PML, int KC, typename Off, bool NT>
 __global__ __launch_bounds__(kBlock) void k_update_H(YeeArgs a, PmlArgs pm) {
-    update_H_block<MatT, PML, false, KC, Off>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
+    update_H_block<MatT, PML, false, KC, Off, NT>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
 }
 
 // The monitor arguments raise the scalar register count of the runtime-chunk variant just past
 // the 8-waves-per-SIMD budget; the attribute keeps the occupancy of the plain update there (a few
 // scalars spill to vector lanes).  The unrolled variants are limited by vector registers anyway.
-template <typename MatT, bool PML, int KC, typename Off>
+template <typename MatT, bool PML, int KC, typename Off, bool NT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KC == 0 ? 8 : 1))) void k_update_H_mon(
     YeeArgs a, PmlArgs pm) {
-    update_H_block<MatT, PML, true, KC, Off>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
+    update_H_block<MatT, PML, true, KC, Off, NT>(a, pm, xcd_logical_block(blockIdx.x, gridDim.x));
 }
 
 struct FaceArgs {
@@ -206,27 +207,34 @@ PmlArgs pml_args(fdtd_dev *d) {
     return pm;
 }
 
-// Off: cell-offset type of the kernels (uint32_t for grids below 2^30 cells, else long long)
+// Off: cell-offset type of the kernels (uint32_t for grids below 2^30 cells, else long long).
+// nt: streaming loads of the own-cell fields (PML grids beyond the last-level cache only).
 template <int KC, typename Off>
-void launch_split(bool e_field, bool mon, bool u8, bool pml, dim3 grid, hipStream_t st, hipEvent_t t0, hipEvent_t t1,
-                  const YeeArgs &a, const PmlArgs &pm) {
+void launch_split(bool e_field, bool mon, bool u8, bool pml, bool nt, dim3 grid, hipStream_t st, hipEvent_t t0,
+                  hipEvent_t t1, const YeeArgs &a, const PmlArgs &pm) {
 #define FDTD_GO(KER) hipExtLaunchKernelGGL(KER, grid, dim3(kBlock), 0, st, t0, t1, 0, a, pm)
+#define FDTD_PML(KERN, MT)                                  \
+    do {                                                    \
+        if (nt) FDTD_GO((KERN<MT, true, KC, Off, true>));   \
+        else FDTD_GO((KERN<MT, true, KC, Off, false>));     \
+    } while (0)
     if (e_field) {
-        if (u8 && pml) FDTD_GO((k_update_E<uint8_t, true, KC, Off>));
-        else if (u8) FDTD_GO((k_update_E<uint8_t, false, KC, Off>));
-        else if (pml) FDTD_GO((k_update_E<uint16_t, true, KC, Off>));
-        else FDTD_GO((k_update_E<uint16_t, false, KC, Off>));
+        if (u8 && pml) FDTD_PML(k_update_E, uint8_t);
+        else if (u8) FDTD_GO((k_update_E<uint8_t, false, KC, Off, false>));
+        else if (pml) FDTD_PML(k_update_E, uint16_t);
+        else FDTD_GO((k_update_E<uint16_t, false, KC, Off, false>));
     } else if (!mon) {
-        if (u8 && pml) FDTD_GO((k_update_H<uint8_t, true, KC, Off>));
-        else if (u8) FDTD_GO((k_update_H<uint8_t, false, KC, Off>));
-        else if (pml) FDTD_GO((k_update_H<uint16_t, true, KC, Off>));
-        else FDTD_GO((k_update_H<uint16_t, false, KC, Off>));
+        if (u8 && pml) FDTD_PML(k_update_H, uint8_t);
+        else if (u8) FDTD_GO((k_update_H<uint8_t, false, KC, Off, false>));
+        else if (pml) FDTD_PML(k_update_H, uint16_t);
+        else FDTD_GO((k_update_H<uint16_t, false, KC, Off, false>));
     } else {
-        if (u8 && pml) FDTD_GO((k_update_H_mon<uint8_t, true, KC, Off>));
-        else if (u8) FDTD_GO((k_update_H_mon<uint8_t, false, KC, Off>));
-        else if (pml) FDTD_GO((k_update_H_mon<uint16_t, true, KC, Off>));
-        else FDTD_GO((k_update_H_mon<uint16_t, false, KC, Off>));
+        if (u8 && pml) FDTD_PML(k_update_H_mon, uint8_t);
+        else if (u8) FDTD_GO((k_update_H_mon<uint8_t, false, KC, Off, false>));
+        else if (pml) FDTD_PML(k_update_H_mon, uint16_t);
+        else FDTD_GO((k_update_H_mon<uint16_t, false, KC, Off, false>));
     }
+#undef FDTD_PML
 #undef FDTD_GO
 }
 
@@ -298,17 +306,18 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
     if (mon) d->n_run_mon = 0;  // consumed
     dim3 grid((unsigned)(a.plane_blocks * zchunks));
     const bool u8 = d->mat_bytes == 1;
+    const bool nt = pml && streaming_loads(d->N, d->stream_mode);
     hipEvent_t t0, t1;
     timing_events(d, e_field ? 0 : 1, &t0, &t1);
     if (d->N < kSmallGridCells) {
         switch (a.kchunk) {
-            case 2: launch_split<2, uint32_t>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
-            case 3: launch_split<3, uint32_t>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
-            case 4: launch_split<4, uint32_t>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
-            default: launch_split<0, uint32_t>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm); break;
+            case 2: launch_split<2, uint32_t>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            case 3: launch_split<3, uint32_t>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            case 4: launch_split<4, uint32_t>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            default: launch_split<0, uint32_t>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm); break;
         }
     } else {
-        launch_split<0, long long>(e_field, mon, u8, pml, grid, d->stream, t0, t1, a, pm);
+        launch_split<0, long long>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm);
     }
     FDTD_TRY(hipGetLastError());
     return 0;

@@ -76,6 +76,11 @@ int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt);
  * rows = 16 or 32 the fused step in LDS tiles of 8 or 16 rows and kchunk z planes per block.
  * Default: the split pair with automatic chunking. */
 int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk);
+/* Cache policy of the split updates under the PML: the loads of each cell's own old field may
+ * carry the streaming (non-temporal) hint, which pays off once the fields exceed the last-level
+ * cache and costs grids that fit it.  mode -1 (default): on when the six field arrays exceed
+ * 240 MB; 0 off; 1 on. */
+int fdtd_dev_set_streaming_loads(fdtd_dev *d, int mode);
 
 /* Device-resident run (extension).  A run holds mode sources whose per-step amplitude and
  * phase are tabulated up front, FP64 mode-overlap monitors with a running DFT at nfreq angular
