@@ -363,3 +363,28 @@ def test_nonfinite_fields_are_detected():
     fd.dev_check(L().fdtd_dev_count_nonfinite(g.d, C.byref(n)), "count")
     assert n.value > 100  # the NaN has spread through the stencil
     g.free()
+
+
+def test_streaming_loads_change_nothing_but_speed():
+    """The split updates with and without the streaming hint on the own-cell loads (forced on and
+    off on a small grid): identical fields after a CPML run with a plane source."""
+    say("streaming loads on / off")
+    I, J, K, disc = 34, 26, 30, 0.02
+    dt = 0.99 * disc / np.sqrt(3)
+    rng = np.random.default_rng(8)
+    ep = rng.uniform(1, 12, 3 * I * J * K).astype(np.float32)
+    f0 = rng.uniform(-1, 1, 6 * I * J * K)
+    out = []
+    for mode in (0, 1):
+        g = fd.DevGrid(I, J, K, disc, ep)
+        g.cpml((4, 3, 5, 2, 3, 4), 3, 1e-6, 1.5, 0.1)
+        fd.dev_check(L().fdtd_dev_set_streaming_loads(g.d, mode), "streaming")
+        g.set_fields(f0)
+        g.run(dt, 0, 25)
+        out.append(g.fields())
+        g.free()
+    assert np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[1])
+    g = fd.DevGrid(I, J, K, disc, ep)
+    assert L().fdtd_dev_set_streaming_loads(g.d, 2) == -2
+    g.free()
