@@ -286,7 +286,7 @@ def adjoint_optimize(family, p0, iters, spec, step=0.1):
 
 
 class SurrogateLoop:
-    """The design loop with the surrogate inside it, full FDTD as the judge and the fallback.
+    """The design loop with the surrogate inside it, full FDTD as the arbiter and the fallback.
 
     Each iteration
       1. draws `pool` fresh candidates and ranks them with the surrogate;

@@ -686,8 +686,9 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the path would shard by geometry or by x-slabs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported (the path shards by "
+              "geometry for batched runs or by z slabs with halo planes for one large grid, "
+              "fdtd_dev_copy_planes; no multi-GPU run exists)", file=sys.stderr)
         sys.exit(2)
     fd = importlib.import_module("3d_fdtd_amd")
     L = fd.lib()

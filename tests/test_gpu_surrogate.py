@@ -4,7 +4,7 @@ network gains over the method without it.
 768 random spline tapers (reduced family: 50 nm cells, 48 x 20 cross-section, 1.5-4 um long) are
 simulated by batched FDTD; the BF16-MFMA surrogate is trained on them on the GPU.  From a pool of
 8192 new candidates, the 48 the surrogate ranks highest and 48 picked without it (the first 48 of
-the pool) are verified by full FDTD, which is the judge of every design.  With the same FDTD
+the pool) are verified by full FDTD, which decides every design.  With the same FDTD
 budget, the surrogate's picks must transmit clearly more (mean band |t|^2 higher by >= 0.1) and
 contain at least twice as many designs above the spec |t|^2 >= 0.7.
 """
