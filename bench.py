@@ -510,6 +510,95 @@ def bench_shard(fd, args):
     print(json.dumps(out), flush=True)
 
 
+def dropin_setup(L, TF, complex_t, steps):
+    """The reference driver's forward-run set-up (3d_fdtd.c:30-92) on library L: 100 x 100 x 600
+    cells of 20 nm, SiO2 background with nitride / BPSG / SiCOH top layers, input taper and
+    waveguide, a 25-segment adjoint taper inserted at planes 150..400, output waveguide and taper.
+    Its mode files are not part of the reference, so Gaussian profiles (60 x 40 cells) of the same
+    role stand in for the launch, output and unwanted modes."""
+    disc, omega = 0.02, 2 * math.pi / 1.310
+    tf = L.timefield3d_new(100, 100, 600, disc)
+    L.timefield3d_set_background(tf, 1.447 ** 2, 1.0)
+    for ep, bottom in ((1.93 ** 2, 55), (1.47 ** 2, 59), (1.39 ** 2, 64)):
+        L.timefield3d_set_top_half_plane(tf, ep, 1.0, bottom)
+    L.timefield3d_set_taper(tf, 3.51 ** 2, 1.0, 50, 50, 41, 59, 0, 95, 46, 54)
+    L.timefield3d_set_waveguide(tf, 3.51 ** 2, 41, 59, 95, 150, 46, 54)
+    seg = 25
+    left = (C.c_int * (seg + 1))(*[41 - i * 9 // seg for i in range(seg + 1)])
+    right = (C.c_int * (seg + 1))(*[59 + i * 9 // seg for i in range(seg + 1)])
+    taper = L.adjoint3d_taper_new(3.51 ** 2, 1.0, seg, 150, 400, left, right, 46, 54, steps)
+    L.adjoint3d_taper_insert(tf, taper)
+    L.timefield3d_set_waveguide(tf, 3.51 ** 2, 32, 68, 400, 455, 46, 54)
+    L.timefield3d_set_taper(tf, 3.51 ** 2, 1.0, 32, 68, 50, 50, 455, 600, 46, 54)
+    modes = [L.mode_gaussian(60, 40, disc, complex_t(1.447 ** 2, 0), complex_t(1.0, 0), omega, w, 0)
+             for w in (6.0, 12.0, 9.0)]
+    return tf, taper, modes, omega, 0.99 * disc / math.sqrt(3)
+
+
+def dropin_steps(L, tf, taper, modes, omega, dt, complex_t, n0, n):
+    """n steps of the reference driver's forward loop (3d_fdtd.c:104-127), call for call."""
+    launch, output, unwanted = modes
+    acc = [complex_t(0, 0) for _ in range(4)]
+    for i in range(n0, n0 + n):
+        amp = math.exp(-(i - 300) ** 2 / (2 * 150.0 ** 2)) if i < 1000 else 0.0
+        phase = i * dt * omega
+        L.timefield3d_mode_source(tf, 50, 50, 125, launch, phase, amp, 0)
+        L.timefield3d_apply_timestep_omp_full(tf, dt)
+        L.timefield3d_update_fourier_fields_y_slice_omp(tf, phase, 50)
+        L.timefield3d_mode_overlap(tf, 50, 50, 130, launch, phase, 0, C.byref(acc[0]))
+        L.timefield3d_mode_overlap(tf, 50, 50, 130, launch, phase, 1, C.byref(acc[1]))
+        L.timefield3d_mode_overlap(tf, 50, 50, 425, output, phase, 0, C.byref(acc[2]))
+        L.timefield3d_mode_overlap(tf, 50, 50, 425, unwanted, phase, 0, C.byref(acc[3]))
+        L.adjoint3d_taper_store_forward_fields(taper, tf, i)
+    return acc
+
+
+def bench_dropin(fd, args):
+    """The reference driver's forward loop through the drop-in C surface, call for call: per step
+    a mode source, the full step (Mur faces), the y-slice DFT, four mode overlaps (each returns a
+    value, so the step synchronises four times) and the taper's forward-history store, on the
+    driver's 100 x 100 x 600 grid.  value = cell-updates/s of that loop (wall clock; the
+    per-step returns make host time part of it).  cpu_baseline: the same calls on the compiled
+    reference (OpenMP, host threads)."""
+    L = fd.lib()
+    W, S = 20, args.steps if args.steps != N_RUN else 300
+    tf, taper, modes, omega, dt = dropin_setup(L, fd.TimeField, fd.Complex, W + S)
+    dropin_steps(L, tf, taper, modes, omega, dt, fd.Complex, 0, W)
+    t0 = time.perf_counter()
+    acc = dropin_steps(L, tf, taper, modes, omega, dt, fd.Complex, W, S)
+    el = time.perf_counter() - t0
+    cells = 100 * 100 * 600
+    print(f"bench: drop-in loop {el / S * 1e3:.3f} ms/step, |launched| {abs(complex(acc[0].re, acc[0].im)):.3e}",
+          file=sys.stderr, flush=True)
+    out = {"metric": METRIC, "value": cells * S / el, "unit": "cell-updates/s", "n_gpus": 1, "steps": S, "warmup": W,
+           "ms_per_step": el / S * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 fields, f64 monitors", "data": "synthetic",
+           "config": {"workload": "the reference driver's forward loop (3d_fdtd.c:104-127) through the drop-in C "
+                                  "surface: 100x100x600 cells of 20 nm, Mur faces, mode source, y-slice DFT, four "
+                                  "mode overlaps and the taper history store per step; Gaussian stand-ins for the "
+                                  "driver's mode files"}}
+    if not args.no_cpu:
+        from oracle import ref_py
+
+        if ref_py.available():
+            threads = int(os.environ.get("FDTD3D_CPU_THREADS", min(16, os.cpu_count() or 1)))
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            R = ref_py.lib()
+            rtf, rtaper, rmodes, omega, dt = dropin_setup(R, ref_py.TimeField, ref_py.Complex, 2000)
+            dropin_steps(R, rtf, rtaper, rmodes, omega, dt, ref_py.Complex, 0, 2)
+            t0 = time.perf_counter()
+            n = 0
+            while time.perf_counter() - t0 < 12.0 and n < 2000 - 2:
+                dropin_steps(R, rtf, rtaper, rmodes, omega, dt, ref_py.Complex, 2 + n, 1)
+                n += 1
+            cel = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": cells * n / cel, "unit": "cell-updates/s", "cores": threads,
+                                   "kind": "reference",
+                                   "sample": f"{n} steps of the same call sequence on the compiled reference "
+                                             f"(FP64, OpenMP), {cel:.1f} s, OMP_NUM_THREADS={threads}"}
+    print(json.dumps(out), flush=True)
+
+
 def bench_surrogate(fd, args):
     """The transmission surrogate's hand-written BF16 matrix-core kernels: full-batch Adam steps on
     4096 samples and screening of 10^5 candidates (the design loop's shapes: 9 -> 256 x 4 -> 32,
@@ -666,13 +755,15 @@ def main():
     ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
     ap.add_argument("--kchunk", type=int, default=None,
                     help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
-    ap.add_argument("--workload", choices=("waveguide", "tapers", "design", "surrogate", "gradient", "shard"),
+    ap.add_argument("--workload", choices=("waveguide", "tapers", "design", "surrogate", "gradient", "shard",
+                                           "dropin"),
                     default="waveguide",
                     help="waveguide: the strip waveguide (default); tapers: the batched taper set; design: "
                          "verified taper designs/hour, surrogate loop vs plain adjoint loop; surrogate: the MLP's "
                          "matrix-core training and screening kernels; gradient: the 620x200x100 taper's forward + "
                          "adjoint runs and DFT eps-gradient; shard: one GPU's z-slab share of that taper with "
-                         "its halo planes")
+                         "its halo planes; dropin: the reference driver's per-step call sequence through the "
+                         "drop-in C surface")
     ap.add_argument("--train", type=int, default=4096, help="design workload: FDTD training designs")
     ap.add_argument("--fit-steps", type=int, default=1500, help="design workload: surrogate training steps")
     ap.add_argument("--campaign", type=float, default=150.0,
@@ -708,6 +799,9 @@ def main():
         return
     if args.workload == "shard":
         bench_shard(fd, args)
+        return
+    if args.workload == "dropin":
+        bench_dropin(fd, args)
         return
     W, S = args.warmup, args.steps
     g, dt, md, mons, omegas = workload(fd, W + S + min(400, max(S // 4, 20)))
