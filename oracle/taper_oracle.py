@@ -13,10 +13,11 @@ from oracle import oracle_py as O
 
 
 def transmission(ep, I, J, K, disc, dt, npml, mode, mi, mj, amp, phase, omegas, src_k, mon_ks,
-                 cpml=dict(order=3, R0=1e-8, kappa_max=1.0, alpha_max=0.05)):
+                 cpml=dict(order=3, R0=1e-8, kappa_max=1.0, alpha_max=0.05), out_mode=None):
     """Complex t(omega) = a(mon_ks[-1]) / a(mon_ks[0]) of the design with permittivity ep
     ([K][J][I][3] float32, flattened); mode is the power-normalised interleaved complex
-    [mj][mi][6] window centred on (I/2, J/2)."""
+    [mj][mi][6] window centred on (I/2, J/2), used by the source and every monitor unless
+    out_mode = (mode, mi, mj) gives the last monitor its own (a port of another width)."""
     g = O.Grid(I, J, K, disc)
     g.ep[:] = ep
     cp = O.Cpml(g, (npml,) * 6, dt, **cpml)
@@ -26,10 +27,14 @@ def transmission(ep, I, J, K, disc, dt, npml, mode, mi, mj, amp, phase, omegas, 
     acc = [np.zeros(len(omegas), complex) for _ in mon_ks]
     cum, inst = np.zeros(2), np.zeros(2)
     L = O.lib()
+    mons = [(mode, mi, mj)] * len(mon_ks)
+    if out_mode is not None:
+        mons[-1] = (np.array(out_mode[0], np.float64), out_mode[1], out_mode[2])
     for n in range(len(amp)):
         L.orc_mode_source(g.ptr, ci, cj, src_k, mi, mj, disc, O.dptr(mode), float(phase[n]), float(amp[n]), 0)
         O.step_cpml(g, cp, dt)
         for q, kz in enumerate(mon_ks):
-            L.orc_mode_overlap(g.ptr, ci, cj, kz, mi, mj, disc, O.dptr(mode), 0.0, 0, O.dptr(cum), O.dptr(inst))
+            m, a, b = mons[q]
+            L.orc_mode_overlap(g.ptr, ci, cj, kz, a, b, disc, O.dptr(m), 0.0, 0, O.dptr(cum), O.dptr(inst))
             acc[q] += np.exp(1j * omegas * n * dt) * complex(inst[0], inst[1])
     return acc[-1] / acc[0]
