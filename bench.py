@@ -79,6 +79,7 @@ def workload(fd, steps_total):
     src = L.fdtd3d_ext_run_add_mode_source(g.tf, ci, cj, 20, md, 0, 0, steps_total, amp.ctypes.data_as(dptr),
                                            phase.ctypes.data_as(dptr))
     fd.check(0 if src >= 0 else src, "add source")
+    g.amp, g.phase = amp, phase  # the source schedule (the full-size transmission tests reuse it)
     omegas = 2 * math.pi / np.linspace(1.50, 1.60, 16)
     mons = []
     for kz, rev in ((25, 0), (25, 1), (K - 20, 0)):
@@ -396,17 +397,6 @@ def bench_gradient(fd, args):
                         "dT_dw_per_um": [round(float(v), 6) for v in dT_dw],
                         "fd_check_w4": {"adjoint": round(float(dT_dw[3]), 6), "central_difference": round(float(fd_w4), 6),
                                         "delta_um": d}}}
-    if args.oracle:
-        # the forward run's transmission from the FP64 oracle on the same inputs (CPU, minutes)
-        from oracle import taper_oracle
-
-        print("bench: FP64 oracle forward run", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        t_ref = taper_oracle.transmission(ep, I, J, K, fam.disc, dt, fam.npml, m_in, mi_in, mj_in, fam.amp,
-                                          fam.phase, om, ks, (ks + 3, km), out_mode=(m_out, mi_out, mj_out))[0]
-        out["gradient"]["fp64_oracle"] = {"t_gpu": [t.real, t.imag], "t_oracle": [t_ref.real, t_ref.imag],
-                                          "abs_dt": abs(t - t_ref), "abs_dT": abs(abs(t) ** 2 - abs(t_ref) ** 2),
-                                          "oracle_s": round(time.perf_counter() - t0, 1)}
     if not args.no_cpu:
         out["cpu_baseline"] = gradient_cpu_baseline(fam, ep, K)
     print(json.dumps(out), flush=True)
@@ -763,8 +753,6 @@ def main():
     ap.add_argument("--steps", type=int, default=N_RUN)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--oracle", action="store_true",
-                    help="gradient workload: also run the forward transmission on the FP64 oracle (CPU, minutes)")
     ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
     ap.add_argument("--kchunk", type=int, default=None,
                     help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
