@@ -177,7 +177,11 @@ def bench_tapers(fd, args):
     cells = info["cells"]
     value = cells * S / elapsed_s
     dom = max(e_us, h_us)
+    dom_name = "batch_update_E" if e_us >= h_us else "batch_update_H"
     achieved = SPLIT_KERNEL_BYTES * cells / (dom * 1e-6) / 1e9 if dom > 0 else 0.0
+    # HBM bytes per launch from the rocprofv3 counter runs of this workload (copy-calibrated);
+    # they belong to the 64-taper set only
+    traffic = read_traffic("tapers_batch_traffic.json") if args.tapers == 64 else None
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": S, "warmup": W,
            "ms_per_step": elapsed_s * 1e3 / S, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
@@ -188,8 +192,9 @@ def bench_tapers(fd, args):
                       "grids": args.tapers, "cells_total": cells,
                       "lengths_cells": [int(round(x / DISC)) + 20 for x in lengths]},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                        "kernel": "batch_update_E" if e_us >= h_us else "batch_update_H",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": traffic.get(dom_name) if traffic else None,
+                        "kernel": dom_name,
                         "kernel_avg_us": round(dom, 1), "e_us": round(e_us, 1), "h_us": round(h_us, 1),
                         "timed_launches": int(min(cnt[0], cnt[1])), "kernel_timing_window_steps": KW,
                         "algorithmic_bytes_per_launch": SPLIT_KERNEL_BYTES * cells,
