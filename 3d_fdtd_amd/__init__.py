@@ -186,6 +186,8 @@ SIGNATURES = {
                                         _i),
     "fdtd_dev_update_E_cpml": ([C.c_void_p, _d], _i),
     "fdtd_dev_update_H_cpml": ([C.c_void_p, _d], _i),
+    "fdtd_dev_step": ([C.c_void_p, _d, _i], _i),
+    "fdtd_dev_set_source_plane": ([C.c_void_p, _i, _P(_f), _P(C.c_ubyte)], _i),
     "fdtd_dev_set_finite_check": ([C.c_void_p, _i], _i),
     "fdtd_dev_cpml_on": ([C.c_void_p, _P(_i), _i, _d, _d, _d], _i),
     "fdtd_dev_set_step_kernel": ([C.c_void_p, _i, _i], _i),
