@@ -12,7 +12,7 @@ the whole plane so that those faces take part.  The 16 planes around the large g
 pulled into a third small grid with fdtd_dev_copy_planes, must therefore equal the small grid's
 fields: the same cells see the same inputs and the same arithmetic, only the offsets differ (the
 small grid's unrolled two-plane variant may round differently from the large grid's looped one,
-hence a 1e-6 relative L2 bound rather than bit equality).
+hence a 1e-6 relative L2 bound rather than bit equality)."""
 import ctypes as C
 import importlib
 import sys
