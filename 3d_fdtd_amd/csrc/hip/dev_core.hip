@@ -194,6 +194,9 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->fourier = nullptr;
     d->partials = nullptr;
     d->red_out = nullptr;
+    d->ovl_count = nullptr;
+    d->ovl_host = nullptr;
+    d->ovl_dev = nullptr;
     d->partial_cap = 0;
     d->use_clock = 0;
     d->field_base = nullptr;
@@ -227,6 +230,10 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     if (e == hipSuccess) e = dev_malloc(&d->shell, (size_t)off * sizeof(float));
     if (e == hipSuccess) e = hipMemsetAsync(d->shell, 0, (size_t)off * sizeof(float), d->stream);
     if (e == hipSuccess) e = dev_malloc(&d->red_out, sizeof(double2) * 4);
+    if (e == hipSuccess) e = dev_malloc(&d->ovl_count, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemsetAsync(d->ovl_count, 0, sizeof(unsigned), d->stream);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&d->ovl_host, sizeof(double2), hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&d->ovl_dev, d->ovl_host, 0);
     if (e == hipSuccess) e = hipEventCreate(&d->ev0);
     if (e == hipSuccess) e = hipEventCreate(&d->ev1);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&d->xev, hipEventDisableTiming);
@@ -255,6 +262,8 @@ void fdtd_dev_destroy(fdtd_dev *d) {
     (void)dev_free(d->finite_buf);
     (void)dev_free(d->partials);
     (void)dev_free(d->red_out);
+    (void)dev_free(d->ovl_count);
+    if (d->ovl_host) (void)hipHostFree(d->ovl_host);
     for (auto &p : d->src) (void)dev_free(p.J);
     for (auto &m : d->modes) (void)dev_free(m.dev);
     if (d->ev0) (void)hipEventDestroy(d->ev0);

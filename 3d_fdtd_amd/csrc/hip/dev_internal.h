@@ -192,6 +192,9 @@ struct fdtd_dev {
     double2 *partials;
     double2 *red_out;
     int partial_cap;
+    unsigned *ovl_count;  // last-block counter of k_overlap (zero between launches)
+    double2 *ovl_host;    // pinned, coherent host memory the overlap result lands in
+    double2 *ovl_dev;     // its device address
 
     std::vector<fdtd::ModeCache> modes;
     unsigned long long use_clock;

@@ -72,11 +72,12 @@ __global__ void k_plane_masked_copy(float *plane, long long cs, const float *src
     for (int m = 0; m < 6; m++) plane[m * cs + p] = src[m * P + p];
 }
 
-// Overlap stage 1: per-block FP64 partial sums of Hx mEy - Hy mEx + Ey mHx - Ex mHy.
-__global__ __launch_bounds__(kBlock) void k_overlap_partial(const float *Ex, const float *Ey, const float *Hx,
-                                                            const float *Hy, int I, int J, long long P, int cz,
-                                                            int cx, int cy, const double *mode, int mi, int mj,
-                                                            double ratio, int reverse, double2 *partials) {
+// Overlap stage 1 (one block): the block's FP64 partial sum of Hx mEy - Hy mEx + Ey mHx - Ex mHy,
+// stored in partials[blockIdx.x] by thread 0.
+__device__ __forceinline__ void overlap_block_partial(const float *Ex, const float *Ey, const float *Hx,
+                                                      const float *Hy, int I, int J, long long P, int cz, int cx,
+                                                      int cy, const double *mode, int mi, int mj, double ratio,
+                                                      int reverse, double2 *partials) {
     __shared__ double sre[kBlock / 64], sim[kBlock / 64];
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double re = 0.0, im = 0.0;
@@ -117,9 +118,8 @@ __global__ __launch_bounds__(kBlock) void k_overlap_partial(const float *Ex, con
     }
 }
 
-// Overlap stage 2: one block sums the partials in a fixed order and scales by -dx^2/2.
-__global__ __launch_bounds__(kBlock) void k_overlap_final(const double2 *partials, int n, double scale,
-                                                          double2 *out) {
+// Overlap stage 2 (one block): sums the n partials in a fixed order and writes scale * sum.
+__device__ __forceinline__ void overlap_block_final(const double2 *partials, int n, double scale, double2 *out) {
     __shared__ double sre[kBlock], sim[kBlock];
     double r = 0, s = 0;
     for (int q = threadIdx.x; q < n; q += kBlock) {
@@ -137,6 +137,30 @@ __global__ __launch_bounds__(kBlock) void k_overlap_final(const double2 *partial
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = make_double2(scale * sre[0], scale * sim[0]);
+}
+
+// The plane overlap in one launch: every block stores its partial; the last block to finish (a
+// counter in device memory, reset by that block for the next launch) sums them in a fixed order,
+// so the result does not depend on which block finishes last.  `out` may be host memory.
+__global__ __launch_bounds__(kBlock) void k_overlap(const float *Ex, const float *Ey, const float *Hx,
+                                                    const float *Hy, int I, int J, long long P, int cz, int cx,
+                                                    int cy, const double *mode, int mi, int mj, double ratio,
+                                                    int reverse, double2 *partials, unsigned *count, double scale,
+                                                    double2 *out) {
+    overlap_block_partial(Ex, Ey, Hx, Hy, I, J, P, cz, cx, cy, mode, mi, mj, ratio, reverse, partials);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's partial before its count
+        last = atomicAdd(count, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every other block's partial after their counts
+    overlap_block_final(partials, (int)gridDim.x, scale, out);
+    if (threadIdx.x == 0) {
+        *count = 0u;
+        __threadfence_system();
+    }
 }
 
 // F += f * twiddle over a z range, optionally one y row (timefield3d.c:2851-2975).
@@ -331,17 +355,16 @@ int fdtd_dev_mode_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mod
         d->partial_cap = nb;
     }
     double ratio = d->disc / mdisc;
-    hipLaunchKernelGGL(k_overlap_partial, dim3(nb), dim3(kBlock), 0, d->stream, d->F[EX], d->F[EY], d->F[HX],
-                       d->F[HY], d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, reverse, d->partials);
+    // one launch writing straight into pinned host memory: the reference returns the overlap from
+    // the call, so each call waits for its own result
+    hipLaunchKernelGGL(k_overlap, dim3(nb), dim3(kBlock), 0, d->stream, d->F[EX], d->F[EY], d->F[HX], d->F[HY], d->I,
+                       d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, reverse, d->partials, d->ovl_count,
+                       -0.5 * d->disc * d->disc, d->ovl_dev);
     FDTD_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(kBlock), 0, d->stream, d->partials, nb,
-                       -0.5 * d->disc * d->disc, d->red_out);
-    FDTD_TRY(hipGetLastError());
-    double2 r;
-    FDTD_TRY(hipMemcpyAsync(&r, d->red_out, sizeof r, hipMemcpyDeviceToHost, d->stream));
     FDTD_TRY(hipStreamSynchronize(d->stream));
-    out[0] = r.x;
-    out[1] = r.y;
+    const volatile double2 *r = d->ovl_host;
+    out[0] = r->x;
+    out[1] = r->y;
     return 0;
 }
 
