@@ -49,6 +49,16 @@ __device__ __forceinline__ long long cell_of(const int x[3], const int N[3]) {
     return (long long)x[0] + (long long)N[0] * ((long long)x[1] + (long long)N[1] * x[2]);
 }
 
+// Enumeration axes of face a: ua runs fastest in the shell and across neighbouring threads, so it
+// is the face's tangential axis that is fastest in memory (x on the y faces, where the cyclic
+// order would put z first and make every thread touch another plane); va is the other one.  The
+// shell keeps the two tangential components in the cyclic order b = (a + 1) % 3, cc = (a + 2) % 3
+// whatever the enumeration.
+__host__ __device__ __forceinline__ void face_enum_axes(int a, int *ua, int *va) {
+    *ua = a == 1 ? 0 : (a + 1) % 3;
+    *va = a == 1 ? 2 : (a + 2) % 3;
+}
+
 // Copy the tangential components of the two outermost layers of every face into the shell
 // buffer (timefield3d.c:246-346 copies all six; only the tangential pairs are ever read back).
 __global__ void k_snapshot(FaceArgs fa, long long total, long long a0, long long a1, long long a2) {
@@ -62,13 +72,15 @@ __global__ void k_snapshot(FaceArgs fa, long long total, long long a0, long long
         f++;
     }
     const int a = f % 3, side = f / 3, b = (a + 1) % 3, cc = (a + 2) % 3;
-    const int layer = (int)(r / area[a]);
-    const long long uv = r - layer * area[a];
-    const int u = (int)(uv % fa.N[b]), v = (int)(uv / fa.N[b]);
+    int ua, va;
+    face_enum_axes(a, &ua, &va);
+    // the two layers of one face cell are neighbouring threads (neighbouring words on x faces)
+    const int layer = (int)(r & 1);
+    const long long uv = r >> 1;
     int x[3];
     x[a] = side == 0 ? layer : fa.N[a] - 1 - layer;
-    x[b] = u;
-    x[cc] = v;
+    x[ua] = (int)(uv % fa.N[ua]);
+    x[va] = (int)(uv / fa.N[ua]);
     const long long c = cell_of(x, fa.N);
     float *sh = fa.shell + fa.shell_off[f] + (long long)layer * 4 * area[a] + uv;
     sh[0 * area[a]] = fa.F[b][c];
@@ -82,11 +94,13 @@ __global__ void k_snapshot(FaceArgs fa, long long total, long long a0, long long
 template <typename MatT>
 __global__ void k_mur_face(FaceArgs fa, int a, int side, int field, int k0, int k1) {
     const int b = (a + 1) % 3, cc = (a + 2) % 3;
+    int ua, va;
+    face_enum_axes(a, &ua, &va);
     const long long area = (long long)fa.N[b] * fa.N[cc];
     // enumerate only the z-restricted part of the face
-    int ulo = 0, uhi = fa.N[b], vlo = 0, vhi = fa.N[cc];
-    if (b == 2) { ulo = k0; uhi = k1; }
-    if (cc == 2) { vlo = k0; vhi = k1; }
+    int ulo = 0, uhi = fa.N[ua], vlo = 0, vhi = fa.N[va];
+    if (ua == 2) { ulo = k0; uhi = k1; }
+    if (va == 2) { vlo = k0; vhi = k1; }
     const long long nu = uhi - ulo;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nu * (vhi - vlo)) return;
@@ -94,10 +108,10 @@ __global__ void k_mur_face(FaceArgs fa, int a, int side, int field, int k0, int 
     int xb[3], xi[3];
     xb[a] = side == 0 ? 0 : fa.N[a] - 1;
     xi[a] = side == 0 ? 1 : fa.N[a] - 2;
-    xb[b] = xi[b] = u;
-    xb[cc] = xi[cc] = v;
+    xb[ua] = xi[ua] = u;
+    xb[va] = xi[va] = v;
     const long long cb = cell_of(xb, fa.N), ci = cell_of(xi, fa.N);
-    const long long uv = u + (long long)fa.N[b] * v;
+    const long long uv = u + (long long)fa.N[ua] * v;
     const int f = a + 3 * side;
     const float *sh0 = fa.shell + fa.shell_off[f] + uv;          // boundary layer
     const float *sh1 = sh0 + 4 * area;                          // inner layer
@@ -405,16 +419,18 @@ int fdtd_dev_set_shells(fdtd_dev *d, const double *old) {
     std::vector<float> sh((size_t)total);
     for (int f = 0; f < 6; f++) {
         int a = f % 3, side = f / 3, b = (a + 1) % 3, cc = (a + 2) % 3;
+        int ua, va;
+        face_enum_axes(a, &ua, &va);
         long long area = (long long)N[b] * N[cc];
         for (int layer = 0; layer < 2; layer++)
-            for (int v = 0; v < N[cc]; v++)
-                for (int u = 0; u < N[b]; u++) {
+            for (int v = 0; v < N[va]; v++)
+                for (int u = 0; u < N[ua]; u++) {
                     int x[3];
                     x[a] = side == 0 ? layer : N[a] - 1 - layer;
-                    x[b] = u;
-                    x[cc] = v;
+                    x[ua] = u;
+                    x[va] = v;
                     long long c = x[0] + (long long)N[0] * (x[1] + (long long)N[1] * x[2]);
-                    float *s = &sh[d->shell_off[f] + layer * 4 * area + u + (long long)N[b] * v];
+                    float *s = &sh[d->shell_off[f] + layer * 4 * area + u + (long long)N[ua] * v];
                     s[0] = (float)old[6 * c + b];
                     s[area] = (float)old[6 * c + cc];
                     s[2 * area] = (float)old[6 * c + 3 + b];
