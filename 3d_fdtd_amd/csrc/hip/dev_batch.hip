@@ -228,6 +228,7 @@ int set_step(fdtd_dev_batch *b, int n0) {
 // Steps n0..n0+n-1 of every grid (n <= kBatchChunk); the first call of an untuned batch spends its
 // first steps timing each chunk-depth candidate (real steps: results do not depend on the depth).
 int batch_chunk(fdtd_dev_batch *b, double dt, int n0, int n) {
+    for (auto *d : b->grids) fields_changed(d);
     for (auto *d : b->grids) FDTD_TRY(hipStreamSynchronize(d->stream));
     b->mon = false;
     for (size_t g = 0; g < b->grids.size(); g++) {

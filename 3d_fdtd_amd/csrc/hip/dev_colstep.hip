@@ -361,6 +361,7 @@ void launch_col(bool mon, bool u8, dim3 grid, hipStream_t st, hipEvent_t t0, hip
 bool col_supported(const fdtd_dev *d) { return d->cpml.on && d->N < kSmallGridCells && d->K >= 3; }
 
 int col_step(fdtd_dev *d, double dt, int kc) {
+    fields_changed(d);
     if (kc != 1 && kc != 2 && kc != 4) {
         set_error("column step: z planes per thread must be 1, 2 or 4");
         return -2;

@@ -130,6 +130,7 @@ const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, do
         mc.last_use = d->use_clock;
         if (std::memcmp(mode, mc.shadow.data(), n * sizeof(double)) == 0) return mc.dev;
         std::memcpy(mc.shadow.data(), mode, n * sizeof(double));
+        mc.version = ++d->mode_uploads;
         hipError_t e = hipMemcpyAsync(mc.dev, mode, n * sizeof(double), hipMemcpyHostToDevice, d->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
         if (e != hipSuccess) {
@@ -153,6 +154,7 @@ const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, do
     mc.mdisc = mdisc;
     mc.shadow.assign(mode, mode + n);
     mc.last_use = d->use_clock;
+    mc.version = ++d->mode_uploads;
     hipError_t e = dev_malloc(&mc.dev, n * sizeof(double));
     if (e == hipSuccess) e = hipMemcpyAsync(mc.dev, mode, n * sizeof(double), hipMemcpyHostToDevice, d->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
@@ -162,6 +164,12 @@ const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, do
     }
     d->modes.push_back(std::move(mc));
     return d->modes.back().dev;
+}
+
+const ModeCache *mode_cached(const fdtd_dev *d, const double *mode, int mi, int mj, double mdisc) {
+    for (const auto &mc : d->modes)
+        if (mc.host_key == mode && mc.mi == mi && mc.mj == mj && mc.mdisc == mdisc) return &mc;
+    return nullptr;
 }
 
 }  // namespace fdtd
@@ -197,6 +205,10 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->ovl_count = nullptr;
     d->ovl_host = nullptr;
     d->ovl_dev = nullptr;
+    d->field_epoch = 1;
+    d->mode_uploads = 0;
+    d->n_seen = d->n_prev = d->n_batch = 0;
+    d->seen_epoch = d->batch_epoch = 0;
     d->partial_cap = 0;
     d->use_clock = 0;
     d->field_base = nullptr;
@@ -230,9 +242,9 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     if (e == hipSuccess) e = dev_malloc(&d->shell, (size_t)off * sizeof(float));
     if (e == hipSuccess) e = hipMemsetAsync(d->shell, 0, (size_t)off * sizeof(float), d->stream);
     if (e == hipSuccess) e = dev_malloc(&d->red_out, sizeof(double2) * 4);
-    if (e == hipSuccess) e = dev_malloc(&d->ovl_count, sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemsetAsync(d->ovl_count, 0, sizeof(unsigned), d->stream);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&d->ovl_host, sizeof(double2), hipHostMallocCoherent);
+    if (e == hipSuccess) e = dev_malloc(&d->ovl_count, kOvlSlots * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemsetAsync(d->ovl_count, 0, kOvlSlots * sizeof(unsigned), d->stream);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&d->ovl_host, kOvlSlots * sizeof(double2), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&d->ovl_dev, d->ovl_host, 0);
     if (e == hipSuccess) e = hipEventCreate(&d->ev0);
     if (e == hipSuccess) e = hipEventCreate(&d->ev1);
@@ -408,6 +420,7 @@ int fdtd_dev_set_materials_extruded(fdtd_dev *d, int nmat, const float *ep, cons
 }
 
 int fdtd_dev_set_fields(fdtd_dev *d, const double *fields) {
+    fields_changed(d);
     d->alt_valid = false;
     double *tmp = nullptr;
     FDTD_TRY(dev_malloc(&tmp, (size_t)6 * d->N * sizeof(double)));
@@ -496,6 +509,7 @@ int fdtd_dev_get_sources(fdtd_dev *d, float *sources) {
 }
 
 int fdtd_dev_zero(fdtd_dev *d) {
+    fields_changed(d);
     FDTD_TRY(hipMemsetAsync(d->field_base, 0, (size_t)6 * d->Nal * sizeof(float), d->stream));
     if (d->alt_base) {
         FDTD_TRY(hipMemsetAsync(d->alt_base, 0, (size_t)6 * d->Nal * sizeof(float), d->stream));

@@ -612,6 +612,7 @@ bool fused_supported(const fdtd_dev *d) {
 }
 
 int fused_step(fdtd_dev *d, double dt, int ty_rows, int kc_len) {
+    fields_changed(d);
     int rc = ensure_coefficients(d, dt);
     if (rc) return rc;
     if (d->cpml.on && (rc = cpml_tables(d, dt))) return rc;

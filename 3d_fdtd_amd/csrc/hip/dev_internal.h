@@ -100,7 +100,17 @@ struct ModeCache {
     std::vector<double> shadow;  // last uploaded content
     double *dev = nullptr;       // 12*mi*mj doubles (interleaved complex)
     unsigned long long last_use = 0;
+    unsigned long long version = 0;  // stamp of the last upload (unique per grid)
 };
+
+// One plane-overlap request of the drop-in surface (timefield3d_mode_overlap's arguments).
+struct OvlReq {
+    int cx, cy, cz, mi, mj, reverse;
+    double mdisc;
+    const double *host_key;
+};
+// Overlap requests evaluated by one launch batch (one synchronisation).
+constexpr int kOvlSlots = 8;
 
 struct TaperHist {
     const void *key;
@@ -192,9 +202,18 @@ struct fdtd_dev {
     double2 *partials;
     double2 *red_out;
     int partial_cap;
-    unsigned *ovl_count;  // last-block counter of k_overlap (zero between launches)
-    double2 *ovl_host;    // pinned, coherent host memory the overlap result lands in
+    unsigned *ovl_count;  // last-block counters of k_overlap, one per slot (zero between launches)
+    double2 *ovl_host;    // pinned, coherent host memory the overlap results land in (one per slot)
     double2 *ovl_dev;     // its device address
+    // Overlap batching: the fields change only through calls that advance field_epoch.  The
+    // requests seen since the last change (ovl_seen) become the pattern (ovl_prev) that the first
+    // request after the next change evaluates along with itself; their results stay valid while
+    // field_epoch == batch_epoch and the mode content is unchanged (same upload version).
+    unsigned long long field_epoch, mode_uploads;
+    fdtd::OvlReq ovl_seen[fdtd::kOvlSlots], ovl_prev[fdtd::kOvlSlots], ovl_req[fdtd::kOvlSlots];
+    unsigned long long ovl_ver[fdtd::kOvlSlots];
+    int n_seen, n_prev, n_batch;
+    unsigned long long seen_epoch, batch_epoch;
 
     std::vector<fdtd::ModeCache> modes;
     unsigned long long use_clock;
@@ -252,6 +271,10 @@ bool step_folds_monitors(const fdtd_dev *d);
 SrcPlane *plane_for(fdtd_dev *d, int k, int *err);
 void run_release(fdtd_dev *d);
 const double *mode_on_device(fdtd_dev *d, const double *mode, int mi, int mj, double mdisc, int *err);
+// The cached device copy of a mode as last uploaded (no upload, no content check), or null.
+const ModeCache *mode_cached(const fdtd_dev *d, const double *mode, int mi, int mj, double mdisc);
+// Every call that may change the field values advances the grid's field epoch (overlap batching).
+inline void fields_changed(fdtd_dev *d) { d->field_epoch++; }
 
 // Blocks over (plane chunks x z chunks) remapped so that each XCD walks a contiguous range
 // (workgroups are dispatched round-robin over the 8 XCDs of an MI355X).

@@ -347,21 +347,67 @@ int fdtd_dev_mode_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mod
     int err;
     const double *dm = mode_on_device(d, mode, mi, mj, mdisc, &err);
     if (!dm) return err;
-    int nb = (int)((d->P + kBlock - 1) / kBlock);
-    if (nb > d->partial_cap) {
+    const unsigned long long ver = mode_cached(d, mode, mi, mj, mdisc)->version;
+    const OvlReq req{cx, cy, cz, mi, mj, reverse, mdisc, mode};
+    auto same = [](const OvlReq &x, const OvlReq &y) {
+        return x.cx == y.cx && x.cy == y.cy && x.cz == y.cz && x.mi == y.mi && x.mj == y.mj &&
+               x.reverse == y.reverse && x.mdisc == y.mdisc && x.host_key == y.host_key;
+    };
+    // the requests since the fields last changed form the pattern of the next batch
+    if (d->seen_epoch != d->field_epoch) {
+        if (d->n_seen > 0) {
+            for (int q = 0; q < d->n_seen; q++) d->ovl_prev[q] = d->ovl_seen[q];
+            d->n_prev = d->n_seen;
+        }
+        d->n_seen = 0;
+        d->seen_epoch = d->field_epoch;
+    }
+    bool seen = false;
+    for (int q = 0; q < d->n_seen; q++) seen = seen || same(d->ovl_seen[q], req);
+    if (!seen && d->n_seen < kOvlSlots) d->ovl_seen[d->n_seen++] = req;
+    // evaluated by the batch launched since the fields last changed, on the same mode content
+    if (d->batch_epoch == d->field_epoch)
+        for (int q = 0; q < d->n_batch; q++)
+            if (same(d->ovl_req[q], req) && d->ovl_ver[q] == ver) {
+                const volatile double2 *r = d->ovl_host + q;
+                out[0] = r->x;
+                out[1] = r->y;
+                return 0;
+            }
+    // a new batch: this request, then the other requests of the last pattern whose modes are
+    // still cached (their results wait for the calls that ask for them)
+    OvlReq batch[kOvlSlots];
+    const double *bmode[kOvlSlots];
+    unsigned long long bver[kOvlSlots];
+    int n = 0;
+    batch[n] = req, bmode[n] = dm, bver[n++] = ver;
+    for (int q = 0; q < d->n_prev && n < kOvlSlots; q++) {
+        const OvlReq &o = d->ovl_prev[q];
+        if (same(o, req)) continue;
+        const ModeCache *mc = mode_cached(d, o.host_key, o.mi, o.mj, o.mdisc);
+        if (!mc) continue;
+        batch[n] = o, bmode[n] = mc->dev, bver[n++] = mc->version;
+    }
+    const int nb = (int)((d->P + kBlock - 1) / kBlock);
+    if ((long long)nb * kOvlSlots > d->partial_cap) {
         (void)dev_free(d->partials);
         d->partials = nullptr;
-        FDTD_TRY(dev_malloc(&d->partials, (size_t)nb * sizeof(double2)));
-        d->partial_cap = nb;
+        FDTD_TRY(dev_malloc(&d->partials, (size_t)nb * kOvlSlots * sizeof(double2)));
+        d->partial_cap = nb * kOvlSlots;
     }
-    double ratio = d->disc / mdisc;
-    // one launch writing straight into pinned host memory: the reference returns the overlap from
-    // the call, so each call waits for its own result
-    hipLaunchKernelGGL(k_overlap, dim3(nb), dim3(kBlock), 0, d->stream, d->F[EX], d->F[EY], d->F[HX], d->F[HY], d->I,
-                       d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, reverse, d->partials, d->ovl_count,
-                       -0.5 * d->disc * d->disc, d->ovl_dev);
-    FDTD_TRY(hipGetLastError());
+    // one launch per overlap, each writing straight into its slot of pinned host memory, and one
+    // synchronisation for the batch: the reference returns each overlap from its own call
+    for (int q = 0; q < n; q++) {
+        const OvlReq &o = batch[q];
+        hipLaunchKernelGGL(k_overlap, dim3(nb), dim3(kBlock), 0, d->stream, d->F[EX], d->F[EY], d->F[HX], d->F[HY],
+                           d->I, d->J, d->P, o.cz, o.cx, o.cy, bmode[q], o.mi, o.mj, d->disc / o.mdisc, o.reverse,
+                           d->partials + (size_t)q * nb, d->ovl_count + q, -0.5 * d->disc * d->disc, d->ovl_dev + q);
+        FDTD_TRY(hipGetLastError());
+    }
     FDTD_TRY(hipStreamSynchronize(d->stream));
+    for (int q = 0; q < n; q++) d->ovl_req[q] = batch[q], d->ovl_ver[q] = bver[q];
+    d->n_batch = n;
+    d->batch_epoch = d->field_epoch;
     const volatile double2 *r = d->ovl_host;
     out[0] = r->x;
     out[1] = r->y;

@@ -21,6 +21,7 @@ int fdtd_dev_copy_planes(fdtd_dev *dst, int kd, fdtd_dev *src, int ks, unsigned 
         set_error("copy_planes: plane index outside the grid or empty component mask");
         return -2;
     }
+    fields_changed(dst);
     // the copy waits for the source's queued work; the source's later work waits for the copy
     FDTD_TRY(hipEventRecord(src->xev, src->stream));
     FDTD_TRY(hipStreamWaitEvent(dst->stream, src->xev, 0));

@@ -301,6 +301,7 @@ int base_args(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool pml, YeeA
 }
 
 int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_field, bool pml) {
+    fields_changed(d);
     k0 = k0 < 1 ? 1 : k0;
     k1 = k1 > d->K - 1 ? d->K - 1 : k1;
     if (k1 <= k0) return ensure_coefficients(d, dt);
@@ -338,6 +339,7 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
 }
 
 int launch_mur(fdtd_dev *d, double dt, int k0, int k1, int field) {
+    fields_changed(d);
     int rc = ensure_coefficients(d, dt);
     if (rc) return rc;
     d->alt_valid = false;  // wall cells change: the fused step's second buffer must be resynced
