@@ -262,6 +262,9 @@ double mode_power_norm(mode psi) { return sqrt(complex_mag(mode_power_inner_prod
 
 static void scale_fields(mode psi, double norm) {
     double inv = 1.0 / norm;
+    /* x * 1.0 == x for every double, so a mode already normalised (the state of every call after
+     * the first in a driver's loop) is left as the pass would leave it, without the pass */
+    if (inv == 1.0) return;
     for (int q = 0; q < 6 * psi.i * psi.j; q++) psi.fields.m[q] = complex_smul(inv, psi.fields.m[q]);
 }
 
