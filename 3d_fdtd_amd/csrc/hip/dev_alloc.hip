@@ -10,7 +10,9 @@
 //    synchronises the device once and then moves every pending block to the free list;
 //  * the cache holds at most kCacheCap bytes; beyond that a released block goes back to the
 //    runtime; an allocation that fails empties the cache and tries once more;
-//  * fdtd_dev_release_cached_memory() returns everything cached to the runtime.
+//  * fdtd_dev_release_cached_memory() returns everything cached to the runtime;
+//  * releasing a block the cache does not hold as live (a second release) is refused with a
+//    message, never passed to hipFree: that would unmap memory the cache may hand out again.
 // Streams (non-blocking, default priority) are pooled the same way; a stream is released only
 // after its owner has synchronised it.
 #include <map>
@@ -109,12 +111,28 @@ hipError_t dev_malloc_bytes(void **p, size_t bytes) {
     return hipSuccess;
 }
 
+// True when p already waits in the idle or pending list (caller holds c.mu).
+bool cached_block(const Cache &c, void *p) {
+    for (const auto *m : {&c.idle, &c.pending})
+        for (const auto &kv : *m)
+            if (kv.second == p) return true;
+    return false;
+}
+
 hipError_t dev_free(void *p) {
     if (!p) return hipSuccess;
     Cache &c = cache();
     std::lock_guard<std::mutex> lock(c.mu);
     auto it = c.live.find(p);
-    if (it == c.live.end()) return hipFree(p);  // not from the cache
+    if (it == c.live.end()) {
+        // Every device block of this library comes from dev_malloc, so this is a second release
+        // (or a pointer that was never ours).  Returning it to the runtime here would unmap a
+        // block the cache may already have handed to another owner, whose next access (a kernel,
+        // or the runtime's host-side copy into device memory) would then fault; refuse loudly.
+        std::fprintf(stderr, "fdtd3d: device block %p released twice or not allocated here (%s); ignored\n", p,
+                     cached_block(c, p) ? "already cached" : "unknown");
+        return hipErrorInvalidValue;
+    }
     const Block b = it->second;
     c.live.erase(it);
     if (c.cached + b.bytes > kCacheCap) return hipFree(p);

@@ -1,24 +1,31 @@
 #!/usr/bin/env python3
-"""Benchmark: FP32 Yee cell-updates/s of the straight Si strip waveguide workload on one MI355X.
+"""Benchmark: FP32 Yee cell-updates/s and fraction of the HBM roofline on one MI355X.
 
-Workload (propagation along x, cross-section y lateral x z vertical, 20 nm cells):
-  240 x 120 x 100 cells, SiO2 background (n = 1.44), Si core (n = 3.48) 500 nm wide x 220 nm
-  tall, 10-cell CPML on every face, a TE mode launched as a soft source at x = 20 with a
-  Gaussian-modulated carrier (lambda0 = 1.55 um, bandwidth 0.2 um), FP64 mode-overlap monitors
-  (launched / reflected at x = 25, transmitted at x = 220) each with a running DFT at 16
-  wavelengths 1.50-1.60 um, 4000 steps.  The grid is stored in the library's own axes
-  (reference z = propagation): I = 120 (lateral), J = 100 (vertical), K = 240 (propagation).
-  The launched mode profile is the library's mode_gaussian (an Hy Gaussian completed to E/Hz by
-  the discrete curl); it stands in for the eigen-solver mode, which the library does not ship.
+Default workload: the batched taper set.  64 Si tapers (seed 20261203), with lengths
+L ~ U[3, 15] um and 8 spline control-point widths ~ U[0.4, 3.2] um.  Each taper is its own grid
+of uneven length (170-770 cells along the propagation axis) with a 200 x 100 cross-section of
+20 nm cells, rasterised on the device.  Each grid has an 8-cell CPML, a pulsed mode source and an
+FP64 overlap monitor with a running DFT at 16 wavelengths 1.50-1.60 um.  All 64 advance in
+lockstep through the batched kernels.  This is the largest configuration that fits one GPU:
+607.9 M cells.
 
-A "step" is one full time step of this workload (sources, E and H updates with CPML, monitors);
---steps K times K consecutive steps of one run after --warmup W untimed steps.
+A "step" is one full time step of every grid: sources, E and H updates with CPML, and monitors.
+--steps K times K consecutive steps after --warmup W untimed steps.  The batch first spends a
+fixed tuning window picking its launch shape (real steps, before the warmup).
 
-Output: one JSON line (see the keys below).  `roofline` covers the dominant kernel (the E or H
-update, whichever takes longer on average), its duration measured with HIP events on the
-grid's stream over the timed region (every 10th launch sampled).  `cpu_baseline` times the
-compiled reference's timefield3d_apply_timestep_omp_full (FP64, Mur boundary) on the same grid
-size on the host cores (oracle/_ref; the clean-room FP64 oracle when that is absent).
+Output: one JSON line.
+- value: cell-updates per second over every grid.
+- roofline: the dominant kernel, at 48 B per cell-update (6 components x (4 B read + 4 B
+  written)).  A split E or H launch advances half a cell-update, so it is credited 24 B per cell.
+  Its mean duration comes from a window in which every launch is event-timed.
+- traffic: copy-calibrated rocprofv3 counters from profiles/.
+- cpu_baseline: the compiled reference (oracle/_ref) timefield3d_apply_timestep_omp_full, FP64
+  with Mur faces, on the median-length taper grid of the same set.  It runs in child processes
+  with OMP_PROC_BIND=close and OMP_PLACES=cores: once with every physical core of this process's
+  CPU set (lscpu), once with the box's OMP_NUM_THREADS share when that differs, and once with
+  1 thread.  Each is the best of 3 samples; the fastest leg is `value`.
+
+Other workloads: --workload waveguide | design | surrogate | gradient | shard | dropin.
 """
 import argparse
 import ctypes as C
@@ -26,6 +33,7 @@ import importlib
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -36,174 +44,202 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Yee cell-updates/s & % HBM roofline; verified taper designs/hour"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell for one split update kernel: 3 components read + written, the other
-# field's 3 components read (FP32)
-SPLIT_KERNEL_BYTES = 36
-STEP_BYTES = 48  # 6 components x (4 B read + 4 B written) per cell-update
+STEP_BYTES = 48  # algorithmic bytes per cell-update: 6 components x (4 B read + 4 B written)
+HALF_STEP_BYTES = STEP_BYTES // 2  # one split E or H launch: half a cell-update
+SPLIT_KERNEL_BYTES = 36  # what one split launch itself reads and writes (3 comps r+w, 3 read)
+TUNE_STEPS = 64  # the batch autotuner's window (10 candidates x 5 steps) on the first run call
 
-DISC = 0.02
-I, J, K = 120, 100, 240
-NPML = 10
+WL = importlib.import_module("3d_fdtd_amd.workloads")
+DISC = WL.DISC
+I, J, K = WL.WG_I, WL.WG_J, WL.WG_K
+NPML = WL.WG_NPML
 N_RUN = 4000
-
-
-def te0_mode(fd, w0, mi=60, mj=50):
-    """Fundamental (TE0) mode of the strip's cross-section from the library's mode solver
-    (mode_solve_beta_squared): a mi x mj window centred on the 25 x 11-cell Si core, zero walls."""
-    ep = np.full((mj, mi, 3), 1.44 ** 2, complex)
-    x0, y0 = mi // 2 - 12, mj // 2 - 5
-    ep[y0:y0 + 11, x0:x0 + 25, :] = 3.48 ** 2
-    beta, md = fd.solve_mode(ep, DISC, w0, tol=1e-9, size=20, confinement=2)
-    print(f"bench: TE0 mode n_eff {beta.real / w0:.4f}", file=sys.stderr, flush=True)
-    return md
-
-
-def workload(fd, steps_total):
-    L = fd.lib()
-    dt = 0.99 * DISC / math.sqrt(3)
-    g = fd.Grid(I, J, K, DISC)
-    L.timefield3d_set_background(g.tf, 1.44 ** 2, 1.0)
-    ci, cj = I // 2, J // 2
-    L.timefield3d_set_prism_permittivity(g.tf, 3.48 ** 2, ci - 12, ci + 13, cj - 5, cj + 6, 0, K)
-    npml = (C.c_int * 6)(*([NPML] * 6))
-    fd.check(L.fdtd3d_ext_set_cpml(g.tf, npml, 3, 1e-8, 1.0, 0.05), "set_cpml")
-    lam0, dlam = 1.55, 0.2
-    w0 = 2 * math.pi / lam0
-    md = te0_mode(fd, w0)
-    sig_t = lam0 * lam0 / (2 * math.pi * dlam)
-    t = np.arange(steps_total) * dt
-    t0 = 4 * sig_t
-    amp = np.exp(-(t - t0) ** 2 / (2 * sig_t ** 2))
-    phase = w0 * t
-    dptr = C.POINTER(C.c_double)
-    src = L.fdtd3d_ext_run_add_mode_source(g.tf, ci, cj, 20, md, 0, 0, steps_total, amp.ctypes.data_as(dptr),
-                                           phase.ctypes.data_as(dptr))
-    fd.check(0 if src >= 0 else src, "add source")
-    g.amp, g.phase = amp, phase  # the source schedule (the full-size transmission tests reuse it)
-    omegas = 2 * math.pi / np.linspace(1.50, 1.60, 16)
-    mons = []
-    for kz, rev in ((25, 0), (25, 1), (K - 20, 0)):
-        m = L.fdtd3d_ext_run_add_overlap(g.tf, ci, cj, kz, md, rev, len(omegas), omegas.ctypes.data_as(dptr))
-        fd.check(0 if m >= 0 else m, "add overlap")
-        mons.append(m)
-    return g, dt, md, mons, omegas
-
-
-# ---- batched workload: 64 Si tapers of uneven length advanced in lockstep ----
-TAPER_SEED = 20261203
-TAPER_I, TAPER_J = 200, 100  # lateral x vertical cross-section (library i, j), 20 nm cells
-TAPER_NPML = 8
-
-
-def taper_geometry(rng, n):
-    """n tapers: length L ~ U[3, 15] um, 8 control-point widths ~ U[0.4, 3.2] um (seeded)."""
-    return rng.uniform(3.0, 15.0, n), rng.uniform(0.4, 3.2, (n, 8))
-
-
-def taper_permittivity(L, widths):
-    """eps of one taper grid in library axes [k][j][i][3]: SiO2 (n=1.44) cladding, Si (n=3.48) core
-    220 nm tall, whose width follows a cubic spline through the 8 control points along the
-    propagation axis (constant beyond the end points: the strip ports)."""
-    from scipy.interpolate import CubicSpline
-
-    K = int(round(L / DISC)) + 20
-    ep = np.full((K, TAPER_J, TAPER_I, 3), 1.44 ** 2, np.float32)
-    k = np.arange(K)
-    xs = np.linspace(10, K - 11, len(widths))
-    w = CubicSpline(xs, widths)(np.clip(k, xs[0], xs[-1])) / DISC  # width in cells
-    i = np.arange(TAPER_I) + 0.5 - TAPER_I / 2
-    core = np.abs(i)[None, :] < (w[:, None] / 2)  # [k][i]
-    j0 = TAPER_J // 2 - 5
-    ep[:, j0:j0 + 11, :, :][np.broadcast_to(core[:, None, :, None], (K, 11, TAPER_I, 3))] = 3.48 ** 2
-    return ep.reshape(-1), K
-
-
-def tapers_workload(fd, n_tapers, steps_total):
-    L = fd.lib()
-    rng = np.random.default_rng(TAPER_SEED)
-    lengths, widths = taper_geometry(rng, n_tapers)
-    lam0, dlam = 1.55, 0.2
-    w0 = 2 * math.pi / lam0
-    dt = 0.99 * DISC / math.sqrt(3)
-    md = L.mode_gaussian(60, 50, DISC, fd.Complex(1.44 ** 2, 0), fd.Complex(1.0, 0), w0, 8.0, 0)
-    mode = fd.normalised_mode(60, 50, DISC, fd.mode_fields(md).copy())
-    L.mode_free(md)
-    sig_t = lam0 * lam0 / (2 * math.pi * dlam)
-    t = np.arange(steps_total) * dt
-    amp = np.exp(-(t - 4 * sig_t) ** 2 / (2 * sig_t ** 2))
-    phase = w0 * t
-    omegas = 2 * math.pi / np.linspace(1.50, 1.60, 16)
-    grids = []
-    for q in range(n_tapers):
-        ep, K = taper_permittivity(lengths[q], widths[q])
-        g = fd.DevGrid(TAPER_I, TAPER_J, K, DISC, ep)
-        del ep
-        g.cpml((TAPER_NPML,) * 6, 3, 1e-8, 1.0, 0.05)
-        g.add_source(TAPER_I // 2, TAPER_J // 2, TAPER_NPML + 1, mode, 60, 50, DISC, amp, phase)
-        g.mon = g.add_overlap(TAPER_I // 2, TAPER_J // 2, K - TAPER_NPML - 2, mode, 60, 50, DISC, omegas)
-        grids.append(g)
-    return grids, dt, omegas, lengths
 
 
 def bench_tapers(fd, args):
     """Batched taper workload: whole-job cell-updates/s over every grid (one JSON line)."""
     W, S = args.warmup, args.steps
-    print(f"bench: building {args.tapers} taper grids", file=sys.stderr, flush=True)
     KW = 10  # kernel-timing window (every E / H launch event-timed, no graph)
-    grids, dt, omegas, lengths = tapers_workload(fd, args.tapers, W + S + KW)
+    print(f"bench: building {args.tapers} taper grids (device raster)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    grids, dt, omegas, lengths, widths, _, _, _ = WL.tapers(args.tapers, TUNE_STEPS + W + S + KW)
     b = fd.Batch(grids)
     if args.kchunk:
         b.set_shape(args.kchunk, args.kchunk <= 4, True)
-    print(f"bench: warmup {W} steps (chunk-depth autotuning inside)", file=sys.stderr, flush=True)
-    b.run(dt, 0, W)
+    setup_s = time.perf_counter() - t0
+    print(f"bench: set-up {setup_s:.1f} s; tuning window {TUNE_STEPS} steps", file=sys.stderr, flush=True)
+    b.run(dt, 0, TUNE_STEPS)
     info = b.info()
+    print(f"bench: shape {info['shape']}; warmup {W} steps", file=sys.stderr, flush=True)
+    n0 = TUNE_STEPS
+    b.run(dt, n0, W)
+    n0 += W
     print(f"bench: timing {S} steps", file=sys.stderr, flush=True)
-    fd.dev_check(fd.lib().fdtd_dev_batch_timer_start(b.b), "timer")
-    t0 = time.perf_counter()
-    b.run(dt, W, S)
-    ms = C.c_double()
-    fd.dev_check(fd.lib().fdtd_dev_batch_timer_stop(b.b, C.byref(ms)), "timer")
-    wall = time.perf_counter() - t0
-    elapsed_s = max(ms.value / 1e3, wall)
-    # kernel durations: a separate window in which every E / H launch carries its own event pair
-    # (graph replays carry none)
     L = fd.lib()
+    fd.dev_check(L.fdtd_dev_batch_timer_start(b.b), "timer")
+    t0 = time.perf_counter()
+    b.run(dt, n0, S)
+    ms = C.c_double()
+    fd.dev_check(L.fdtd_dev_batch_timer_stop(b.b, C.byref(ms)), "timer")
+    wall = time.perf_counter() - t0
+    n0 += S
+    elapsed_s = max(ms.value / 1e3, wall)
+    # kernel durations: a separate window in which every launch carries its own event pair
+    # (graph replays carry none)
     fd.dev_check(L.fdtd_dev_batch_kernel_timing(b.b, 1), "kernel timing")
-    b.run(dt, W + S, KW)
+    b.run(dt, n0, KW)
     cnt, kms = (C.c_int * 2)(), (C.c_double * 2)()
     fd.dev_check(L.fdtd_dev_batch_kernel_times(b.b, cnt, kms), "kernel times")
     fd.dev_check(L.fdtd_dev_batch_kernel_timing(b.b, 0), "kernel timing off")
     e_us, h_us = (kms[w] / cnt[w] * 1e3 if cnt[w] else 0.0 for w in range(2))
     cells = info["cells"]
     value = cells * S / elapsed_s
-    dom = max(e_us, h_us)
-    dom_name = "batch_update_E" if e_us >= h_us else "batch_update_H"
-    achieved = SPLIT_KERNEL_BYTES * cells / (dom * 1e-6) / 1e9 if dom > 0 else 0.0
-    # HBM bytes per launch from the rocprofv3 counter runs of this workload (copy-calibrated);
-    # they belong to the 64-taper set only
+    fused = info.get("fused", False)
+    if fused:  # one launch per step: a whole cell-update per cell
+        dom, dom_name, per_launch = e_us, "batch_step_fused", STEP_BYTES * cells
+    else:
+        dom = max(e_us, h_us)
+        dom_name = "batch_update_E" if e_us >= h_us else "batch_update_H"
+        per_launch = HALF_STEP_BYTES * cells
+    achieved = per_launch / (dom * 1e-6) / 1e9 if dom > 0 else 0.0
     traffic = read_traffic("tapers_batch_traffic.json") if args.tapers == 64 else None
+    t_mon = [grids[q].overlap(grids[q].mon, len(omegas))[0] for q in (0, len(grids) - 1)]
+    print(f"bench: monitor |a_out| at 1.55 um (grids 0, last): {abs(t_mon[0][7]):.3e} {abs(t_mon[1][7]):.3e}",
+          file=sys.stderr, flush=True)
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": S, "warmup": W,
            "ms_per_step": elapsed_s * 1e3 / S, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"{args.tapers} Si tapers (seed {TAPER_SEED}), lengths U[3,15] um, 8 spline "
-                                  "control-point widths U[0.4,3.2] um, 200x100 cross-section, 8-cell CPML, mode "
-                                  "source, FP32 fields, FP64 overlap monitor x 16 wavelengths per taper, batched "
-                                  "persistent kernels with hipGraph-captured steps",
-                      "grids": args.tapers, "cells_total": cells,
-                      "lengths_cells": [int(round(x / DISC)) + 20 for x in lengths]},
+           "config": {"workload": f"batched tapers: {args.tapers} Si tapers (seed {WL.TAPER_SEED}), lengths "
+                                  "U[3,15] um, 8 spline control-point widths U[0.4,3.2] um, 200x100 cross-section "
+                                  "of 20 nm cells, 8-cell CPML, mode source, FP32 fields, FP64 overlap monitor x 16 "
+                                  "wavelengths per taper; all grids advance in lockstep (persistent batched "
+                                  "kernels, hipGraph-captured steps)",
+                      "grids": args.tapers, "cells_total": cells, "setup_s": round(setup_s, 2),
+                      "tuning_steps_before_warmup": TUNE_STEPS,
+                      "lengths_cells": [WL.taper_length_cells(x) for x in lengths]},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": traffic.get(dom_name) if traffic else None,
-                        "kernel": dom_name,
-                        "kernel_avg_us": round(dom, 1), "e_us": round(e_us, 1), "h_us": round(h_us, 1),
-                        "timed_launches": int(min(cnt[0], cnt[1])), "kernel_timing_window_steps": KW,
-                        "algorithmic_bytes_per_launch": SPLIT_KERNEL_BYTES * cells,
-                        "step_frac_48B": round(STEP_BYTES * cells / (elapsed_s / S) / 1e9 / HBM_PEAK_GBS, 4),
+                        "kernel": dom_name, "kernel_avg_us": round(dom, 1), "e_us": round(e_us, 1),
+                        "h_us": round(h_us, 1), "timed_launches": int(max(cnt[0], cnt[1])),
+                        "kernel_timing_window_steps": KW,
+                        "algorithmic_bytes_per_launch": per_launch,
+                        "bytes_per_cell_update": STEP_BYTES,
+                        "frac_of_value_48B": round(STEP_BYTES * value / 1e9 / HBM_PEAK_GBS, 4),
                         "autotuned_shape": info["shape"], "tuned_ms_per_step": info["tuned"]}}
     b.free()
     for g in grids:
         g.free()
+    if not args.no_cpu:
+        print("bench: CPU baseline (compiled reference, child processes)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_taper(args, lengths)
     print(json.dumps(out), flush=True)
+
+
+def physical_cores():
+    """(physical cores, logical CPUs) in this process's CPU set: lscpu's distinct (socket, core)
+    pairs over the CPUs of the affinity mask."""
+    aff = os.sched_getaffinity(0)
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+    except (OSError, subprocess.CalledProcessError):
+        return len(aff), len(aff)
+    cores = set()
+    for ln in out.splitlines():
+        if not ln or ln.startswith("#"):
+            continue
+        cpu, core, sock = ln.split(",")[:3]
+        if int(cpu) in aff:
+            cores.add((sock, core))
+    return (len(cores) or len(aff)), len(aff)
+
+
+def cpu_baseline_taper(args, lengths):
+    """The compiled reference's full step on the median-length taper grid, in child processes
+    (OpenMP reads its environment when the runtime starts)."""
+    q = int(np.argsort(lengths)[len(lengths) // 2])
+    ncores, ncpu = physical_cores()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    legs = [ncores] + ([share] if 0 < share < ncores else []) + [1]
+    results = []
+    for n in legs:
+        env = dict(os.environ, OMP_NUM_THREADS=str(n), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", "--cpu-taper", str(q), "--tapers",
+               str(args.tapers), "--cpu-sample", "3.0" if n > 1 else "1.0"]
+        try:
+            r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=300)
+        except subprocess.TimeoutExpired:
+            print(f"bench: CPU leg with {n} threads timed out", file=sys.stderr, flush=True)
+            continue
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        if r.returncode != 0 or not line:
+            print(f"bench: CPU leg with {n} threads failed: {r.stderr[-400:]}", file=sys.stderr, flush=True)
+            continue
+        res = json.loads(line[-1])
+        res["threads"] = n
+        results.append(res)
+        print(f"bench: CPU leg {n} threads: {res['value'] / 1e6:.1f} M cell-updates/s", file=sys.stderr, flush=True)
+    if not results:
+        return None
+    best = max(results, key=lambda r: r["value"])
+    return {"value": best["value"], "unit": "cell-updates/s", "cores": best["threads"], "kind": best["kind"],
+            "sample": f"{best['kind']} FP64 full step (Mur faces) on taper {q} of the set "
+                      f"({best['grid']}), best of 3 samples of {best['steps']} steps each; "
+                      f"OMP_PROC_BIND=close OMP_PLACES=cores; host: {ncores} physical cores / {ncpu} CPUs "
+                      "in the CPU set",
+            "legs": [{"threads": r["threads"], "value": r["value"], "best_s_per_step": r["best_s_per_step"]}
+                     for r in results]}
+
+
+def cpu_leg(args):
+    """Child process of cpu_baseline_taper: time the compiled reference's full step (the
+    clean-room FP64 oracle where the reference library is absent) on taper --cpu-taper of the
+    set; prints one JSON line."""
+    from oracle import ref_py
+
+    lengths, widths = WL.taper_geometry(args.tapers)
+    q = args.cpu_taper
+    ep, K = WL.taper_permittivity(lengths[q], widths[q])
+    I, J = WL.TAPER_I, WL.TAPER_J
+    dt = 0.99 * DISC / math.sqrt(3)
+    rng = np.random.default_rng(0)
+    if ref_py.available():
+        R = ref_py.lib()
+        tf = R.timefield3d_new(I, J, K, DISC)
+        A = ref_py.tf_arrays(tf)
+        A["ep"][:] = ep
+        A["fields"][:] = rng.uniform(-1, 1, A["fields"].size)
+
+        def step():
+            R.timefield3d_apply_timestep_omp_full(tf, dt)
+
+        kind = "reference"
+    else:
+        from oracle import oracle_py as O
+
+        o = O.Grid(I, J, K, DISC)
+        o.ep[:] = ep
+        o.f[:] = rng.uniform(-1, 1, o.f.size)
+
+        def step():
+            o.step(dt)
+
+        kind = "port"
+    del ep
+    step()  # warm
+    t0 = time.perf_counter()
+    step()
+    est = time.perf_counter() - t0
+    n = max(1, int(args.cpu_sample / max(est, 1e-6)))
+    best = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            step()
+        best = min(best, (time.perf_counter() - t0) / n)
+    cells = I * J * K
+    print(json.dumps({"value": cells / best, "kind": kind, "steps": n, "best_s_per_step": best,
+                      "grid": f"{I}x{J}x{K} library cells"}), flush=True)
 
 
 def bench_design(fd, args):
@@ -222,7 +258,7 @@ def bench_design(fd, args):
     design = importlib.import_module("3d_fdtd_amd.design")
     fam = design.TaperFamily()
     spec = 0.7
-    rng = np.random.default_rng(TAPER_SEED)
+    rng = np.random.default_rng(WL.TAPER_SEED)
     design.band_transmission(fam.simulate(fam.random(rng, 4)))  # load kernels, fill the caches
     print("bench: plain loop (single-geometry FDTD + adjoint)", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -241,7 +277,7 @@ def bench_design(fd, args):
     t_sim = time.perf_counter() - t0
     net = design.fit(fam, train, t_train, steps=args.fit_steps)
     t_fit = time.perf_counter() - t0 - t_sim
-    loop = design.SurrogateLoop(fam, net, train, t_train, spec=spec, seed=TAPER_SEED + 1)
+    loop = design.SurrogateLoop(fam, net, train, t_train, spec=spec, seed=WL.TAPER_SEED + 1)
     loop.run(seconds=max(args.campaign - (time.perf_counter() - t0), 0.0))
     t_sur = time.perf_counter() - t0
     net.free()
@@ -277,19 +313,6 @@ def bench_design(fd, args):
     print(json.dumps(out), flush=True)
 
 
-def port_mode(fd, fam, p, k, w0, margin=12):
-    """TE0 of the cross-section at plane k of design p (the mode solver on a window inside the
-    PML), as the power-normalised interleaved array the run API takes, and its window size."""
-    ep, K = fam.permittivity(p)
-    sec = ep.reshape(K, fam.J, fam.I, 3)[k].astype(np.complex128)
-    mi, mj = fam.I - 2 * margin, fam.J - 2 * margin
-    i0, j0 = fam.I // 2 - mi // 2, fam.J // 2 - mj // 2
-    beta, md = fd.solve_mode(sec[j0:j0 + mj, i0:i0 + mi], fam.disc, w0, tol=1e-9, size=20, confinement=2)
-    arr = fd.normalised_mode(mi, mj, fam.disc, fd.mode_fields(md).copy())
-    fd.lib().mode_free(md)
-    return arr, mi, mj, beta / w0
-
-
 def bench_gradient(fd, args):
     """The 3-D taper with its DFT-domain permittivity gradient: a linear Si taper 0.5 -> 3.0 um
     over 10 um between strip ports (8 spline control widths), 620 x 200 x 100 cells of 20 nm
@@ -309,8 +332,8 @@ def bench_gradient(fd, args):
     dt, w0 = fam.dt, 2 * math.pi / 1.55
     ci, cj, ks, km = I // 2, J // 2, fam.npml + 1, K - fam.npml - 2
     print("bench: solving the port modes", file=sys.stderr, flush=True)
-    m_in, mi_in, mj_in, n_in = port_mode(fd, fam, p, ks, w0)
-    m_out, mi_out, mj_out, n_out = port_mode(fd, fam, p, km, w0)
+    m_in, mi_in, mj_in, n_in = WL.port_mode(fam, p, ks, w0)
+    m_out, mi_out, mj_out, n_out = WL.port_mode(fam, p, km, w0)
     j0 = J // 2 - fam.core // 2
     klo, khi = fam.npml + fam.port + 2, K - fam.npml - fam.port - 2
     box = (0, I, j0, j0 + fam.core, klo, khi)
@@ -379,7 +402,7 @@ def bench_gradient(fd, args):
     gk.free()
     n = I * J * K
     dom = max(e_us, h_us)
-    achieved = SPLIT_KERNEL_BYTES * n / (dom * 1e-6) / 1e9
+    achieved = HALF_STEP_BYTES * n / (dom * 1e-6) / 1e9
     dom_name = "yee_update_E" if e_us >= h_us else "yee_update_H"
     traffic = read_traffic("yee_traffic_cfg3_taper.json")
     out = {"metric": METRIC, "value": 2 * steps * n / (t_fwd + t_adj), "unit": "cell-updates/s", "n_gpus": 1,
@@ -395,7 +418,7 @@ def bench_gradient(fd, args):
                         "traffic": traffic.get(dom_name) if traffic else None,
                         "kernel": dom_name,
                         "kernel_avg_us": round(dom, 2), "e_us": round(e_us, 2), "h_us": round(h_us, 2),
-                        "algorithmic_bytes_per_launch": SPLIT_KERNEL_BYTES * n,
+                        "algorithmic_bytes_per_launch": HALF_STEP_BYTES * n,
                         "step_frac_48B": round(STEP_BYTES * n / ((e_us + h_us) * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
            "gradient": {"setup_s": round(t_setup, 3), "forward_s": round(t_fwd, 3), "adjoint_s": round(t_adj, 3), "gradient_s": round(t_grad, 4),
                         "T": abs(t) ** 2, "n_eff_in": round(float(n_in.real), 4), "n_eff_out": round(float(n_out.real), 4),
@@ -411,7 +434,7 @@ def gradient_cpu_baseline(fam, ep, K, target_s=10.0):
     """FP64 oracle CPML steps on the same grid (host threads), about target_s of them."""
     from oracle import oracle_py as O
 
-    threads = int(os.environ.get("FDTD3D_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    threads = int(os.environ.get("FDTD3D_CPU_THREADS", physical_cores()[0]))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     g = O.Grid(fam.I, fam.J, K, fam.disc)
     g.ep[:] = ep
@@ -497,7 +520,7 @@ def bench_shard(fd, args):
         g.free()
     cells = I * J * owned
     dom = max(e_us, h_us)
-    achieved = SPLIT_KERNEL_BYTES * cells / (dom * 1e-6) / 1e9
+    achieved = HALF_STEP_BYTES * cells / (dom * 1e-6) / 1e9
     out = {"metric": METRIC, "value": cells / ((e_us + h_us) * 1e-6), "unit": "cell-updates/s", "n_gpus": 1,
            "steps": S, "warmup": 50, "ms_per_step": (e_us + h_us) / 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -509,7 +532,7 @@ def bench_shard(fd, args):
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                         "kernel": "yee_update_E" if e_us >= h_us else "yee_update_H",
                         "kernel_avg_us": round(dom, 2), "e_us": round(e_us, 2), "h_us": round(h_us, 2),
-                        "timed_launches": int(min(cnt)), "algorithmic_bytes_per_launch": SPLIT_KERNEL_BYTES * cells},
+                        "timed_launches": int(min(cnt)), "algorithmic_bytes_per_launch": HALF_STEP_BYTES * cells},
            "shard": {"pair_step_us_two_shards_one_gpu": round(pair_s * 1e6, 1),
                      "halo_copy_pair_us_device_to_device": round(copy_us, 1),
                      "full_grid_cells_per_shard_fraction": round(owned / K, 4)}}
@@ -587,7 +610,7 @@ def bench_dropin(fd, args):
         from oracle import ref_py
 
         if ref_py.available():
-            threads = int(os.environ.get("FDTD3D_CPU_THREADS", min(16, os.cpu_count() or 1)))
+            threads = int(os.environ.get("FDTD3D_CPU_THREADS", physical_cores()[0]))
             os.environ["OMP_NUM_THREADS"] = str(threads)
             R = ref_py.lib()
             rtf, rtaper, rmodes, omega, dt = dropin_setup(R, ref_py.TimeField, ref_py.Complex, 2000)
@@ -613,7 +636,7 @@ def bench_surrogate(fd, args):
     own dispatch-recorded durations, against the dense BF16 matrix-core peak."""
     design = importlib.import_module("3d_fdtd_amd.design")
     dims = (9, 256, 4, 32)
-    rng = np.random.default_rng(TAPER_SEED)
+    rng = np.random.default_rng(WL.TAPER_SEED)
     n_train, n_pool = 4096, 100000
     x = rng.uniform(-1, 1, (n_train, dims[0])).astype(np.float32)
     t = np.sin(x @ rng.normal(0, 0.6, (dims[0], dims[3]))).astype(np.float32)
@@ -684,7 +707,7 @@ def design_cpu_baseline(fam, loop, runs_per_design, n=3):
 
 def cpu_baseline(target_s=12.0):
     """Reference FP64 step on the same grid size, host cores, bounded sample."""
-    threads = int(os.environ.get("FDTD3D_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    threads = int(os.environ.get("FDTD3D_CPU_THREADS", physical_cores()[0]))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     dt = 0.99 * DISC / math.sqrt(3)
     from oracle import ref_py
@@ -752,65 +775,11 @@ def read_traffic(name="yee_traffic_cfg2.json"):
         return json.load(fh)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=N_RUN)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
-    ap.add_argument("--kchunk", type=int, default=None,
-                    help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
-    ap.add_argument("--workload", choices=("waveguide", "tapers", "design", "surrogate", "gradient", "shard",
-                                           "dropin"),
-                    default="waveguide",
-                    help="waveguide: the strip waveguide (default); tapers: the batched taper set; design: "
-                         "verified taper designs/hour, surrogate loop vs plain adjoint loop; surrogate: the MLP's "
-                         "matrix-core training and screening kernels; gradient: the 620x200x100 taper's forward + "
-                         "adjoint runs and DFT eps-gradient; shard: one GPU's z-slab share of that taper with "
-                         "its halo planes; dropin: the reference driver's per-step call sequence through the "
-                         "drop-in C surface")
-    ap.add_argument("--train", type=int, default=4096, help="design workload: FDTD training designs")
-    ap.add_argument("--fit-steps", type=int, default=1500, help="design workload: surrogate training steps")
-    ap.add_argument("--campaign", type=float, default=150.0,
-                    help="design workload: surrogate campaign wall time in s (training set and training included)")
-    ap.add_argument("--plain-starts", type=int, default=32, help="design workload: plain-loop starting designs")
-    ap.add_argument("--tapers", type=int, default=64, help="number of taper grids of the batched workload")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only run a plain copy kernel over MIB MiB (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.calibrate_copy:
-        calibrate_copy(args.calibrate_copy)
-        return
-    if args.gpus != 1:
-        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported (the path shards by "
-              "geometry for batched runs or by z slabs with halo planes for one large grid, "
-              "fdtd_dev_copy_planes; no multi-GPU run exists)", file=sys.stderr)
-        sys.exit(2)
-    fd = importlib.import_module("3d_fdtd_amd")
+def bench_waveguide(fd, args):
+    """The strip waveguide (one grid, device-resident run API): cell-updates/s of K steps."""
     L = fd.lib()
-    if args.workload == "tapers":
-        if args.steps == N_RUN:
-            args.steps = 200
-        bench_tapers(fd, args)
-        return
-    if args.workload == "design":
-        bench_design(fd, args)
-        return
-    if args.workload == "surrogate":
-        bench_surrogate(fd, args)
-        return
-    if args.workload == "gradient":
-        bench_gradient(fd, args)
-        return
-    if args.workload == "shard":
-        bench_shard(fd, args)
-        return
-    if args.workload == "dropin":
-        bench_dropin(fd, args)
-        return
     W, S = args.warmup, args.steps
-    g, dt, md, mons, omegas = workload(fd, W + S + min(400, max(S // 4, 20)))
+    g, dt, md, mons, omegas = WL.strip_waveguide(W + S + min(400, max(S // 4, 20)))
     if args.rows is not None or args.kchunk is not None:
         rows = 0 if args.rows is None else args.rows
         kchunk = args.kchunk if args.kchunk is not None else (0 if rows == 0 else 8)
@@ -849,15 +818,15 @@ def main():
           if (launched[q].re or launched[q].im) else float("nan") for q in range(len(omegas))]
     print(f"bench: |t|^2 over 1.50-1.60 um: {min(tr):.4f} .. {max(tr):.4f}", file=sys.stderr, flush=True)
     avg = [sums[w] / counts[w] if counts[w] else 0.0 for w in range(3)]
-    if counts[2]:  # fused step kernel: 48 algorithmic bytes per cell
+    if counts[2]:  # fused step kernel: a whole cell-update per cell and launch
         dom_name, dom_ms, dom_bytes = "yee_step_fused", avg[2], STEP_BYTES
         step_ms = avg[2]
-    else:  # split pair: 36 algorithmic bytes per cell in each kernel
+    else:  # split pair: each launch advances half a cell-update
         dom_name, dom_ms = ("yee_update_E", avg[0]) if avg[0] >= avg[1] else ("yee_update_H", avg[1])
-        dom_bytes = SPLIT_KERNEL_BYTES
+        dom_bytes = HALF_STEP_BYTES
         step_ms = avg[0] + avg[1]
     achieved = dom_bytes * cells / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
-    traffic = read_traffic()
+    traffic = read_traffic("yee_traffic_cfg2.json")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic.get(dom_name) if traffic else None,
@@ -866,8 +835,7 @@ def main():
             "timed_launches": int(max(counts[0], counts[1], counts[2])),
             "kernel_timing_window_steps": KW,
             "step_kernels_avg_us": round(step_ms * 1e3, 2),
-            "step_frac_48B": round(STEP_BYTES * cells / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
-            if step_ms > 0 else None}
+            "frac_of_value_48B": round(STEP_BYTES * value / 1e9 / HBM_PEAK_GBS, 4)}
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": S, "warmup": W,
            "ms_per_step": elapsed_s * 1e3 / S, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
@@ -882,6 +850,74 @@ def main():
     L.mode_free(md)
     g.free()
     print(json.dumps(out), flush=True)
+
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 20 for the batched tapers, 4000 for the waveguide)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before the timed ones")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--rows", type=int, default=None, help="fused step rows per lane (0 = split kernels)")
+    ap.add_argument("--kchunk", type=int, default=None,
+                    help="z planes per wavefront (fused step) or per thread (split kernels, 0 = automatic)")
+    ap.add_argument("--workload", choices=("tapers", "waveguide", "design", "surrogate", "gradient", "shard",
+                                           "dropin"),
+                    default="tapers",
+                    help="tapers: the batched 64-taper set (default, the largest configuration); waveguide: the "
+                         "strip waveguide; design: verified taper designs/hour, surrogate loop vs plain adjoint "
+                         "loop; surrogate: the MLP's matrix-core training and screening kernels; gradient: the "
+                         "620x200x100 taper's forward + adjoint runs and DFT eps-gradient; shard: one GPU's z-slab "
+                         "share of that taper with its halo planes; dropin: the reference driver's per-step call "
+                         "sequence through the drop-in C surface")
+    ap.add_argument("--train", type=int, default=4096, help="design workload: FDTD training designs")
+    ap.add_argument("--fit-steps", type=int, default=1500, help="design workload: surrogate training steps")
+    ap.add_argument("--campaign", type=float, default=150.0,
+                    help="design workload: surrogate campaign wall time in s (training set and training included)")
+    ap.add_argument("--plain-starts", type=int, default=32, help="design workload: plain-loop starting designs")
+    ap.add_argument("--tapers", type=int, default=64, help="number of taper grids of the batched workload")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel over MIB MiB (HBM counter calibration)")
+    # child-process CPU leg of the batched workload's baseline (no GPU use)
+    ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-taper", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-sample", type=float, default=3.0, help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.cpu_leg:
+        cpu_leg(args)
+        return
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+    if args.gpus != 1:
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported (the path shards by "
+              "geometry for batched runs or by z slabs with halo planes for one large grid, "
+              "fdtd_dev_copy_planes; no multi-GPU run exists)", file=sys.stderr)
+        sys.exit(2)
+    fd = importlib.import_module("3d_fdtd_amd")
+    fd.lib()
+    if args.workload == "tapers":
+        args.steps = 20 if args.steps is None else args.steps
+        args.warmup = 5 if args.warmup is None else args.warmup
+        bench_tapers(fd, args)
+        return
+    args.warmup = 100 if args.warmup is None else args.warmup
+    if args.steps is None:
+        args.steps = N_RUN
+    if args.workload == "design":
+        bench_design(fd, args)
+    elif args.workload == "surrogate":
+        bench_surrogate(fd, args)
+    elif args.workload == "gradient":
+        bench_gradient(fd, args)
+    elif args.workload == "shard":
+        bench_shard(fd, args)
+    elif args.workload == "dropin":
+        bench_dropin(fd, args)
+    else:
+        bench_waveguide(fd, args)
 
 
 if __name__ == "__main__":

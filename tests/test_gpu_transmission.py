@@ -17,13 +17,13 @@ import sys
 import numpy as np
 import pytest
 
-import bench
 from oracle import taper_oracle
 
 pytestmark = pytest.mark.gpu
 
 fd = importlib.import_module("3d_fdtd_amd")
 design = importlib.import_module("3d_fdtd_amd.design")
+WL = importlib.import_module("3d_fdtd_amd.workloads")
 
 
 def say(*a):
@@ -33,9 +33,9 @@ def say(*a):
 
 def test_strip_waveguide_transmission_vs_fp64_oracle():
     steps = 4000
-    say("strip waveguide", bench.I, bench.J, bench.K, steps, "steps vs FP64 oracle")
+    say("strip waveguide", WL.WG_I, WL.WG_J, WL.WG_K, steps, "steps vs FP64 oracle")
     L = fd.lib()
-    g, dt, md, mons, omegas = bench.workload(fd, steps)
+    g, dt, md, mons, omegas = WL.strip_waveguide(steps)
     fd.check(L.fdtd3d_ext_run(g.tf, dt, 0, steps, 0.0), "run")
     acc = []
     for m in (mons[0], mons[2]):
@@ -43,9 +43,9 @@ def test_strip_waveguide_transmission_vs_fp64_oracle():
         fd.check(L.fdtd3d_ext_run_overlap(g.tf, m, cum, None), "overlap")
         acc.append(np.array([complex(c.re, c.im) for c in cum]))
     t_gpu = acc[1] / acc[0]
-    t_ref = taper_oracle.transmission(np.array(g.a["ep"]), bench.I, bench.J, bench.K, bench.DISC, dt, bench.NPML,
+    t_ref = taper_oracle.transmission(np.array(g.a["ep"]), WL.WG_I, WL.WG_J, WL.WG_K, WL.DISC, dt, WL.WG_NPML,
                                       fd.mode_fields(md).copy(), 60, 50, g.amp, g.phase, omegas, 20,
-                                      (25, bench.K - 20))
+                                      (25, WL.WG_K - 20))
     L.mode_free(md)
     g.free()
     err = np.abs(t_gpu - t_ref).max()
@@ -56,15 +56,15 @@ def test_strip_waveguide_transmission_vs_fp64_oracle():
 
 def test_taper_forward_transmission_vs_fp64_oracle():
     steps = 6000
-    fam = design.TaperFamily(disc=bench.DISC, I=200, J=100, core_cells=11, npml=10, port=48, steps=steps,
+    fam = design.TaperFamily(disc=WL.DISC, I=200, J=100, core_cells=11, npml=10, port=48, steps=steps,
                              lam=(1.55, 1.55), nfreq=1)
     p = np.concatenate([np.linspace(0.5, 3.0, 8), [10.0]])
     ep, K = fam.permittivity(p)
     say("taper", fam.I, fam.J, K, steps, "steps vs FP64 oracle")
     w0 = 2 * math.pi / 1.55
     ks, km = fam.npml + 1, K - fam.npml - 2
-    m_in, mi_in, mj_in, _ = bench.port_mode(fd, fam, p, ks, w0)
-    m_out, mi_out, mj_out, _ = bench.port_mode(fd, fam, p, km, w0)
+    m_in, mi_in, mj_in, _ = WL.port_mode(fam, p, ks, w0)
+    m_out, mi_out, mj_out, _ = WL.port_mode(fam, p, km, w0)
     om = np.array([w0])
     g = fd.DevGrid(fam.I, fam.J, K, fam.disc, ep)
     g.cpml((fam.npml,) * 6, 3, 1e-8, 1.0, 0.05)
