@@ -201,6 +201,8 @@ SIGNATURES = {
     "fdtd_dev_run_dft_result": ([C.c_void_p, _i, _P(_d)], _i),
     "fdtd_dev_dft_gradient": ([C.c_void_p, _i, C.c_void_p, _i, _i, _d, _d, _P(_d)], _i),
     "fdtd_dev_synchronize": ([C.c_void_p], _i),
+    "fdtd_dev_mode_overlap": ([C.c_void_p, _i, _i, _i, _P(_d), _i, _i, _d, _i, _P(_d)], _i),
+    "fdtd_dev_mode_overlap_noalloc": ([C.c_void_p, _i, _i, _i, C.c_void_p, _i, _i, _d, _i, C.c_void_p], _i),
     "fdtd_dev_release_cached_memory": ([], _i),
     "fdtd_dev_kernel_timing": ([C.c_void_p, _i], _i),
     "fdtd_dev_kernel_times": ([C.c_void_p, _P(_i), _P(_d)], _i),

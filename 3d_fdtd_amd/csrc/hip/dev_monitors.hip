@@ -7,10 +7,20 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 
 #include "dev_internal.h"
 
 namespace fdtd {
+
+// Scratch of the allocation-free overlap (fdtd_dev_mode_overlap_noalloc): kNoAllocPartials block
+// partials, then the last-block counter in the first word of the final slot.  Each GPU holds its
+// own copy until the process ends, and every call on that GPU shares it.  Host code never passes
+// this name to a kernel: it stands for the host-side shadow, not the device address, which
+// noalloc_scratch() looks up once per device.
+constexpr int kNoAllocPartials = 8192;  // planes of up to 8192 * 128 cells
+static __device__ __align__(256) double2 scratch[kNoAllocPartials + 1];
+
 namespace {
 
 std::map<const void *, TaperHist> g_tapers;
@@ -161,6 +171,38 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const float *Ex, const float
         *count = 0u;
         __threadfence_system();
     }
+}
+
+__global__ void k_clear_counter(unsigned *count) {
+    if (threadIdx.x == 0) *count = 0u;
+}
+
+// Device address of this GPU's scratch array, looked up (hipGetSymbolAddress) on the first call
+// on each device, with that device current; the first call also zeroes the counter with a kernel
+// on `st` (every overlap's last block leaves it at zero again).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double2 *g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(hipStream_t st, double2 **out) {
+    int dev = 0;
+    FDTD_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("overlap (no-alloc): device index beyond the scratch table");
+        return -2;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return fail("hipGetSymbolAddress(scratch)", e != hipSuccess ? e : hipErrorUnknown);
+        auto *p = static_cast<double2 *>(ptr);
+        hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned *>(p + kNoAllocPartials));
+        FDTD_TRY(hipGetLastError());
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
 }
 
 // F += f * twiddle over a z range, optionally one y row (timefield3d.c:2851-2975).
@@ -411,6 +453,37 @@ int fdtd_dev_mode_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mod
     const volatile double2 *r = d->ovl_host;
     out[0] = r->x;
     out[1] = r->y;
+    return 0;
+}
+
+int fdtd_dev_mode_overlap_noalloc(fdtd_dev *d, int cx, int cy, int cz, const double *mode_dev, int mi, int mj,
+                                  double mdisc, int reverse, double *out_dev) {
+    if (!d || !mode_dev || !out_dev) {
+        set_error("overlap (no-alloc): null grid, mode or output pointer");
+        return -2;
+    }
+    if (reinterpret_cast<uintptr_t>(out_dev) % alignof(double2) != 0) {
+        set_error("overlap (no-alloc): the output must be 16-byte aligned device memory");
+        return -2;
+    }
+    if (cz < 0 || cz >= d->K || mi <= 0 || mj <= 0 || !(mdisc > 0)) {
+        set_error("overlap (no-alloc): plane outside the grid or empty mode window");
+        return -2;
+    }
+    const long long nb = (d->P + kBlock - 1) / kBlock;
+    if (nb > kNoAllocPartials) {
+        set_error("overlap (no-alloc): planes are limited to " + std::to_string((long long)kNoAllocPartials * kBlock) +
+                  " cells (this grid has " + std::to_string(d->P) + ")");
+        return -2;
+    }
+    double2 *s = nullptr;
+    const int rc = noalloc_scratch(d->stream, &s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_overlap, dim3((unsigned)nb), dim3(kBlock), 0, d->stream, d->F[EX], d->F[EY], d->F[HX],
+                       d->F[HY], d->I, d->J, d->P, cz, cx, cy, mode_dev, mi, mj, d->disc / mdisc, reverse, s,
+                       reinterpret_cast<unsigned *>(s + kNoAllocPartials), -0.5 * d->disc * d->disc,
+                       reinterpret_cast<double2 *>(out_dev));
+    FDTD_TRY(hipGetLastError());
     return 0;
 }
 

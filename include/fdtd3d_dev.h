@@ -151,6 +151,18 @@ int fdtd_dev_set_source_plane(fdtd_dev *d, int k, const float *plane, const unsi
 int fdtd_dev_mode_overlap(fdtd_dev *d, int cx, int cy, int cz, const double *mode, int mi, int mj, double mdisc,
                           int reverse, double out[2]);
 
+/* The same instantaneous overlap without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  mode_dev is the power-normalised mode already in
+ * device memory (interleaved complex double[6*mi*mj], as for fdtd_dev_mode_overlap), and out_dev is
+ * 16-byte-aligned device memory that receives (re, im).  The call only enqueues on the grid's
+ * stream (synchronise it before reading out_dev) and allocates nothing.  Its block partials and
+ * last-block counter live in a static per-GPU scratch array: planes are limited to 8192 * 128 =
+ * 1,048,576 cells (larger ones fail with -2), and calls that overlap in time on one GPU race on
+ * that array. */
+int fdtd_dev_mode_overlap_noalloc(fdtd_dev *d, int cx, int cy, int cz, const double *mode_dev, int mi, int mj,
+                                  double mdisc, int reverse, double *out_dev);
+
 /* Single-frequency running DFT with a host-computed twiddle; y_slice < 0 means the volume;
  * [k0, k1) bounds z (timefield3d.c:2851-2975). */
 int fdtd_dev_dft(fdtd_dev *d, double tw_re, double tw_im, int y_slice, int k0, int k1);
