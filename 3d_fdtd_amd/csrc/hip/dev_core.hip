@@ -222,6 +222,7 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->fused_rows = 0;  // split E/H kernels: measured faster than the fused step on MI355X
     d->fused_kchunk = 8;
     d->split_kchunk = 0;
+    d->leap_kc = 0;
     d->finite_every = 0;
     d->finite_buf = nullptr;
     d->stream_mode = -1;
@@ -551,9 +552,18 @@ int fdtd_dev_set_fourier(fdtd_dev *d, const double *in) {
 }
 
 int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk) {
+    if (rows == -2) {  // single-pass leap launches for device-resident runs
+        if (kchunk < 0) {
+            set_error("leap launches: z planes per chunk must be >= 0 (0 = automatic)");
+            return -2;
+        }
+        d->fused_rows = -2;
+        d->leap_kc = kchunk;
+        return 0;
+    }
     if (rows != -1 && rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16 && rows != 32) {
-        set_error("step kernel rows must be -1 (fused columns), 0 (split kernels), 1, 2, 4, 8 (fused register "
-                  "strips) or 16, 32 (fused LDS tiles of 8 / 16 rows)");
+        set_error("step kernel rows must be -2 (single-pass leap launches), -1 (fused columns), 0 (split kernels), "
+                  "1, 2, 4, 8 (fused register strips) or 16, 32 (fused LDS tiles of 8 / 16 rows)");
         return -2;
     }
     if (rows != 0 && kchunk < 1) {

@@ -175,6 +175,9 @@ struct fdtd_dev {
     int fused_rows;     // rows per lane of the fused step (0 = split E/H kernels)
     int fused_kchunk;   // z planes per wavefront of the fused step
     int split_kchunk;   // z planes per thread of the split E/H kernels (0 = chosen per launch)
+    // single-pass leap launches of device-resident runs (fused_rows == -2): z planes per chunk
+    // (0 = chosen per grid)
+    int leap_kc;
 
     // materials
     int nmat;

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <vector>
 
+#include "dev_leap.h"
 #include "dev_stencil.h"
 
 namespace fdtd {
@@ -66,7 +67,27 @@ struct Run {
     double2 *mpart = nullptr;
     double2 *minst = nullptr;
     long long mstep_cap = 0;
+    int mslot_cap = 0;  // partial slots per step that mpart holds (plane blocks or leap tiles)
 };
+
+// Monitor partial buffers for n steps of `slots` partials per monitor and step (layout
+// [monitor][step][slot], step stride slots); grows, after synchronising st, when too small.
+int ensure_monitor_parts(Run *r, int n, int slots, hipStream_t st) {
+    if (n <= r->mstep_cap && slots <= r->mslot_cap) return 0;
+    FDTD_TRY(hipStreamSynchronize(st));
+    const long long cap = std::max<long long>(n, r->mstep_cap);
+    const int sc = std::max(slots, r->mslot_cap);
+    (void)dev_free(r->mpart);
+    (void)dev_free(r->minst);
+    r->mpart = r->minst = nullptr;
+    r->mstep_cap = 0;
+    r->mslot_cap = 0;
+    FDTD_TRY(dev_malloc(&r->mpart, (size_t)kMaxRunMon * cap * sc * sizeof(double2)));
+    FDTD_TRY(dev_malloc(&r->minst, (size_t)kMaxRunMon * cap * sizeof(double2)));
+    r->mstep_cap = cap;
+    r->mslot_cap = sc;
+    return 0;
+}
 
 Run *run_of(fdtd_dev *d) {
     if (!d->run) d->run = new Run();
@@ -79,6 +100,19 @@ __global__ void k_run_source(float *J, long long cs, const float *PR, const floa
     if (p >= P || !win[p]) return;
 #pragma unroll
     for (int m = 0; m < 6; m++) J[m * cs + p] = c.x * PR[m * P + p] + c.y * PI[m * P + p];
+}
+
+// The plane of a run source for a leap launch: J (components 0-2) at the E half's step, M
+// (components 3-5) at the H half's step, inside the mode window.
+__global__ void k_run_source_halves(float *J, long long cs, const float *PR, const float *PI, const unsigned char *win,
+                                    long long P, float2 cJ, float2 cM) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !win[p]) return;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const float2 c = m < 3 ? cJ : cM;
+        J[m * cs + p] = c.x * PR[m * P + p] + c.y * PI[m * P + p];
+    }
 }
 
 // All overlap monitors in one launch, blockIdx.y selects the monitor: per-block FP64 partial of
@@ -515,6 +549,64 @@ namespace {
 // per-step monitor partials).
 constexpr int kRunChunk = 4096;
 
+// Steps n0..n0+n-1 as single-pass leap launches: E(n0) alone, then H(s) with E(s+1) for
+// s = n0..n0+n-2 (reading one buffer, writing the other), then H(n0+n-1) alone.  The run sources
+// are evaluated inside the launches with the coefficients of the step each half belongs to; the
+// step-s overlap partials are formed by the launch that computes H(s).
+int run_chunk_leap(fdtd_dev *d, Run *r, double dt, int n0, int n, int nf, const std::vector<SrcPlane *> &planes) {
+    int rc = fused_prepare(d);
+    if (rc) return rc;
+    if ((rc = ensure_coefficients(d, dt)) || (rc = cpml_tables(d, dt))) return rc;
+    fields_changed(d);
+    const int kc = d->leap_kc > 0 ? d->leap_kc : leap_default_chunk(d);
+    const int slots = leap_plane_slots(d);
+    const int nmon = (int)r->ovl.size(), ns = (int)r->src.size();
+    if (nmon > 0 && (rc = ensure_monitor_parts(r, n, slots, d->stream))) return rc;
+    auto coef_at = [](const RunSource &rs, int step) {
+        return step >= 0 && step < rs.nsteps ? rs.coef[step] : make_float2(0.0f, 0.0f);
+    };
+    const bool u8 = d->mat_bytes == 1;
+    const unsigned nbp = (unsigned)((d->P + kBlock - 1) / kBlock);
+    for (int j = 0; j <= n; j++) {
+        const int mode = j == 0 ? 0 : (j == n ? 2 : 1);
+        const int sh = n0 + j - 1;  // step of the H half (modes 1, 2)
+        // each source plane holds J of the E half's step and M of the H half's step
+        for (int q = 0; q < ns; q++) {
+            const RunSource &rs = r->src[q];
+            hipLaunchKernelGGL(k_run_source_halves, dim3(nbp), dim3(kBlock), 0, d->stream, planes[q]->J,
+                               planes[q]->stride, rs.PR, rs.PI, rs.win, d->P, coef_at(rs, mode == 0 ? n0 : n0 + j),
+                               coef_at(rs, sh));
+            FDTD_TRY(hipGetLastError());
+        }
+        LeapArgs L;
+        leap_fill(d, mode, kc, &L);
+        L.nmon = mode == 0 ? 0 : nmon;
+        for (int m = 0; m < L.nmon; m++)
+            L.mon[m] = RunMonArg{r->ovl[m].k, r->ovl[m].W, r->mpart + ((size_t)m * r->mstep_cap + (j - 1)) * slots, 0};
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (mode == 1) timing_events(d, 2, &t0, &t1);
+        if ((rc = leap_launch(L, u8, d->stream, t0, t1))) return rc;
+        if (mode == 1) fused_swap(d);
+        if (mode != 0 && (rc = finite_watch_scan(d, sh))) return rc;
+    }
+    const double scale = -0.5 * d->disc * d->disc;
+    if (nmon > 0) {
+        hipLaunchKernelGGL(k_run_mon_steps, dim3((n + kBlock - 1) / kBlock, nmon), dim3(kBlock), 0, d->stream,
+                           (const double2 *)r->mpart, r->minst, n, slots, r->mstep_cap, scale);
+        hipLaunchKernelGGL(k_run_mon_dft, dim3(nmon), dim3(kBlock), 0, d->stream, (const MonDev *)r->mons,
+                           (const double2 *)r->minst, n, r->mstep_cap, (const double2 *)r->tw, nf);
+        FDTD_TRY(hipGetLastError());
+    }
+    // leave each source's plane holding the values of the last step, as a per-step write would
+    for (int q = 0; q < ns; q++) {
+        const RunSource &rs = r->src[q];
+        hipLaunchKernelGGL(k_run_source, dim3(nbp), dim3(kBlock), 0, d->stream, planes[q]->J, planes[q]->stride,
+                           rs.PR, rs.PI, rs.win, d->P, coef_at(rs, n0 + n - 1));
+        FDTD_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
 int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     Run *r = run_of(d);
     // twiddle table for steps n0..n0+n-1 over every monitor frequency: e^{i omega (step + t_offset) dt}
@@ -587,17 +679,18 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     };
     const double scale = -0.5 * d->disc * d->disc;
     const int ninl = (int)r->src.size() < kMaxRunSrc ? (int)r->src.size() : kMaxRunSrc;
+    if (d->fused_rows == -2 && leap_supported(d) && r->dft.empty() && (int)r->src.size() <= kMaxRunSrc &&
+        nmon <= kMaxRunMon) {
+        bool ring = true;
+        for (auto &o : r->ovl) ring = ring && o.ring_free;
+        if (ring) return run_chunk_leap(d, r, dt, n0, n, nf, planes);
+    }
     // overlap monitors accumulate inside the H update when every one of them can
     bool fold = nmon > 0 && nmon <= kMaxRunMon && step_folds_monitors(d);
     for (auto &o : r->ovl) fold = fold && o.ring_free;
-    if (fold && n > r->mstep_cap) {
-        FDTD_TRY(hipStreamSynchronize(d->stream));
-        (void)dev_free(r->mpart);
-        (void)dev_free(r->minst);
-        r->mpart = r->minst = nullptr;
-        FDTD_TRY(dev_malloc(&r->mpart, (size_t)kMaxRunMon * n * nbp * sizeof(double2)));
-        FDTD_TRY(dev_malloc(&r->minst, (size_t)kMaxRunMon * n * sizeof(double2)));
-        r->mstep_cap = n;
+    if (fold) {
+        int rc = ensure_monitor_parts(r, n, (int)nbp, d->stream);
+        if (rc) return rc;
     }
     for (int s = 0; s < n; s++) {
         const int step = n0 + s;
@@ -733,14 +826,7 @@ int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, Ye
             r->mon_cap = nmon;
         }
         FDTD_TRY(hipMemcpy(r->mons, md.data(), md.size() * sizeof(MonDev), hipMemcpyHostToDevice));
-        if (n > r->mstep_cap) {
-            (void)dev_free(r->mpart);
-            (void)dev_free(r->minst);
-            r->mpart = r->minst = nullptr;
-            FDTD_TRY(dev_malloc(&r->mpart, (size_t)kMaxRunMon * n * nbp * sizeof(double2)));
-            FDTD_TRY(dev_malloc(&r->minst, (size_t)kMaxRunMon * n * sizeof(double2)));
-            r->mstep_cap = n;
-        }
+        if ((rc = ensure_monitor_parts(r, n, (int)nbp, d->stream))) return rc;
     }
     for (auto &s : r->src) {
         int err;

@@ -382,12 +382,12 @@ int cpml_zero(fdtd_dev *d) {
 bool step_uses_split(const fdtd_dev *d) {
     if (!d->cpml.on) return true;
     if (d->fused_rows > 0) return !fused_supported(d);
-    if (d->fused_rows < 0) return !col_supported(d);
-    return true;
+    if (d->fused_rows == -1) return !col_supported(d);
+    return true;  // split pair, or leap launches (a lone step is a split pair; runs use leap_run)
 }
 
 bool step_folds_monitors(const fdtd_dev *d) {
-    return step_uses_split(d) || (d->cpml.on && d->fused_rows < 0 && col_supported(d));
+    return step_uses_split(d) || (d->cpml.on && d->fused_rows == -1 && col_supported(d));
 }
 
 int fill_step_args(fdtd_dev *d, double dt, YeeArgs *a, PmlArgs *pm) {
@@ -476,7 +476,7 @@ int fdtd_dev_step(fdtd_dev *d, double dt, int scheme) {
     int rc;
     if (d->cpml.on && scheme == 0) {
         if (d->fused_rows > 0 && fused_supported(d)) return fused_step(d, dt, d->fused_rows, d->fused_kchunk);
-        if (d->fused_rows < 0 && col_supported(d)) return col_step(d, dt, d->fused_kchunk);
+        if (d->fused_rows == -1 && col_supported(d)) return col_step(d, dt, d->fused_kchunk);
         if ((rc = fdtd_dev_update_E_cpml(d, dt))) return rc;
         return fdtd_dev_update_H_cpml(d, dt);
     }

@@ -69,7 +69,10 @@ int fdtd_dev_cpml_off(fdtd_dev *d);
 int fdtd_dev_update_E_cpml(fdtd_dev *d, double dt);
 int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt);
 /* Kernel used by fdtd_dev_step under the PML boundary: rows = 0 selects the split E / H pair
- * (kchunk > 0 fixes their z planes per thread, 0 lets each launch choose), rows = -1 the fused
+ * (kchunk > 0 fixes their z planes per thread, 0 lets each launch choose), rows = -2 the
+ * single-pass leap launches for device-resident runs (fdtd_dev_run: H of step s and E of step
+ * s + 1 in one launch, each field component read and written once; kchunk = z planes per chunk,
+ * 0 automatic; a lone fdtd_dev_step stays a split pair), rows = -1 the fused
  * single-kernel step in columns (one thread per (i, j) column, kchunk = 1, 2 or 4 z planes, the
  * neighbours' new E recomputed; evaluates run monitors itself), rows = 1, 2, 4 or 8 the fused
  * step in register strips with that many y rows per lane and kchunk z planes per wavefront,
