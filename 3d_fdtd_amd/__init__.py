@@ -113,6 +113,10 @@ SIGNATURES = {
     "mode_power_normalize": ([Mode], None),
     "mode_H_inner_product": ([Mode, Mode], Complex),
     "mode_H_normalize": ([Mode], None),
+    "mode_inner_product": ([Mode, Mode], Complex),
+    "mode_norm": ([Mode], _d),
+    "mode_normalize": ([Mode], None),
+    "mode_apply_zero_gradient_bc": ([Mode], None),
     "mode_set_background": ([Mode, Complex, Complex], None),
     "mode_make_rectangle": ([Mode, Complex, Complex, _i, _i, _i, _i], None),
     "mode_set_pml": ([Mode, _d, _i, _i], None),
@@ -123,6 +127,10 @@ SIGNATURES = {
     "matrix_new": ([_i, _i], Matrix),
     "matrix_free": ([Matrix], None),
     "matrix_diagonalize": ([Matrix, _d, _i], Matrix),
+    "matrix_get_submatrix": ([Matrix, _i, _i], Matrix),
+    "matrix_determinant": ([Matrix], Complex),
+    "matrix_diagonalize_non_hermitian": ([Matrix, _d, _i], Matrix),
+    "matrix_diagonalize_2_by_2": ([Matrix], Matrix),
     "matrix_eigenvector": ([Matrix, Complex], Matrix),
     # complex.h
     "complex_exp": ([Complex], Complex),

@@ -63,6 +63,82 @@ void matrix_set_row(matrix A, matrix row, int i) {
     for (int c = 0; c < A.j; c++) A.m[c + i * A.j] = row.m[c];
 }
 
+/* A without row i and column j (matrix.c:162).  The reference's bounds check compares the row
+ * index with both dimensions and never looks at the column index; here each index is checked
+ * against its own dimension, with the reference's message. */
+matrix matrix_get_submatrix(matrix A, int i, int j) {
+    need(i >= 0 && j >= 0 && i < A.i && j < A.j, "Row or column is out of bounds for getting minor matrix!");
+    matrix M = matrix_new(A.i - 1, A.j - 1);
+    for (int r = 0, rr = 0; r < A.i; r++) {
+        if (r == i) continue;
+        for (int c = 0, cc = 0; c < A.j; c++) {
+            if (c == j) continue;
+            M.m[cc + rr * M.j] = A.m[c + r * A.j];
+            cc++;
+        }
+        rr++;
+    }
+    return M;
+}
+
+/* Laplace expansion along the first row, sum_c (-1)^c A[0][c] det(minor(0, c)), accumulated in
+ * column order: the reference's definition (matrix.c:200), used up to 4 x 4 where it costs at
+ * most 40 products and reproduces the reference's rounding exactly. */
+static complex det_laplace(const complex *a, int n) {
+    if (n == 1) return a[0];
+    complex minor[3 * 3];
+    complex det = complex_zero();
+    double sign = 1.0;
+    for (int c = 0; c < n; c++) {
+        for (int r = 1; r < n; r++)
+            for (int q = 0, qq = 0; q < n; q++)
+                if (q != c) minor[qq++ + (r - 1) * (n - 1)] = a[q + r * n];
+        complex t = complex_mul(det_laplace(minor, n - 1), a[c]);
+        det = complex_add(det, complex_smul(sign, t));
+        sign = -sign;
+    }
+    return det;
+}
+
+/* Determinant of a square matrix (matrix.c:200).  Above 4 x 4 the reference's recursion costs
+ * n! products; here LU factorisation with partial pivoting (Golub & Van Loan 3.4): the product
+ * of the pivots times the sign of the row permutation, n^3/3 complex multiply-adds.  An exactly
+ * zero pivot column gives 0. */
+complex matrix_determinant(matrix A) {
+    need(A.i == A.j, "A is not square for finding determinant!");
+    const int n = A.i;
+    if (n <= 4) return det_laplace(A.m, n);
+    complex *U = malloc((size_t)n * n * sizeof(complex));
+    need(U != NULL, "out of memory");
+    memcpy(U, A.m, (size_t)n * n * sizeof(complex));
+    complex det = complex_from_doubles(1, 0);
+    for (int col = 0; col < n; col++) {
+        int piv = col;
+        for (int r = col + 1; r < n; r++)
+            if (complex_mag(U[col + r * n]) > complex_mag(U[col + piv * n])) piv = r;
+        if (complex_mag(U[col + piv * n]) == 0.0) {
+            free(U);
+            return complex_zero();
+        }
+        if (piv != col) {
+            for (int c = col; c < n; c++) {
+                const complex t = U[c + col * n];
+                U[c + col * n] = U[c + piv * n];
+                U[c + piv * n] = t;
+            }
+            det = complex_smul(-1.0, det);
+        }
+        const complex inv = complex_reciprocal(U[col + col * n]);
+        for (int r = col + 1; r < n; r++) {
+            const complex f = complex_mul(U[col + r * n], inv);
+            for (int c = col + 1; c < n; c++) U[c + r * n] = complex_sub(U[c + r * n], complex_mul(f, U[c + col * n]));
+        }
+        det = complex_mul(det, U[col + col * n]);
+    }
+    free(U);
+    return det;
+}
+
 complex matrix_trace(matrix A) {
     need(A.i == A.j, "trace needs a square matrix!");
     complex t = complex_zero();
@@ -286,16 +362,15 @@ static void hessenberg(complex *H, int n) {
     }
 }
 
-matrix matrix_diagonalize(matrix A, double tol, int max_iterations) {
-    (void)tol;
-    need(A.i == A.j, "Matrix is not square for QL diagonalization!");
-    const int n = A.i;
-    matrix D = matrix_new(n, n);
-    if (n == 0) return D;
+/* Eigenvalues of the n x n matrix a into ev[0..n) (deflation order); 0 when some eigenvalue took
+ * more than max(max_iterations, 30) QR steps (the ones not deflated are then the diagonal entries
+ * reached so far). */
+static int schur_eigenvalues(const complex *a, int n, int max_iterations, complex *ev) {
     complex *H = malloc((size_t)n * n * sizeof(complex));
     need(H != NULL, "out of memory");
-    memcpy(H, A.m, (size_t)n * n * sizeof(complex));
+    memcpy(H, a, (size_t)n * n * sizeof(complex));
     hessenberg(H, n);
+    int converged = 1;
     const double eps = 2.220446049250313e-16;
     int hi = n - 1, its = 0;
     const int cap = max_iterations > 30 ? max_iterations : 30;
@@ -314,7 +389,10 @@ matrix matrix_diagonalize(matrix A, double tol, int max_iterations) {
             its = 0;
             continue;
         }
-        if (++its > cap) break;
+        if (++its > cap) {
+            converged = 0;
+            break;
+        }
         /* Wilkinson shift: the eigenvalue of the trailing 2 x 2 block nearer its last entry */
         const complex a = H[(hi - 1) + (hi - 1) * n], b = H[hi + (hi - 1) * n];
         const complex c = H[(hi - 1) + hi * n], d = H[hi + hi * n];
@@ -348,8 +426,89 @@ matrix matrix_diagonalize(matrix A, double tol, int max_iterations) {
             }
         }
     }
-    for (int q = 0; q < n; q++) D.m[q + q * n] = H[q + q * n];
+    for (int q = 0; q < n; q++) ev[q] = H[q + q * n];
     free(H);
+    return converged;
+}
+
+matrix matrix_diagonalize(matrix A, double tol, int max_iterations) {
+    (void)tol;
+    need(A.i == A.j, "Matrix is not square for QL diagonalization!");
+    const int n = A.i;
+    matrix D = matrix_new(n, n);
+    if (n == 0) return D;
+    complex *ev = malloc((size_t)n * sizeof(complex));
+    need(ev != NULL, "out of memory");
+    schur_eigenvalues(A.m, n, max_iterations, ev);
+    for (int q = 0; q < n; q++) D.m[q + q * n] = ev[q];
+    free(ev);
+    return D;
+}
+
+/* The two roots of lambda^2 - (a + d) lambda + (a d - b c) for [[a, b], [c, d]]:
+ * (a + d +/- sqrt((a - d)^2 + 4 b c)) / 2, the root with the principal square root first. */
+static void roots_2x2(complex a, complex b, complex c, complex d, complex *l1, complex *l2) {
+    const complex amd = complex_sub(a, d);
+    const complex root = complex_sqrt(complex_add(complex_mul(amd, amd), complex_smul(4, complex_mul(b, c))));
+    *l1 = complex_smul(0.5, complex_add(complex_add(a, d), root));
+    *l2 = complex_smul(0.5, complex_sub(complex_add(a, d), root));
+}
+
+/* Eigenvalues of any 2 x 2 matrix as diag(lambda1, lambda2) (matrix.c:1084). */
+matrix matrix_diagonalize_2_by_2(matrix A) {
+    need(A.i == 2 && A.j == 2, "Matrix is not 2x2 for 2x2 diagonalization!");
+    matrix D = matrix_new(2, 2);
+    roots_2x2(A.m[0], A.m[1], A.m[2], A.m[3], &D.m[0], &D.m[3]);
+    return D;
+}
+
+static int by_magnitude(const void *x, const void *y) {
+    const double mx = complex_mag(*(const complex *)x), my = complex_mag(*(const complex *)y);
+    return (mx > my) - (mx < my);
+}
+
+/* Eigenvalues of a general square matrix on the diagonal of D (matrix.c:873).  The reference runs
+ * unshifted QL sweeps until every entry above the diagonal but isolated first-superdiagonal ones
+ * is below tol (at most max_iterations sweeps, then it prints a notice and stops), which leaves the
+ * eigenvalues in increasing magnitude down the diagonal, and then re-solves the leading 2 x 2
+ * block in closed form.  Here the eigenvalues come from the shifted Hessenberg QR above (converged
+ * to working precision; tol unused), are placed in increasing magnitude (ties in deflation order),
+ * and the leading pair is ordered as the closed form orders the roots of diag(d0, d1).  The
+ * off-diagonal entries are zero; the reference's result keeps its lower triangle.  Equal
+ * eigenvalues give finite results here (the reference's closed form takes sqrt(0) there). */
+matrix matrix_diagonalize_non_hermitian(matrix A, double tol, int max_iterations) {
+    (void)tol;
+    need(A.i == A.j, "Matrix is not square for QL diagonalization!");
+    const int n = A.i;
+    matrix D = matrix_new(n, n);
+    if (n == 0) return D;
+    complex *ev = malloc((size_t)n * sizeof(complex));
+    need(ev != NULL, "out of memory");
+    if (!schur_eigenvalues(A.m, n, max_iterations, ev)) printf("QL algorithm reached maximum iterations!\n");
+    /* insertion sort keeps equal magnitudes in deflation order (qsort is not stable) */
+    for (int q = 1; q < n; q++) {
+        const complex v = ev[q];
+        int r = q - 1;
+        while (r >= 0 && by_magnitude(&ev[r], &v) > 0) {
+            ev[r + 1] = ev[r];
+            r--;
+        }
+        ev[r + 1] = v;
+    }
+    /* the closed form on diag(d0, d1) puts first d0 + (principal sqrt((d0 - d1)^2) - (d0 - d1)) / 2,
+     * i.e. d0 when d0 - d1 lies in the right half-plane (or on the positive imaginary axis), else
+     * d1; decided here without the square root, which the series-based complex_sqrt returns as
+     * NaN at 0 (equal eigenvalues) */
+    if (n >= 2) {
+        const complex dd = complex_sub(ev[0], ev[1]);
+        if (!(dd.re > 0 || (dd.re == 0 && dd.im >= 0))) {
+            const complex t = ev[0];
+            ev[0] = ev[1];
+            ev[1] = t;
+        }
+    }
+    for (int q = 0; q < n; q++) D.m[q + q * n] = ev[q];
+    free(ev);
     return D;
 }
 

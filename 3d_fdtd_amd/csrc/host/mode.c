@@ -227,7 +227,16 @@ void mode_initialize_ones(mode psi) {
     for (int q = 0; q < 6 * psi.i * psi.j; q++) psi.fields.m[q] = complex_from_doubles(1, 0);
 }
 
-/* ---- inner products and normalisation (mode.c:1237-1379) -------------------------------- */
+/* ---- inner products and normalisation (mode.c:1208-1379) -------------------------------- */
+
+/* disc^2 sum over every stored value of conj(psi) phi, in storage order (all six components). */
+complex mode_inner_product(mode psi, mode phi) {
+    same_shape(psi, phi, 1);
+    complex s = cz();
+    for (int q = 0; q < 6 * psi.i * psi.j; q++)
+        s = complex_add(s, complex_mul(complex_conj(psi.fields.m[q]), phi.fields.m[q]));
+    return complex_smul(psi.disc * psi.disc, s);
+}
 
 complex mode_H_inner_product(mode psi, mode phi) {
     same_shape(psi, phi, 1);
@@ -257,6 +266,7 @@ complex mode_power_inner_product(mode psi, mode phi) {
     return complex_smul(-0.5 * psi.disc * psi.disc, s);
 }
 
+double mode_norm(mode psi) { return sqrt(complex_mag(mode_inner_product(psi, psi))); }
 double mode_H_norm(mode psi) { return sqrt(complex_mag(mode_H_inner_product(psi, psi))); }
 double mode_power_norm(mode psi) { return sqrt(complex_mag(mode_power_inner_product(psi, psi))); }
 
@@ -268,6 +278,7 @@ static void scale_fields(mode psi, double norm) {
     for (int q = 0; q < 6 * psi.i * psi.j; q++) psi.fields.m[q] = complex_smul(inv, psi.fields.m[q]);
 }
 
+void mode_normalize(mode psi) { scale_fields(psi, mode_norm(psi)); }
 void mode_H_normalize(mode psi) { scale_fields(psi, mode_H_norm(psi)); }
 void mode_power_normalize(mode psi) { scale_fields(psi, mode_power_norm(psi)); }
 

@@ -116,6 +116,23 @@ void mode_apply_zero_gradient_bc_vertical(mode psi) {
     }
 }
 
+/* Zero-gradient copy on all four edges (mode.c:1484): the side columns, then the top and bottom
+ * rows without the corners.  The reference's left-edge copy takes component c of the edge cell
+ * from storage slot 6 + c + 1 after it (c = 1..5; c = 0 from slot 6), one slot further than the
+ * right edge does; that shift is kept so results match, and a slot past the array's end (a mode
+ * one or two cells wide) is not read. */
+void mode_apply_zero_gradient_bc(mode psi) {
+    const int total = 6 * psi.i * psi.j;
+    for (int y = 0; y < psi.j; y++) {
+        copy_cell(psi, y * psi.i + psi.i - 1, y * psi.i + psi.i - 2);
+        const int left = 6 * y * psi.i;
+        psi.fields.m[left] = psi.fields.m[left + 6];
+        for (int c = 1; c < 6; c++)
+            if (left + 7 + c < total) psi.fields.m[left + c] = psi.fields.m[left + 7 + c];
+    }
+    mode_apply_zero_gradient_bc_vertical(psi);
+}
+
 void mode_apply_zero_bc_pml(mode psi) {
     /* every cell where either stretch factor has a positive imaginary part (mode.c:1587-1612) */
     for (int c = 0; c < psi.i * psi.j; c++)

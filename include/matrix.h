@@ -33,6 +33,8 @@ matrix matrix_get_column(matrix A, int j);
 matrix matrix_get_row(matrix A, int i);
 void matrix_set_column(matrix A, matrix col, int j);
 void matrix_set_row(matrix A, matrix row, int i);
+matrix matrix_get_submatrix(matrix A, int i, int j);   /* row i and column j removed (matrix.c:162) */
+complex matrix_determinant(matrix A);                  /* square A (matrix.c:200)                    */
 complex matrix_trace(matrix A);
 matrix matrix_hermitian(matrix A);
 matrix matrix_smul(double scalar, matrix A);
@@ -51,6 +53,11 @@ void matrix_gram_schmidt(matrix A);
  * stopped at an off-diagonal ratio tol; here converged to working precision) and the unit
  * eigenvector for one of them (matrix.c:1114; phase: components sum to a positive real). */
 matrix matrix_diagonalize(matrix A, double tol, int max_iterations);
+/* Eigenvalues of a general square matrix on the diagonal, in increasing magnitude with the leading
+ * pair in closed-form root order (matrix.c:873; off-diagonal entries zero here), and the two
+ * roots of any 2 x 2 matrix as diag(l1, l2), l1 with the principal square root (matrix.c:1084). */
+matrix matrix_diagonalize_non_hermitian(matrix A, double tol, int max_iterations);
+matrix matrix_diagonalize_2_by_2(matrix A);
 matrix matrix_eigenvector(matrix A, complex eigenvalue);
 
 complex matrix_vector_inner_product(matrix A, matrix B);

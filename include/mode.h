@@ -61,10 +61,13 @@ mode mode_file_load(FILE *mode_data);                                      /* mo
 double mode_max_index(mode psi);                                           /* mode.c:1131 */
 void mode_initialize_ones(mode psi);                                       /* mode.c:1150 */
 
+complex mode_inner_product(mode psi, mode phi);                            /* mode.c:1208 */
 complex mode_H_inner_product(mode psi, mode phi);                          /* mode.c:1237 */
 complex mode_power_inner_product(mode psi, mode phi);                      /* mode.c:1267 */
+double mode_norm(mode psi);                                                /* mode.c:1312 */
 double mode_H_norm(mode psi);                                              /* mode.c:1321 */
 double mode_power_norm(mode psi);                                          /* mode.c:1330 */
+void mode_normalize(mode psi);                                             /* mode.c:1339 */
 void mode_H_normalize(mode psi);                                           /* mode.c:1353 */
 void mode_power_normalize(mode psi);                                       /* mode.c:1367 */
 
@@ -96,6 +99,8 @@ void mode_apply_zero_bc_vertical(mode psi);                                /* mo
 void mode_apply_zero_bc_horizontal(mode psi);                              /* mode.c:1457 */
 void mode_apply_zero_gradient_bc_vertical(mode psi);                       /* mode.c:1532 */
 void mode_apply_zero_gradient_bc_horizontal(mode psi);                     /* mode.c:1560 */
+/* both pairs of edges; the left edge copies with the reference's one-slot shift (mode.c:1484) */
+void mode_apply_zero_gradient_bc(mode psi);
 void mode_apply_zero_bc_pml(mode psi);                                     /* mode.c:1587 */
 void mode_apply_bc(mode psi, int confinement);                             /* mode.c:1614 */
 void mode_set_pml(mode psi, double max_ep_im, int order, int pml_thickness); /* mode.c:1634 */

@@ -83,7 +83,17 @@ def lib():
         "timefield3d_set_waveguide": ([TimeField, f, i, i, i, i, i, i], None),
         "timefield3d_set_prism_permittivity": ([TimeField, f, i, i, i, i, i, i], None),
         "timefield3d_file_write_permittivity_y_slice": ([TimeField, C.c_char_p, i, i, i, i, i, d, d], None),
+        "matrix_new": ([i, i], Matrix),
+        "matrix_free": ([Matrix], None),
+        "matrix_get_submatrix": ([Matrix, i, i], Matrix),
+        "matrix_determinant": ([Matrix], Complex),
+        "matrix_diagonalize_non_hermitian": ([Matrix, d, i], Matrix),
+        "matrix_diagonalize_2_by_2": ([Matrix], Matrix),
         "mode_new": ([i, i, d], Mode),
+        "mode_inner_product": ([Mode, Mode], Complex),
+        "mode_norm": ([Mode], d),
+        "mode_normalize": ([Mode], None),
+        "mode_apply_zero_gradient_bc": ([Mode], None),
         "mode_free": ([Mode], None),
         "mode_gaussian": ([i, i, d, Complex, Complex, d, d, i], Mode),
         "mode_power_normalize": ([Mode], None),
@@ -142,3 +152,19 @@ def mode_fields(md):
     """Interleaved (re, im) float64 view of a mode's 6*i*j complex field array."""
     n = 6 * md.i * md.j
     return np.ctypeslib.as_array(C.cast(md.fields.m, C.POINTER(C.c_double)), shape=(2 * n,))
+
+
+def matrix_array(M):
+    """Copy of a matrix's entries as a complex (i, j) numpy array."""
+    raw = np.ctypeslib.as_array(C.cast(M.m, C.POINTER(C.c_double)), shape=(2 * M.i * M.j,)) if M.i * M.j else np.zeros(0)
+    return (raw[0::2] + 1j * raw[1::2]).reshape(M.i, M.j)
+
+
+def matrix_from(L, A):
+    """New reference matrix holding the complex 2-D array A."""
+    A = np.asarray(A, complex)
+    M = L.matrix_new(A.shape[0], A.shape[1])
+    raw = np.ctypeslib.as_array(C.cast(M.m, C.POINTER(C.c_double)), shape=(2 * A.size,))
+    raw[0::2] = A.reshape(-1).real
+    raw[1::2] = A.reshape(-1).imag
+    return M

@@ -357,6 +357,87 @@ def f11(L, manifest):
                                      "pml": "mode_set_pml(0.8, order 2, 4 cells)", "seed": 1111}
 
 
+def f12(L, manifest):
+    """F12: the general matrix and mode helpers -- every minor of a 4 x 5 matrix, determinants of
+    1 x 1 to 6 x 6 random and of a singular 5 x 5, the closed-form 2 x 2 eigenvalues, the QL
+    eigenvalue sweep on matrices with eigenvalue magnitudes 1..n, and the plain mode inner
+    product, norm and normalisation of a 7 x 5 mode and its zero-gradient edges (matrix.c:162-240,
+    873-1111; mode.c:1208-1350, 1484-1530)."""
+    rng = np.random.default_rng(1212)
+
+    def rc(*shape):
+        return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+    def call(fn, A, *args):
+        M = R.matrix_from(L, A)
+        out = fn(M, *args)
+        L.matrix_free(M)
+        return out
+
+    sub_in = rc(4, 5)
+    subs = []
+    for i in range(4):
+        for j in range(5):
+            out = call(L.matrix_get_submatrix, sub_in, i, j)
+            subs.append(R.matrix_array(out))
+            L.matrix_free(out)
+    det_in = [rc(n, n) for n in range(1, 7)]
+    sing = rc(5, 5)
+    sing[3] = sing[1] * (0.5 - 0.25j)  # row 3 is a multiple of row 1
+    det_in.append(sing)
+    dets = []
+    for A in det_in:
+        z = call(L.matrix_determinant, A)
+        dets.append(z.re + 1j * z.im)
+    two_in = rc(20, 2, 2)
+    two_in[5] = [[1, 2], [0, 1]]         # defective: one double root
+    two_in[6] = [[0, -1], [1, 0]]        # roots +-i
+    two_out = []
+    for A in two_in:
+        out = call(L.matrix_diagonalize_2_by_2, A)
+        two_out.append(np.diag(R.matrix_array(out)))
+        L.matrix_free(out)
+    nh_in, nh_out = [], []
+    for n in (2, 3, 4, 6):
+        lam = np.arange(1, n + 1) * np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+        V = rc(n, n) + 3 * np.eye(n)
+        A = V @ np.diag(lam) @ np.linalg.inv(V)
+        out = call(L.matrix_diagonalize_non_hermitian, A, 1e-12, 400)
+        nh_in.append(A)
+        nh_out.append(R.matrix_array(out))
+        L.matrix_free(out)
+    mi, mj, mdisc = 7, 5, 0.05
+    mds = []
+    for _ in range(2):
+        md = L.mode_new(mi, mj, mdisc)
+        mf = R.mode_fields(md)
+        mf[:] = rng.uniform(-1, 1, mf.size)
+        mds.append(md)
+    psi0, phi0 = R.mode_fields(mds[0]).copy(), R.mode_fields(mds[1]).copy()
+    ip = L.mode_inner_product(mds[0], mds[1])
+    nrm = L.mode_norm(mds[0])
+    L.mode_normalize(mds[0])
+    psi1 = R.mode_fields(mds[0]).copy()
+    nrm1 = L.mode_norm(mds[0])
+    L.mode_apply_zero_gradient_bc(mds[1])
+    phi_bc = R.mode_fields(mds[1]).copy()
+    for md in mds:
+        L.mode_free(md)
+    save("F12_matrix_mode.npz", phi_bc=phi_bc, sub_in=sub_in, subs=np.array(subs), det_sizes=np.array([len(A) for A in det_in]),
+         det_in=np.concatenate([A.reshape(-1) for A in det_in]), dets=np.array(dets), two_in=two_in,
+         two_out=np.array(two_out), nh_sizes=np.array([len(A) for A in nh_in]),
+         nh_in=np.concatenate([A.reshape(-1) for A in nh_in]),
+         nh_out=np.concatenate([A.reshape(-1) for A in nh_out]), nh_tol=1e-12, nh_max_iterations=400,
+         mi=mi, mj=mj, mdisc=mdisc, psi0=psi0, phi0=phi0, inner=np.array([ip.re, ip.im]), norm=nrm,
+         psi1=psi1, norm1=nrm1)
+    manifest["F12_matrix_mode"] = {
+        "matrix": "U[-1,1]^2 complex entries seed 1212: minors of 4x5; det 1..6 and singular 5x5; 20 2x2 "
+                  "(one defective, one rotation); QL sweep (tol 1e-12, 400 sweeps) on V diag(1..n e^{i phi}) V^-1, "
+                  "n 2,3,4,6",
+        "mode": "two 7x5 modes, disc 0.05, fields U[-1,1]: inner product, norm, normalize, norm after; "
+                "zero-gradient edges of the second"}
+
+
 def main():
     L = R.lib()
     only = sys.argv[1:]  # fixture function names (e.g. f11): regenerate just those
@@ -382,6 +463,7 @@ def main():
     f7(L, manifest)
     f9(L, manifest)
     f11(L, manifest)
+    f12(L, manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, sort_keys=True)
     print("fixtures written to", HERE)

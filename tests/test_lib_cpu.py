@@ -157,3 +157,169 @@ def test_reference_driver_links_unchanged():
         assert r.returncode == 0, r.stderr
     finally:
         shutil.rmtree(d)
+
+
+def _mat(A):
+    A = np.asarray(A, complex)
+    M = fd.lib().matrix_new(A.shape[0], A.shape[1])
+    v = fd.matrix_view(M, A.size)
+    v[0::2] = A.reshape(-1).real
+    v[1::2] = A.reshape(-1).imag
+    return M
+
+
+def _arr(M):
+    v = fd.matrix_view(M, M.i * M.j) if M.i * M.j else np.zeros(0)
+    return (v[0::2] + 1j * v[1::2]).reshape(M.i, M.j)
+
+
+def _cx(z):
+    return complex(z.re, z.im)
+
+
+def test_matrix_helpers_match_reference_fixture(golden_dir):
+    """Minors and 2 x 2 roots bit-exact, determinants bit-exact up to 4 x 4 (the reference's own
+    expansion) and to 1e-13 above (LU), each also against numpy; QL eigenvalues against the
+    reference's (converged to its tol) and numpy's."""
+    z = np.load(os.path.join(golden_dir, "F12_matrix_mode.npz"))
+    L = fd.lib()
+    A = _mat(z["sub_in"])
+    q = 0
+    for i in range(4):
+        for j in range(5):
+            S = L.matrix_get_submatrix(A, i, j)
+            got = _arr(S)
+            assert np.array_equal(got, z["subs"][q])
+            assert np.array_equal(got, np.delete(np.delete(z["sub_in"], i, 0), j, 1))
+            L.matrix_free(S)
+            q += 1
+    L.matrix_free(A)
+    off = 0
+    for n, want in zip(z["det_sizes"], z["dets"]):
+        Am = z["det_in"][off:off + n * n].reshape(n, n)
+        off += n * n
+        M = _mat(Am)
+        got = _cx(L.matrix_determinant(M))
+        L.matrix_free(M)
+        scale = np.prod(np.linalg.norm(Am, axis=1))  # Hadamard bound on |det|
+        if n <= 4:
+            assert got == want
+        else:
+            assert abs(got - want) <= 1e-13 * scale
+        assert abs(got - np.linalg.det(Am)) <= 1e-13 * scale
+    for Am, want in zip(z["two_in"], z["two_out"]):
+        M = _mat(Am)
+        D = L.matrix_diagonalize_2_by_2(M)
+        got = _arr(D)
+        L.matrix_free(M)
+        L.matrix_free(D)
+        assert np.array_equal(np.diag(got), want, equal_nan=True) and got[0, 1] == 0 and got[1, 0] == 0
+        if np.isnan(want).any():
+            # a double root: the reference's series-based complex_sqrt(0) is NaN, and so is this
+            assert abs((Am[0, 0] - Am[1, 1]) ** 2 + 4 * Am[0, 1] * Am[1, 0]) == 0
+            continue
+        # the two roots solve the characteristic polynomial: sum = trace, product = det (to the
+        # accuracy of the series-based complex_sqrt, about 1e-13 relative)
+        assert abs(want.sum() - np.trace(Am)) <= 1e-12 * (1 + abs(np.trace(Am)))
+        assert abs(want.prod() - np.linalg.det(Am)) <= 1e-11 * (1 + np.abs(Am).max() ** 2)
+    off = 0
+    for n in z["nh_sizes"]:
+        Am = z["nh_in"][off:off + n * n].reshape(n, n)
+        ref = z["nh_out"][off:off + n * n].reshape(n, n)
+        off += n * n
+        M = _mat(Am)
+        D = L.matrix_diagonalize_non_hermitian(M, float(z["nh_tol"]), int(z["nh_max_iterations"]))
+        got = _arr(D)
+        L.matrix_free(M)
+        L.matrix_free(D)
+        ev = np.diag(got)
+        assert np.count_nonzero(got - np.diag(ev)) == 0
+        # same eigenvalues in the same order as the reference's QL sweep (which stops at tol 1e-12
+        # above the diagonal with its lower triangle of order 1, so agreement is to ~1e-6)
+        assert np.abs(ev - np.diag(ref)).max() <= 1e-5
+        # and to working precision against LAPACK
+        lap = np.linalg.eigvals(Am)
+        for e in ev:
+            assert np.abs(lap - e).min() <= 1e-11 * np.abs(lap).max()
+        assert np.all(np.diff(np.abs(ev[1:])) >= 0)  # increasing magnitude after the leading pair
+    # repeated eigenvalues stay finite (the identity, and a Jordan block)
+    for Am in (np.eye(3), np.array([[2, 1, 0], [0, 2, 0], [0, 0, 5]], complex)):
+        M = _mat(Am)
+        D = L.matrix_diagonalize_non_hermitian(M, 1e-12, 100)
+        ev = np.diag(_arr(D))
+        L.matrix_free(M)
+        L.matrix_free(D)
+        assert np.allclose(np.sort_complex(ev), np.sort_complex(np.linalg.eigvals(Am)), atol=1e-7)
+
+
+def test_mode_plain_inner_product_and_edges_match_reference_fixture(golden_dir):
+    z = np.load(os.path.join(golden_dir, "F12_matrix_mode.npz"))
+    L = fd.lib()
+    mi, mj, md = int(z["mi"]), int(z["mj"]), float(z["mdisc"])
+    psi = fd.mode_from_array(mi, mj, md, z["psi0"])
+    phi = fd.mode_from_array(mi, mj, md, z["phi0"])
+    ip = L.mode_inner_product(psi, phi)
+    assert ip.re == z["inner"][0] and ip.im == z["inner"][1]
+    a = z["psi0"][0::2] + 1j * z["psi0"][1::2]
+    b = z["phi0"][0::2] + 1j * z["phi0"][1::2]
+    assert abs(_cx(ip) - md * md * np.vdot(a, b)) <= 1e-14 * md * md * np.abs(a).sum() * np.abs(b).max()
+    assert L.mode_norm(psi) == z["norm"]
+    L.mode_normalize(psi)
+    assert np.array_equal(fd.mode_fields(psi), z["psi1"])
+    assert L.mode_norm(psi) == z["norm1"] and abs(z["norm1"] - 1) < 1e-14
+    L.mode_apply_zero_gradient_bc(phi)
+    assert np.array_equal(fd.mode_fields(phi), z["phi_bc"])
+    # the interior is untouched and the right edge is a plain copy of the column beside it
+    got = fd.mode_fields(phi).reshape(mj, mi, 12)
+    before = z["phi0"].reshape(mj, mi, 12)
+    assert np.array_equal(got[1:-1, 1:-1], before[1:-1, 1:-1])
+    assert np.array_equal(got[:, -1], before[:, -2])
+    for m in (psi, phi):
+        L.mode_free(m)
+
+
+# Reference declarations with no definition here, by reason (the reference header they come from
+# in the comment).  Everything else any /root/reference/*.h declares must be exported.
+OUT_OF_SCOPE = {
+    # timefield3d.h:91-92: declared by the reference but defined nowhere in it (cannot be linked)
+    "timefield3d_apply_E_timestep_use_polarization", "timefield3d_apply_timestep_use_polarization",
+    # mode.h: the first-order beta solver and its operators (mode.c:1792-2198, 2399-2795)
+    "mode_solve_beta", "mode_apply_beta_operator", "mode_apply_beta_operator_cylindrical",
+    "mode_apply_E_curl", "mode_apply_omega_squared_operator", "mode_arnoldi_eigensolver",
+    # mode.h: radial / angular variants (mode.c:1038-1130, 3146-3349, 3404-3460)
+    "mode_angular_file_load", "mode_angular_file_print", "mode_apply_radial_H_curl", "mode_apply_radial_H_div",
+    "mode_apply_radial_beta_squared_operator", "mode_solve_radial_beta_squared",
+    # mode.h: one-sided and graded PML profiles (mode.c:1662-1791)
+    "mode_set_grade_to_constant_pml", "mode_set_pml_bottom", "mode_set_pml_left", "mode_set_pml_right",
+    "mode_set_pml_top",
+    # mode.h: susceptibility integrals (mode.c:3827-4493)
+    "mode_loss_susceptibility_bottom_half_plane", "mode_loss_susceptibility_rectangle",
+    "mode_loss_susceptibility_rectangle_bottom", "mode_loss_susceptibility_rectangle_sides",
+    "mode_loss_susceptibility_rectangle_top", "mode_n_susceptibility_bottom_half_plane",
+    "mode_n_susceptibility_rectangle", "mode_n_susceptibility_rectangle_bottom",
+    "mode_n_susceptibility_rectangle_edge", "mode_n_susceptibility_rectangle_exclude_rectangle",
+    "mode_n_susceptibility_rectangle_sides", "mode_n_susceptibility_rectangle_sides_longitudinal",
+    "mode_n_susceptibility_rectangle_sides_normal", "mode_n_susceptibility_rectangle_sides_outside",
+    "mode_n_susceptibility_rectangle_sides_tangential", "mode_n_susceptibility_rectangle_top",
+}
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference"), reason="reference headers not present")
+def test_every_reference_declaration_is_exported():
+    ref_names = set()
+    for h in glob.glob("/root/reference/*.h"):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", " ", src, flags=re.S)
+        src = re.sub(r"//[^\n]*", " ", src)
+        src = re.sub(r"#[^\n]*", " ", src)
+        for m in re.finditer(r"\b([A-Za-z_]\w*)\s*\(([^;{}()]|\([^()]*\))*\)\s*;", src):
+            if m.group(1) not in ("if", "while", "return", "sizeof"):
+                ref_names.add(m.group(1))
+    assert len(ref_names) > 200
+    lib = C.CDLL(fd.LIB_PATH)
+    missing = sorted(n for n in ref_names - OUT_OF_SCOPE if not hasattr(lib, n))
+    assert not missing, missing
+    # every exported reference name is also declared in include/*.h, so a caller compiles
+    undeclared = sorted(n for n in ref_names - OUT_OF_SCOPE if n not in declared_functions())
+    assert not undeclared, undeclared
+    assert OUT_OF_SCOPE <= ref_names  # the list names real reference declarations only
