@@ -167,6 +167,8 @@ SIGNATURES = {
     "fdtd3d_ext_kernel_times": ([TimeField, _P(_i), _P(_d)], _i),
     "fdtd3d_ext_set_step_kernel": ([TimeField, _i, _i], _i),
     "fdtd3d_ext_count_nonfinite": ([TimeField, _P(C.c_longlong)], _i),
+    "fdtd3d_ext_device_materials": ([TimeField, _P(C.c_float), _P(C.c_float), _P(C.c_float)], _i),
+    "fdtd3d_ext_material_counts": ([TimeField, _P(C.c_longlong)], _i),
     "fdtd3d_ext_set_finite_check": ([TimeField, _i], _i),
     "fdtd3d_ext_set_cpml": ([TimeField, _P(_i), _i, _d, _d, _d], _i),
     "fdtd3d_ext_set_mur": ([TimeField], _i),
@@ -188,6 +190,10 @@ SIGNATURES = {
     "fdtd_dev_get_fields": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_set_fields": ([C.c_void_p, _P(_d)], _i),
     "fdtd_dev_count_nonfinite": ([C.c_void_p, _P(C.c_longlong)], _i),
+    "fdtd_dev_paint_taper": ([C.c_void_p, C.c_float, C.c_float] + [_i] * 8, _i),
+    "fdtd_dev_paint_prism": ([C.c_void_p, C.c_float, C.c_float] + [_i] * 7, _i),
+    "fdtd_dev_get_materials": ([C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float)], _i),
+    "fdtd_dev_material_counts": ([C.c_void_p, _P(C.c_longlong)], _i),
     "fdtd_dev_copy_planes": ([C.c_void_p, _i, C.c_void_p, _i, C.c_uint], _i),
     "fdtd_dev_set_streaming_loads": ([C.c_void_p, _i], _i),
     "fdtd_dev_set_materials_extruded": ([C.c_void_p, _i, _P(_f), _P(_f), _P(_f), _P(C.c_uint16), _i, _i, _i],

@@ -292,8 +292,9 @@ fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n) {
         return nullptr;
     }
     for (int g = 0; g < n; g++) {
-        if (!grids[g] || grids[g]->mat_bytes != grids[0]->mat_bytes) {
-            set_error("batch: null grid or grids with different material-id widths");
+        if (!grids[g] || grids[g]->mat_bytes != grids[0]->mat_bytes || grids[g]->mat_bytes > 2) {
+            set_error("batch: null grid, grids with different material-id widths, or more than 65536 materials "
+                      "in a grid");
             return nullptr;
         }
         if (grids[g]->N >= kSmallGridCells) {

@@ -358,7 +358,9 @@ void launch_col(bool mon, bool u8, dim3 grid, hipStream_t st, hipEvent_t t0, hip
 
 }  // namespace
 
-bool col_supported(const fdtd_dev *d) { return d->cpml.on && d->N < kSmallGridCells && d->K >= 3; }
+bool col_supported(const fdtd_dev *d) {
+    return d->cpml.on && d->mat_bytes <= 2 && d->N < kSmallGridCells && d->K >= 3;
+}
 
 int col_step(fdtd_dev *d, double dt, int kc) {
     fields_changed(d);

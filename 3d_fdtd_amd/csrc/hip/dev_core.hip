@@ -50,6 +50,44 @@ __global__ void k_extrude_ids(T *__restrict__ mat, const uint16_t *__restrict__ 
     mat[c] = (T)(j >= j0 && j < j1 ? ids[(long long)k * I + i] : background);
 }
 
+// Region of a repaint: x in [lo, hi) of rows y in [y0, y1), z in [z0, z1).  For a linear taper the
+// x range of plane k is [x0 + round(ls (k - z0)), x1 + round(rs (k - z0))), evaluated in double as
+// the reference evaluates it (timefield3d.c:1511-1535); for a prism it is [x0, x1).  Cells outside
+// the grid are skipped.
+struct PaintArgs {
+    int taper;
+    int x0, x1, y0, y1, z0, z1;  // z0, z1: planes of the launch (inside the grid)
+    int zs;                      // taper: plane where the slopes start (start_z)
+    double ls, rs;
+};
+
+// Cells of the region take material remap[id] (the same tuple with the new permittivity and
+// permeability); one block per (row, 256 columns).
+template <typename T>
+__global__ void k_paint(T *__restrict__ mat, const uint32_t *__restrict__ remap, int I, int J, PaintArgs p,
+                        int xblocks) {
+    const long long row = blockIdx.x / xblocks;
+    const int i = (int)(blockIdx.x % xblocks) * blockDim.x + threadIdx.x;
+    const int ny = p.y1 - p.y0;
+    const int j = p.y0 + (int)(row % ny), k = p.z0 + (int)(row / ny);
+    int lo = p.x0;
+    double hi = p.x1;
+    if (p.taper) {
+        lo = (int)(p.x0 + round(p.ls * (k - p.zs)));
+        hi = p.x1 + round(p.rs * (k - p.zs));
+    }
+    if (i < lo || i < 0 || i >= I || !((double)i < hi)) return;
+    const long long c = i + (long long)I * (j + (long long)J * k);
+    mat[c] = (T)remap[mat[c]];
+}
+
+// Wider material ids (the table outgrew 1 or 2 bytes).
+template <typename S, typename T>
+__global__ void k_widen_ids(const S *__restrict__ src, T *__restrict__ dst, long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < N) dst[c] = (T)src[c];
+}
+
 // AoS double[6N] -> six SoA float arrays.
 __global__ void k_aos_to_soa(const double *__restrict__ aos, float *__restrict__ base, long long N, long long Nal) {
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -290,7 +328,7 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
     // kernels already enqueued may still read the id volume and the tables
     FDTD_TRY(hipStreamSynchronize(d->stream));
     std::unordered_map<MatKey, int, MatKeyHash> ids;
-    std::vector<uint16_t> cell_id((size_t)d->N);
+    std::vector<uint32_t> cell_id((size_t)d->N);
     std::vector<float> props;
     static const float kOne3[3] = {1.0f, 1.0f, 1.0f};
     static const float kZero6[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -301,7 +339,7 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
         const float *e3 = ep + 3 * c, *m3 = mu ? mu + 3 * c : kOne3, *s6 = sig ? sig + 6 * c : kZero6;
         if (prev_id >= 0 && std::memcmp(e3, prev.v, 3 * sizeof(float)) == 0 &&
             std::memcmp(m3, prev.v + 3, 3 * sizeof(float)) == 0 && std::memcmp(s6, prev.v + 6, 6 * sizeof(float)) == 0) {
-            cell_id[c] = (uint16_t)prev_id;
+            cell_id[c] = (uint32_t)prev_id;
             continue;
         }
         MatKey k;
@@ -312,10 +350,6 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
         auto it = ids.find(k);
         if (it == ids.end()) {
             id = (int)ids.size();
-            if (id >= 65536) {
-                set_error("more than 65536 distinct (ep, mu, sigma) tuples are not supported");
-                return -2;
-            }
             ids.emplace(k, id);
             props.resize((size_t)(id + 1) * kPropStride, 0.0f);
             std::memcpy(&props[(size_t)id * kPropStride], k.v, sizeof k.v);
@@ -324,10 +358,12 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
         }
         prev = k;
         prev_id = id;
-        cell_id[c] = (uint16_t)id;
+        cell_id[c] = (uint32_t)id;
     }
+    // ids of 1, 2 or 4 bytes: a per-cell graded medium (every cell its own tuple) takes 4-byte ids
+    // and a table as long as the grid, and runs on the split kernels
     int nmat = (int)ids.size();
-    int bytes = nmat <= 256 ? 1 : 2;
+    int bytes = nmat <= 256 ? 1 : (nmat <= 65536 ? 2 : 4);
     if (d->mat && bytes != d->mat_bytes) {
         FDTD_TRY(dev_free(d->mat));
         d->mat = nullptr;
@@ -337,8 +373,12 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
         std::vector<uint8_t> c8((size_t)d->N);
         for (long long c = 0; c < d->N; c++) c8[c] = (uint8_t)cell_id[c];
         FDTD_TRY(hipMemcpy(d->mat, c8.data(), (size_t)d->N, hipMemcpyHostToDevice));
+    } else if (bytes == 2) {
+        std::vector<uint16_t> c16((size_t)d->N);
+        for (long long c = 0; c < d->N; c++) c16[c] = (uint16_t)cell_id[c];
+        FDTD_TRY(hipMemcpy(d->mat, c16.data(), (size_t)d->N * 2, hipMemcpyHostToDevice));
     } else {
-        FDTD_TRY(hipMemcpy(d->mat, cell_id.data(), (size_t)d->N * 2, hipMemcpyHostToDevice));
+        FDTD_TRY(hipMemcpy(d->mat, cell_id.data(), (size_t)d->N * 4, hipMemcpyHostToDevice));
     }
     if (nmat != d->nmat || !d->d_props) {
         (void)dev_free(d->d_props);
@@ -354,6 +394,177 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
     d->mat_bytes = bytes;
     d->coef_dt = NAN;
     d->have_mats = true;
+    d->mat_uploads++;
+    return 0;
+}
+
+namespace {
+
+// Material ids -> per-cell AoS ep[3], mu[3], sig[6] (any of them NULL to skip).
+int expand_materials(fdtd_dev *d, float *ep, float *mu, float *sig) {
+    FDTD_TRY(hipStreamSynchronize(d->stream));
+    std::vector<unsigned char> raw((size_t)d->N * d->mat_bytes);
+    FDTD_TRY(hipMemcpy(raw.data(), d->mat, raw.size(), hipMemcpyDeviceToHost));
+    for (long long c = 0; c < d->N; c++) {
+        const size_t id = d->mat_bytes == 1   ? raw[c]
+                          : d->mat_bytes == 2 ? reinterpret_cast<const uint16_t *>(raw.data())[c]
+                                              : reinterpret_cast<const uint32_t *>(raw.data())[c];
+        const float *pr = &d->props[id * kPropStride];
+        for (int m = 0; m < 3; m++) {
+            if (ep) ep[3 * c + m] = pr[m];
+            if (mu) mu[3 * c + m] = pr[3 + m];
+        }
+        if (sig)
+            for (int m = 0; m < 6; m++) sig[6 * c + m] = pr[6 + m];
+    }
+    return 0;
+}
+
+// Repaint a region on the device: each cell keeps its conductivities and takes permittivity ep
+// (and permeability mu when set_mu) on all three components.  The host maps every table entry to
+// its repainted tuple (appending the new ones), widens the id volume when the table outgrows it,
+// and a kernel rewrites the ids of the region's cells: no per-cell upload.
+int paint(fdtd_dev *d, PaintArgs p, float ep, float mu, bool set_mu) {
+    if (!d->have_mats) {
+        set_error("paint: the grid has no materials yet");
+        return -2;
+    }
+    if (p.y0 < 0) p.y0 = 0;
+    if (p.y1 > d->J) p.y1 = d->J;
+    if (p.z0 < 0) p.z0 = 0;
+    if (p.z1 > d->K) p.z1 = d->K;
+    if (p.y1 <= p.y0 || p.z1 <= p.z0) return 0;
+    FDTD_TRY(hipStreamSynchronize(d->stream));  // kernels already enqueued may still read the ids
+    std::unordered_map<MatKey, int, MatKeyHash> ids;
+    for (int m = 0; m < d->nmat; m++) {
+        MatKey k;
+        std::memcpy(k.v, &d->props[(size_t)m * kPropStride], sizeof k.v);
+        ids.emplace(k, m);
+    }
+    const int nmat0 = d->nmat;
+    std::vector<uint32_t> remap((size_t)nmat0);
+    std::vector<float> props = d->props;
+    for (int m = 0; m < nmat0; m++) {
+        MatKey k;
+        std::memcpy(k.v, &d->props[(size_t)m * kPropStride], sizeof k.v);
+        for (int q = 0; q < 3; q++) {
+            k.v[q] = ep;
+            if (set_mu) k.v[3 + q] = mu;
+        }
+        auto it = ids.find(k);
+        if (it == ids.end()) {
+            const int id = (int)ids.size();
+            it = ids.emplace(k, id).first;
+            props.resize((size_t)(id + 1) * kPropStride, 0.0f);
+            std::memcpy(&props[(size_t)id * kPropStride], k.v, sizeof k.v);
+        }
+        remap[m] = (uint32_t)it->second;
+    }
+    const int nmat = (int)ids.size();
+    const int bytes = nmat <= 256 ? 1 : (nmat <= 65536 ? 2 : 4);
+    const unsigned nbN = (unsigned)((d->N + kBlock - 1) / kBlock);
+    if (bytes > d->mat_bytes) {
+        void *w = nullptr;
+        FDTD_TRY(dev_malloc(&w, (size_t)d->N * bytes));
+        if (d->mat_bytes == 1 && bytes == 2)
+            hipLaunchKernelGGL((k_widen_ids<uint8_t, uint16_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
+                               static_cast<const uint8_t *>(d->mat), static_cast<uint16_t *>(w), d->N);
+        else if (d->mat_bytes == 1)
+            hipLaunchKernelGGL((k_widen_ids<uint8_t, uint32_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
+                               static_cast<const uint8_t *>(d->mat), static_cast<uint32_t *>(w), d->N);
+        else
+            hipLaunchKernelGGL((k_widen_ids<uint16_t, uint32_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
+                               static_cast<const uint16_t *>(d->mat), static_cast<uint32_t *>(w), d->N);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+        if (e != hipSuccess) {
+            (void)dev_free(w);
+            return fail("paint: widen ids", e);
+        }
+        (void)dev_free(d->mat);
+        d->mat = w;
+        d->mat_bytes = bytes;
+    }
+    uint32_t *drm = nullptr;
+    FDTD_TRY(dev_malloc(&drm, remap.size() * sizeof(uint32_t)));
+    hipError_t e = hipMemcpyAsync(drm, remap.data(), remap.size() * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) {
+        const int xb = (d->I + kBlock - 1) / kBlock;
+        const unsigned nb = (unsigned)((long long)(p.y1 - p.y0) * (p.z1 - p.z0) * xb);
+        if (d->mat_bytes == 1)
+            hipLaunchKernelGGL(k_paint<uint8_t>, dim3(nb), dim3(kBlock), 0, d->stream, static_cast<uint8_t *>(d->mat),
+                               drm, d->I, d->J, p, xb);
+        else if (d->mat_bytes == 2)
+            hipLaunchKernelGGL(k_paint<uint16_t>, dim3(nb), dim3(kBlock), 0, d->stream,
+                               static_cast<uint16_t *>(d->mat), drm, d->I, d->J, p, xb);
+        else
+            hipLaunchKernelGGL(k_paint<uint32_t>, dim3(nb), dim3(kBlock), 0, d->stream,
+                               static_cast<uint32_t *>(d->mat), drm, d->I, d->J, p, xb);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)dev_free(drm);
+    if (e != hipSuccess) return fail("paint", e);
+    if (nmat != nmat0) {
+        (void)dev_free(d->d_props);
+        (void)dev_free(d->d_coef);
+        d->d_props = nullptr;
+        d->d_coef = nullptr;
+        FDTD_TRY(dev_malloc(&d->d_props, (size_t)nmat * kPropStride * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->d_coef, (size_t)2 * nmat * kCoefStride * sizeof(float)));
+        FDTD_TRY(hipMemcpy(d->d_props, props.data(), props.size() * sizeof(float), hipMemcpyHostToDevice));
+        d->props = std::move(props);
+        d->nmat = nmat;
+        d->coef_dt = NAN;
+    }
+    d->mat_paints++;
+    return 0;
+}
+
+}  // namespace
+
+int fdtd_dev_paint_taper(fdtd_dev *d, float ep, float mu, int start_left, int start_right, int end_left,
+                         int end_right, int start_z, int end_z, int bottom, int top) {
+    PaintArgs p{};
+    p.taper = 1;
+    p.x0 = start_left;
+    p.x1 = start_right;
+    p.y0 = bottom;
+    p.y1 = top;
+    p.z0 = start_z;
+    p.z1 = end_z;
+    p.zs = start_z;
+    if (end_z > start_z) {
+        p.rs = (double)(end_right - start_right) / ((double)(end_z - start_z));
+        p.ls = (double)(end_left - start_left) / ((double)(end_z - start_z));
+    }
+    return paint(d, p, ep, mu, true);
+}
+
+int fdtd_dev_paint_prism(fdtd_dev *d, float ep, float mu, int set_mu, int x0, int x1, int y0, int y1, int z0, int z1) {
+    PaintArgs p{};
+    p.x0 = x0 < 0 ? 0 : x0;
+    p.x1 = x1 > d->I ? d->I : x1;
+    p.y0 = y0;
+    p.y1 = y1;
+    p.z0 = z0;
+    p.z1 = z1;
+    if (p.x1 <= p.x0) return 0;
+    return paint(d, p, ep, mu, set_mu != 0);
+}
+
+int fdtd_dev_get_materials(fdtd_dev *d, float *ep, float *mu, float *sig) {
+    if (!d->have_mats) {
+        set_error("get_materials: the grid has no materials yet");
+        return -2;
+    }
+    return expand_materials(d, ep, mu, sig);
+}
+
+int fdtd_dev_material_counts(fdtd_dev *d, long long counts[3]) {
+    counts[0] = d->mat_uploads;
+    counts[1] = d->mat_paints;
+    counts[2] = d->nmat;
     return 0;
 }
 
@@ -417,6 +628,7 @@ int fdtd_dev_set_materials_extruded(fdtd_dev *d, int nmat, const float *ep, cons
     d->mat_bytes = bytes;
     d->coef_dt = NAN;
     d->have_mats = true;
+    d->mat_uploads++;
     return 0;
 }
 

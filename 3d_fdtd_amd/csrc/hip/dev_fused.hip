@@ -605,7 +605,7 @@ void fused_swap(fdtd_dev *d) {
 }
 
 bool fused_supported(const fdtd_dev *d) {
-    if (!d->cpml.on || d->dense_src || d->src.size() > 2) return false;
+    if (!d->cpml.on || d->dense_src || d->src.size() > 2 || d->mat_bytes > 2) return false;
     const Cpml &cp = d->cpml;
     long long pmem = 4 * (cp.psi_n[0] + cp.psi_n[1] + cp.psi_n[2]);
     return 6 * d->Nal < (1LL << 31) && pmem < (1LL << 31);

@@ -181,7 +181,8 @@ struct fdtd_dev {
 
     // materials
     int nmat;
-    int mat_bytes;        // 1 or 2
+    int mat_bytes;        // 1, 2 or 4 (4: split kernels only)
+    long long mat_uploads = 0, mat_paints = 0;  // full material uploads, device repaints
     void *mat;            // per-cell id
     std::vector<float> props;  // nmat * kPropStride
     float *d_props;

@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kLeapBlockW * kLeapBlockH) void k_leap(LeapArgs L) 
 bool leap_supported(const fdtd_dev *d) {
     const Cpml &cp = d->cpml;
     const long long pmem = 4 * (cp.psi_n[0] + cp.psi_n[1] + cp.psi_n[2]);
-    return cp.on && cp.psi_base && !d->dense_src && d->I >= 3 && d->J >= 3 && d->K >= 3 && 6 * d->Nal < (1LL << 30) &&
+    return cp.on && cp.psi_base && !d->dense_src && d->mat_bytes <= 2 && d->I >= 3 && d->J >= 3 && d->K >= 3 && 6 * d->Nal < (1LL << 30) &&
            pmem < (1LL << 30) && (long long)d->src.size() <= kMaxSrcPlanes;
 }
 

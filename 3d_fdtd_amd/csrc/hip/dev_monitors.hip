@@ -340,8 +340,12 @@ int fdtd_dev_mode_source(fdtd_dev *d, int cx, int cy, int cz, const double *mode
         hipLaunchKernelGGL(k_mode_source<uint8_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, sp->stride, d->mat,
                            d->d_props, d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase,
                            amplitude, msign, adjoint);
-    else
+    else if (d->mat_bytes == 2)
         hipLaunchKernelGGL(k_mode_source<uint16_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, sp->stride, d->mat,
+                           d->d_props, d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase,
+                           amplitude, msign, adjoint);
+    else
+        hipLaunchKernelGGL(k_mode_source<uint32_t>, dim3(nb), dim3(kBlock), 0, d->stream, sp->J, sp->stride, d->mat,
                            d->d_props, d->I, d->J, d->P, cz, cx, cy, dm, mi, mj, ratio, cos_phase, sin_phase,
                            amplitude, msign, adjoint);
     FDTD_TRY(hipGetLastError());

@@ -338,14 +338,16 @@ int fdtd_dev_run_add_source(fdtd_dev *d, int cx, int cy, int cz, const double *m
     const double ratio = d->disc / mdisc;
     const int msign = -1 + 2 * reverse;
     // ep/mu of the plane cells from the material table
-    std::vector<uint16_t> ids((size_t)P);
+    std::vector<uint32_t> ids((size_t)P);
     {
         std::vector<unsigned char> raw((size_t)P * d->mat_bytes);
         FDTD_TRY(hipStreamSynchronize(d->stream));
         FDTD_TRY(hipMemcpy(raw.data(), (const char *)d->mat + (size_t)cz * P * d->mat_bytes, raw.size(),
                            hipMemcpyDeviceToHost));
         for (long long p = 0; p < P; p++)
-            ids[p] = d->mat_bytes == 1 ? raw[p] : reinterpret_cast<const uint16_t *>(raw.data())[p];
+            ids[p] = d->mat_bytes == 1   ? raw[p]
+                     : d->mat_bytes == 2 ? reinterpret_cast<const uint16_t *>(raw.data())[p]
+                                         : reinterpret_cast<const uint32_t *>(raw.data())[p];
     }
     std::vector<float> PR((size_t)6 * P, 0.0f), PI((size_t)6 * P, 0.0f);
     std::vector<unsigned char> win((size_t)P, 0);

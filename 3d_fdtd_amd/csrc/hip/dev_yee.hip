@@ -223,8 +223,10 @@ PmlArgs pml_args(fdtd_dev *d) {
 
 // Off: cell-offset type of the kernels (uint32_t for grids below 2^30 cells, else long long).
 // nt: streaming loads of the own-cell fields (PML grids beyond the last-level cache only).
+// mb: material id bytes; 4-byte ids (per-cell media) take the runtime-chunk kernels only (KC 0),
+// without the streaming hint.
 template <int KC, typename Off>
-void launch_split(bool e_field, bool mon, bool u8, bool pml, bool nt, dim3 grid, hipStream_t st, hipEvent_t t0,
+void launch_split(bool e_field, bool mon, int mb, bool pml, bool nt, dim3 grid, hipStream_t st, hipEvent_t t0,
                   hipEvent_t t1, const YeeArgs &a, const PmlArgs &pm) {
 #define FDTD_GO(KER) hipExtLaunchKernelGGL(KER, grid, dim3(kBlock), 0, st, t0, t1, 0, a, pm)
 #define FDTD_PML(KERN, MT)                                  \
@@ -232,6 +234,18 @@ void launch_split(bool e_field, bool mon, bool u8, bool pml, bool nt, dim3 grid,
         if (nt) FDTD_GO((KERN<MT, true, KC, Off, true>));   \
         else FDTD_GO((KERN<MT, true, KC, Off, false>));     \
     } while (0)
+    if constexpr (KC == 0) {
+        if (mb == 4) {
+            if (e_field && pml) FDTD_GO((k_update_E<uint32_t, true, 0, Off, false>));
+            else if (e_field) FDTD_GO((k_update_E<uint32_t, false, 0, Off, false>));
+            else if (!mon && pml) FDTD_GO((k_update_H<uint32_t, true, 0, Off, false>));
+            else if (!mon) FDTD_GO((k_update_H<uint32_t, false, 0, Off, false>));
+            else if (pml) FDTD_GO((k_update_H_mon<uint32_t, true, 0, Off, false>));
+            else FDTD_GO((k_update_H_mon<uint32_t, false, 0, Off, false>));
+            return;
+        }
+    }
+    const bool u8 = mb == 1;
     if (e_field) {
         if (u8 && pml) FDTD_PML(k_update_E, uint8_t);
         else if (u8) FDTD_GO((k_update_E<uint8_t, false, KC, Off, false>));
@@ -320,19 +334,21 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
     for (int q = 0; q < a.nmon; q++) a.mon[q] = d->run_mon[q];
     if (mon) d->n_run_mon = 0;  // consumed
     dim3 grid((unsigned)(a.plane_blocks * zchunks));
-    const bool u8 = d->mat_bytes == 1;
+    const int mb = d->mat_bytes;
     const bool nt = pml && streaming_loads(d->N, d->stream_mode);
     hipEvent_t t0, t1;
     timing_events(d, e_field ? 0 : 1, &t0, &t1);
-    if (d->N < kSmallGridCells) {
+    if (d->N < kSmallGridCells && mb != 4) {
         switch (a.kchunk) {
-            case 2: launch_split<2, uint32_t>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm); break;
-            case 3: launch_split<3, uint32_t>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm); break;
-            case 4: launch_split<4, uint32_t>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm); break;
-            default: launch_split<0, uint32_t>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            case 2: launch_split<2, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            case 3: launch_split<3, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            case 4: launch_split<4, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            default: launch_split<0, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm); break;
         }
+    } else if (d->N < kSmallGridCells) {
+        launch_split<0, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm);
     } else {
-        launch_split<0, long long>(e_field, mon, u8, pml, nt, grid, d->stream, t0, t1, a, pm);
+        launch_split<0, long long>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm);
     }
     FDTD_TRY(hipGetLastError());
     return 0;
@@ -360,8 +376,11 @@ int launch_mur(fdtd_dev *d, double dt, int k0, int k1, int field) {
             unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
             if (d->mat_bytes == 1)
                 hipLaunchKernelGGL(k_mur_face<uint8_t>, dim3(nb), dim3(kBlock), 0, d->stream, fa, a, side, field, k0, k1);
-            else
+            else if (d->mat_bytes == 2)
                 hipLaunchKernelGGL(k_mur_face<uint16_t>, dim3(nb), dim3(kBlock), 0, d->stream, fa, a, side, field, k0,
+                                   k1);
+            else
+                hipLaunchKernelGGL(k_mur_face<uint32_t>, dim3(nb), dim3(kBlock), 0, d->stream, fa, a, side, field, k0,
                                    k1);
             FDTD_TRY(hipGetLastError());
         }
@@ -380,7 +399,7 @@ int cpml_zero(fdtd_dev *d) {
 }
 
 bool step_uses_split(const fdtd_dev *d) {
-    if (!d->cpml.on) return true;
+    if (!d->cpml.on || d->mat_bytes == 4) return true;
     if (d->fused_rows > 0) return !fused_supported(d);
     if (d->fused_rows == -1) return !col_supported(d);
     return true;  // split pair, or leap launches (a lone step is a split pair; runs use leap_run)

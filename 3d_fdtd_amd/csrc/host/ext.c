@@ -62,6 +62,14 @@ int fdtd3d_ext_count_nonfinite(timefield3d psi, long long *count) {
 
 int fdtd3d_ext_set_finite_check(timefield3d psi, int every) { return fdtd_dev_set_finite_check(tf_dev(psi), every); }
 
+int fdtd3d_ext_device_materials(timefield3d psi, float *ep, float *mu, float *sig) {
+    return fdtd_dev_get_materials(tf_dev(psi), ep, mu, sig);
+}
+
+int fdtd3d_ext_material_counts(timefield3d psi, long long counts[3]) {
+    return fdtd_dev_material_counts(tf_dev(psi), counts);
+}
+
 int fdtd3d_ext_set_step_kernel(timefield3d psi, int rows, int kchunk) {
     return fdtd_dev_set_step_kernel(tf_dev(psi), rows, kchunk);
 }

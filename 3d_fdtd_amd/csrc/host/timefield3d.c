@@ -78,6 +78,13 @@ fdtd_dev *tf_dev(timefield3d psi) {
 
 static void mark_materials(timefield3d psi) { tf_lookup(psi)->mats_dirty = 1; }
 
+/* The device state of psi when its materials are current there (a setter can then repaint the
+ * device ids itself instead of leaving the whole volume to the next upload), else NULL. */
+static fdtd_dev *current_device(timefield3d psi) {
+    tf_record *r = tf_lookup(psi);
+    return r->dev && !r->mats_dirty ? r->dev : NULL;
+}
+
 static long long ncell(timefield3d psi) { return (long long)psi.i * psi.j * psi.k; }
 
 static void pull_fields(timefield3d psi) { dev_check(fdtd_dev_get_fields(tf_dev(psi), psi.fields), "field download"); }
@@ -325,7 +332,11 @@ void timefield3d_set_prism_permittivity(timefield3d psi, float ep, int x0, int x
                 long long q = 3 * (i + (long long)psi.i * (j + (long long)psi.j * k));
                 psi.ep[q] = psi.ep[q + 1] = psi.ep[q + 2] = ep;
             }
-    mark_materials(psi);
+    fdtd_dev *d = current_device(psi);
+    if (d)
+        dev_check(fdtd_dev_paint_prism(d, ep, 1.0f, 0, x0, x1, y0, y1, z0, z1), "prism on the device");
+    else
+        mark_materials(psi);
 }
 
 void timefield3d_set_prism_electric_conductivity(timefield3d psi, float sig, int x0, int x1, int y0, int y1, int z0,
@@ -371,7 +382,13 @@ void timefield3d_set_taper(timefield3d psi, double ep, double mu, int start_left
                 }
             }
         }
-    mark_materials(psi);
+    fdtd_dev *d = current_device(psi);
+    if (d)
+        dev_check(fdtd_dev_paint_taper(d, (float)ep, (float)mu, start_left, start_right, end_left, end_right, start_z,
+                                       end_z, bottom, top),
+                  "taper on the device");
+    else
+        mark_materials(psi);
 }
 
 /* ------------------------------------------------------------------------------------------ */

@@ -27,8 +27,10 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc);
 void fdtd_dev_destroy(fdtd_dev *d);                                           /* timefield3d.c:43  */
 
 /* Materials from host AoS float arrays: unique (ep[3], mu[3], sig[6]) tuples become a table and
- * each cell stores a 1-byte id (2-byte when more than 256 tuples).  mu or sig may be NULL:
- * mu = 1 and sig = 0 everywhere. */
+ * each cell stores a 1-byte id (2-byte above 256 tuples, 4-byte above 65536: a per-cell graded
+ * medium such as a block after adjoint3d_adjust_permittivity, whose table is then as long as the
+ * grid).  Grids with 4-byte ids always step on the split kernels; batched runs and the fused and
+ * leap launch shapes refuse them.  mu or sig may be NULL: mu = 1 and sig = 0 everywhere. */
 int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const float *sig);
 
 /* Extruded geometry rasterised on the device (no reference counterpart; it serves the design
@@ -39,6 +41,22 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
  * K * I ids instead of 3 * N permittivities. */
 int fdtd_dev_set_materials_extruded(fdtd_dev *d, int nmat, const float *ep, const float *mu, const float *sig,
                                     const uint16_t *ids, int j0, int j1, int background);
+
+/* The reference's geometry setters applied to the device material ids, for grids whose device
+ * materials are current (no per-cell upload; the host builds a table remap and a kernel rewrites
+ * the ids of the region).  Each cell of the region keeps its conductivities.
+ * paint_taper: the rule of timefield3d_set_taper (timefield3d.c:1511-1535): for y in [bottom, top)
+ * and z in [start_z, end_z), x in [start_left + round(sl (z - start_z)), start_right +
+ * round(sr (z - start_z))) with the slopes sl, sr in double; ep and mu of all three components.
+ * paint_prism: x in [x0, x1), y in [y0, y1), z in [z0, z1) (timefield3d.c:1455-1474, the
+ * waveguide setter of 1506-1509); ep, and mu when set_mu.  Cells outside the grid are skipped. */
+int fdtd_dev_paint_taper(fdtd_dev *d, float ep, float mu, int start_left, int start_right, int end_left,
+                         int end_right, int start_z, int end_z, int bottom, int top);
+int fdtd_dev_paint_prism(fdtd_dev *d, float ep, float mu, int set_mu, int x0, int x1, int y0, int y1, int z0, int z1);
+/* Per-cell materials as the device holds them (AoS ep[3N], mu[3N], sig[6N]; NULL skips one), and
+ * counts[0] full material uploads, counts[1] device repaints, counts[2] table entries. */
+int fdtd_dev_get_materials(fdtd_dev *d, float *ep, float *mu, float *sig);
+int fdtd_dev_material_counts(fdtd_dev *d, long long counts[3]);
 
 /* Fields / sources / shells / DFT accumulators between host AoS arrays and the device. */
 int fdtd_dev_set_fields(fdtd_dev *d, const double *fields);

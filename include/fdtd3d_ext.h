@@ -35,6 +35,14 @@ int fdtd3d_ext_kernel_times(timefield3d psi, int counts[3], double ms[3]);
  * see fdtd_dev_count_nonfinite / fdtd_dev_set_finite_check. */
 int fdtd3d_ext_count_nonfinite(timefield3d psi, long long *count);
 int fdtd3d_ext_set_finite_check(timefield3d psi, int every);
+
+/* The device's copy of the materials (no counterpart in the reference): per-cell AoS ep, mu, sig
+ * (NULL skips one) expanded from the device ids, and counts[0] full material uploads, counts[1]
+ * setter calls applied on the device (timefield3d_set_taper, _set_prism_permittivity,
+ * _set_waveguide and adjoint3d_taper_insert repaint the device ids when the device materials are
+ * current), counts[2] material table entries. */
+int fdtd3d_ext_device_materials(timefield3d psi, float *ep, float *mu, float *sig);
+int fdtd3d_ext_material_counts(timefield3d psi, long long counts[3]);
 /* Under the PML boundary: rows = 0 selects the split E/H kernels (kchunk > 0: z planes per
  * thread, 0: automatic), rows = -1 the fused one-kernel step in columns (kchunk = 1, 2 or 4 z
  * planes per thread), rows = 1, 2, 4 or 8 the fused step in register strips (rows per lane,

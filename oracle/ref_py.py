@@ -90,6 +90,10 @@ def lib():
         "matrix_diagonalize_non_hermitian": ([Matrix, d, i], Matrix),
         "matrix_diagonalize_2_by_2": ([Matrix], Matrix),
         "mode_new": ([i, i, d], Mode),
+        "mode_file_print": ([C.c_void_p, Mode, i], None),
+        "mode_file_load": ([C.c_void_p], Mode),
+        "timefield3d_TE_gaussian_source": ([TimeField, i, i, i, d, d, d, d], None),
+        "timefield3d_TM_gaussian_source": ([TimeField, i, i, i, d, d, d, d], None),
         "mode_inner_product": ([Mode, Mode], Complex),
         "mode_norm": ([Mode], d),
         "mode_normalize": ([Mode], None),

@@ -357,6 +357,68 @@ def f11(L, manifest):
                                      "pml": "mode_set_pml(0.8, order 2, 4 cells)", "seed": 1111}
 
 
+def f10(L, manifest):
+    """F10: the mode text file (mode.c:843-1038).  A 9 x 7 mode with random complex ep, mu and
+    fields (magnitudes up to 1e3, both signs) written by the reference with dimension 2 and 1, and
+    the reference's own loads of those two files."""
+    import tempfile
+    rng = np.random.default_rng(1010)
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    mi, mj, mdisc = 9, 7, 0.0125
+    md = L.mode_new(mi, mj, mdisc)
+    arrays = {}
+    for name, mat, n in (("ep", md.ep, 3), ("mu", md.mu, 3), ("fields", md.fields, 6)):
+        v = R.view(mat.m, n * mi * mj, np.float64)  # n*mi*mj complex entries as interleaved doubles
+        v[:] = rng.uniform(-1, 1, v.size) * 10.0 ** rng.integers(-3, 4, v.size)
+        arrays[name] = v.copy()
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for dim in (2, 1):
+            path = os.path.join(tmp, f"mode{dim}.txt").encode()
+            f = libc.fopen(path, b"w")
+            L.mode_file_print(f, md, dim)
+            libc.fclose(f)
+            out[f"file{dim}"] = np.frombuffer(open(path, "rb").read(), np.uint8).copy()
+            f = libc.fopen(path, b"r")
+            ld = L.mode_file_load(f)
+            libc.fclose(f)
+            out[f"load{dim}_ep"] = R.view(ld.ep.m, 3 * mi * mj, np.float64).copy()
+            out[f"load{dim}_mu"] = R.view(ld.mu.m, 3 * mi * mj, np.float64).copy()
+            out[f"load{dim}_fields"] = R.view(ld.fields.m, 6 * mi * mj, np.float64).copy()
+            L.mode_free(ld)
+    L.mode_free(md)
+    save("F10_mode_file.npz", mi=mi, mj=mj, mdisc=mdisc, **{"in_" + k: v for k, v in arrays.items()}, **out)
+    manifest["F10_mode_file"] = {"mode": "9x7, disc 0.0125, ep/mu/fields U[-1,1] x 10^U{-3..3} seed 1010",
+                                 "files": "dimension 2 and dimension 1, with the reference's loads of each"}
+
+
+def f13(L, manifest):
+    """F13: the Gaussian TE and TM sources (timefield3d.c:3070-3174) on a 30 x 26 x 12 grid with
+    random ep, mu and sigma: the whole source array after a TE source of width 0.2 um at
+    (15, 13, 5) (its 30 x 30 mode window wraps across rows) and a TM source of width 0.1 um at
+    (4, 20, 8) near the corner."""
+    rng = np.random.default_rng(1313)
+    I, J, K = 30, 26, 12
+    tf, A = new_tf(L, I, J, K)
+    A["ep"][:] = rng.uniform(1.0, 12.0, A["ep"].size).astype(np.float32)
+    A["mu"][:] = rng.uniform(1.0, 2.0, A["mu"].size).astype(np.float32)
+    A["sig"][:] = rng.uniform(0.0, 0.5, A["sig"].size).astype(np.float32)
+    ep, mu, sig = A["ep"].copy(), A["mu"].copy(), A["sig"].copy()
+    omega = 2 * np.pi / 1.55
+    L.timefield3d_TE_gaussian_source(tf, 15, 13, 5, 0.2, omega, 0.7, 1.3)
+    L.timefield3d_TM_gaussian_source(tf, 4, 20, 8, 0.1, omega, -2.1, 0.6)
+    save("F13_gaussian_sources.npz", dims=np.array([I, J, K]), disc=DISC, ep=ep, mu=mu, sig=sig, omega=omega,
+         te=np.array([15, 13, 5, 0.2, 0.7, 1.3]), tm=np.array([4, 20, 8, 0.1, -2.1, 0.6]),
+         sources=A["sources"].copy())
+    manifest["F13_gaussian_sources"] = {"grid": [I, J, K], "materials": "ep U[1,12], mu U[1,2], sig U[0,0.5] seed 1313",
+                                        "TE": "(15,13,5) width 0.2 phase 0.7 amplitude 1.3",
+                                        "TM": "(4,20,8) width 0.1 phase -2.1 amplitude 0.6", "lambda_um": 1.55}
+    L.timefield3d_free(tf)
+
+
 def f12(L, manifest):
     """F12: the general matrix and mode helpers -- every minor of a 4 x 5 matrix, determinants of
     1 x 1 to 6 x 6 random and of a singular 5 x 5, the closed-form 2 x 2 eigenvalues, the QL
@@ -462,8 +524,10 @@ def main():
     f6(L, manifest)
     f7(L, manifest)
     f9(L, manifest)
+    f10(L, manifest)
     f11(L, manifest)
     f12(L, manifest)
+    f13(L, manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, sort_keys=True)
     print("fixtures written to", HERE)

@@ -325,3 +325,124 @@ def test_taper_store_and_gradients(golden_dir):
     rg = np.ctypeslib.as_array(tp.right_grad, shape=(seg + 1,))
     assert np.allclose(lg, z["left_grad"], rtol=1e-12, atol=0) and np.allclose(rg, z["right_grad"], rtol=1e-12, atol=0)
     g.free()
+
+
+def test_block_adjust_over_65536_cells_vs_oracle():
+    """A design step of the block adjoint over 76,000 cells gives nearly every cell of the block its own
+    permittivity (random forward/adjoint fields, so every gradient value differs): more distinct
+    (ep, mu, sigma) tuples than 16-bit material ids hold.  The drop-in step then runs on 4-byte ids
+    and must match the FP64 oracle on the same permittivity over 30 steps."""
+    I, J, K = 56, 46, 44
+    box = (3, 53, 3, 43, 3, 41)  # 50 x 40 x 38 = 76,000 cells
+    T = 2
+    rng = np.random.default_rng(6553)
+    g = fd.Grid(I, J, K, 0.02)
+    blk = L().ajoint3d_new(*box, T)
+    n6 = g.a["fields"].size
+    for t in range(T):
+        g.a["fields"][:] = rng.uniform(-1, 1, n6)
+        g.push()
+        L().adjoint3d_store_fwd_fields(blk, g.tf, t)
+    for t in range(T - 1, -1, -1):
+        g.a["fields"][:] = rng.uniform(-1, 1, n6)
+        g.push()
+        L().adjoint3d_update_gradient(blk, g.tf, t)
+    L().timefield3d_set_background(g.tf, 2.0, 1.0)
+    # the step moves the permittivity by improvement * g / (|g|^2 disc^2 omega): 0.1 rms here, so
+    # nearly every cell takes its own float value (a smaller step repeats values at the float
+    # spacing near 2, a larger one clips at the bounds)
+    L().adjoint3d_adjust_permittivity(blk, g.tf, 4.0, 10.0, 1.5, 12.0)
+    ep = g.a["ep"].copy()
+    distinct = len(np.unique(ep.reshape(-1, 3), axis=0))
+    say("distinct permittivity tuples after the design step:", distinct)
+    assert distinct > 65536
+    f0 = rng.uniform(-1, 1, n6)
+    g.a["fields"][:] = f0
+    g.push()
+    dt = 0.99 * 0.02 / np.sqrt(3)
+    g.step(dt, 30)
+    got = g.fields().copy()
+    o = O.Grid(I, J, K, 0.02)
+    o.ep[:] = ep
+    o.f[:] = f0
+    for _ in range(30):
+        o.step(dt)
+    err = rel_l2(got, o.f)
+    say("graded block, 30 drop-in steps: rel L2 vs FP64 oracle", err)
+    assert err <= REL_L2
+    L().adjoint3d_free(blk)
+    g.free()
+
+
+def test_gaussian_sources_vs_reference(golden_dir):
+    """F13: TE and TM Gaussian sources written through the drop-in surface equal the reference's
+    whole source array bit for bit (same double arithmetic, rounded once to float)."""
+    z = load(golden_dir, "F13_gaussian_sources.npz")
+    I, J, K = (int(v) for v in z["dims"])
+    g = fd.Grid(I, J, K, float(z["disc"]))
+    g.a["ep"][:] = z["ep"]
+    g.a["mu"][:] = z["mu"]
+    g.a["sig"][:] = z["sig"]
+    om = float(z["omega"])
+    te, tm = z["te"], z["tm"]
+    L().timefield3d_TE_gaussian_source(g.tf, int(te[0]), int(te[1]), int(te[2]), float(te[3]), om, float(te[4]),
+                                       float(te[5]))
+    L().timefield3d_TM_gaussian_source(g.tf, int(tm[0]), int(tm[1]), int(tm[2]), float(tm[3]), om, float(tm[4]),
+                                       float(tm[5]))
+    got = g.pull()["sources"].copy()
+    diff = int(np.count_nonzero(got != z["sources"]))
+    say("gaussian sources: values differing from the reference:", diff, "of", got.size)
+    assert np.array_equal(got, z["sources"])
+    g.free()
+
+
+def test_device_raster_of_reference_setters_vs_reference(golden_dir):
+    """F6's raster (background, top half plane, set_taper, adjoint3d_taper_insert, set_waveguide)
+    with the last three applied to the device ids by repaints (the device materials being current
+    after the first two): the device's per-cell ep and mu equal the reference's raster bit for bit,
+    with one full upload in total, and steps on the repainted grid equal steps on a grid whose
+    materials were uploaded from the same host arrays."""
+    z = load(golden_dir, "F6_taper.npz")
+    seg, start, end, bottom, top = (int(z[k]) for k in ("seg", "start", "end", "bottom", "top"))
+    RI, RJ, RK = (int(v) for v in z["raster_dims"])
+    left = np.ascontiguousarray(z["left"].astype(np.int32))
+    right = np.ascontiguousarray(z["right"].astype(np.int32))
+    ip = C.POINTER(C.c_int)
+    counts = (C.c_longlong * 3)()
+    g = fd.Grid(RI, RJ, RK, 0.02)
+    L().timefield3d_set_background(g.tf, 1.447 ** 2, 1.0)
+    L().timefield3d_set_top_half_plane(g.tf, 1.93 ** 2, 1.0, 20)
+    f0 = np.random.default_rng(66).uniform(-1, 1, g.a["fields"].size)
+    g.a["fields"][:] = f0
+    g.push()  # fields and materials to the device: its materials are current from here on
+    fd.check(L().fdtd3d_ext_material_counts(g.tf, counts), "material counts")
+    assert list(counts)[:2] == [1, 0]
+    L().timefield3d_set_taper(g.tf, 3.51 ** 2, 1.0, 20, 20, 14, 26, 0, 10, 12, 16)
+    tp = L().adjoint3d_taper_new(3.51 ** 2, 1.0, seg, start, end, left.ctypes.data_as(ip), right.ctypes.data_as(ip),
+                                 bottom, top, 1)
+    L().adjoint3d_taper_insert(g.tf, tp)
+    L().timefield3d_set_waveguide(g.tf, 3.51 ** 2, 10, 30, 35, 50, 12, 16)
+    fd.check(L().fdtd3d_ext_material_counts(g.tf, counts), "material counts")
+    say("material uploads, device repaints, table entries:", list(counts))
+    assert counts[0] == 1 and counts[1] == 1 + seg + 1
+    n = RI * RJ * RK
+    ep = np.zeros(3 * n, np.float32)
+    mu = np.zeros(3 * n, np.float32)
+    fp = C.POINTER(C.c_float)
+    fd.check(L().fdtd3d_ext_device_materials(g.tf, ep.ctypes.data_as(fp), mu.ctypes.data_as(fp), None), "materials")
+    assert np.array_equal(ep, z["raster_ep"]) and np.array_equal(mu, z["raster_mu"])
+    assert np.array_equal(g.a["ep"], z["raster_ep"]) and np.array_equal(g.a["mu"], z["raster_mu"])
+    # stepping on the repainted ids equals stepping on the same permittivity uploaded from the host
+    h = fd.Grid(RI, RJ, RK, 0.02)
+    h.a["ep"][:] = z["raster_ep"]
+    h.a["mu"][:] = z["raster_mu"]
+    h.a["fields"][:] = f0
+    h.push()
+    dt = 0.99 * 0.02 / np.sqrt(3)
+    g.step(dt, 20)
+    h.step(dt, 20)
+    fd.check(L().fdtd3d_ext_material_counts(g.tf, counts), "material counts")
+    assert counts[0] == 1  # the steps used the repainted ids, no upload
+    assert np.array_equal(g.fields(), h.fields())
+    g.free()
+    h.free()

@@ -323,3 +323,45 @@ def test_every_reference_declaration_is_exported():
     undeclared = sorted(n for n in ref_names - OUT_OF_SCOPE if n not in declared_functions())
     assert not undeclared, undeclared
     assert OUT_OF_SCOPE <= ref_names  # the list names real reference declarations only
+
+
+def test_mode_file_matches_reference_fixture(golden_dir, tmp_path):
+    """F10: a mode written by the product is byte-identical to the reference's file of the same
+    mode (dimension 2 and 1), so the reference reads the product's files as its own; and the
+    product loads the reference's files to exactly the arrays the reference loads."""
+    z = np.load(os.path.join(golden_dir, "F10_mode_file.npz"))
+    L = fd.lib()
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    L.mode_file_print.argtypes = [C.c_void_p, fd.Mode, C.c_int]
+    L.mode_file_load.argtypes = [C.c_void_p]
+    L.mode_file_load.restype = fd.Mode
+    mi, mj = int(z["mi"]), int(z["mj"])
+    m = L.mode_new(mi, mj, float(z["mdisc"]))
+    fd.matrix_view(m.ep, 3 * mi * mj)[:] = z["in_ep"]
+    fd.matrix_view(m.mu, 3 * mi * mj)[:] = z["in_mu"]
+    fd.matrix_view(m.fields, 6 * mi * mj)[:] = z["in_fields"]
+    for dim in (2, 1):
+        path = str(tmp_path / f"p{dim}.txt").encode()
+        f = libc.fopen(path, b"w")
+        L.mode_file_print(f, m, dim)
+        libc.fclose(f)
+        assert open(path, "rb").read() == bytes(z[f"file{dim}"])
+        ref_path = tmp_path / f"r{dim}.txt"
+        ref_path.write_bytes(bytes(z[f"file{dim}"]))
+        f = libc.fopen(str(ref_path).encode(), b"r")
+        r = L.mode_file_load(f)
+        libc.fclose(f)
+        assert (r.i, r.j, r.disc) == (mi, mj, float(z["mdisc"]))
+        assert np.array_equal(fd.matrix_view(r.ep, 3 * mi * mj), z[f"load{dim}_ep"])
+        assert np.array_equal(fd.matrix_view(r.mu, 3 * mi * mj), z[f"load{dim}_mu"])
+        assert np.array_equal(fd.matrix_view(r.fields, 6 * mi * mj), z[f"load{dim}_fields"])
+        L.mode_free(r)
+    # the 6-decimal text keeps every value to 5e-7 absolute; dimension 1 replicates column i/2
+    got = z["load2_fields"]
+    assert np.abs(got - z["in_fields"]).max() <= 5e-7
+    f1 = z["load1_fields"].reshape(mj, mi, 12)
+    assert np.abs(f1 - z["in_fields"].reshape(mj, mi, 12)[:, mi // 2:mi // 2 + 1, :]).max() <= 5e-7
+    L.mode_free(m)
