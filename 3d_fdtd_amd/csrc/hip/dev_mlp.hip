@@ -15,9 +15,11 @@
 // X (one LDS transpose each, batch dimension padded to a multiple of 8 with zeros).  A block of four
 // 64-wide wavefronts computes a 128 x 128 tile of C: 128 x 32 slices of A and B go through LDS with
 // 16-byte global loads (double-buffered: the next slice's loads are in flight during this slice's
-// matrix-core work), and each wavefront owns a 64 x 64 sub-tile, 2 x 2 tiles of
-// v_mfma_f32_32x32x8_bf16 (lane l feeds A(m0 + l%32, k + 4(l/32) + r) and B(k + 4(l/32) + r,
-// n0 + l%32), r = 0..3, and holds C(m0 + 8j + 4(l/32) + r, n0 + l%32) in accumulator 4j + r).
+// matrix-core work), and each wavefront owns a 64 x 64 sub-tile, 2 x 2 tiles of gfx950's
+// v_mfma_f32_32x32x16_bf16 (lane l feeds A(m0 + l%32, k + 8(l/32) + r) and B(k + 8(l/32) + r,
+// n0 + l%32), r = 0..7, one 16-byte LDS read per operand, and holds C(m0 + 8j + 4(l/32) + r,
+// n0 + l%32), r = 0..3, in accumulator 4j + r): K = 16 per instruction, twice the rate of the
+// 32x32x8 form.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -28,7 +30,7 @@
 namespace fdtd {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __host__ __device__ __forceinline__ unsigned short to_bf16(float x) {
@@ -106,22 +108,22 @@ __global__ __launch_bounds__(256) void k_gemm_nt(NtArgs g) {
     stash(0);
     __syncthreads();
     const int nk = (kend - kbeg + kGBK - 1) / kGBK;
-    const int fr = lane & 31, fk = 4 * (lane >> 5);
+    const int fr = lane & 31, fk = 8 * (lane >> 5);
     for (int kt = 0; kt < nk; kt++) {
         const int cur = kt & 1;
         if (kt + 1 < nk) fetch(kbeg + (kt + 1) * kGBK);
 #pragma unroll
-        for (int ks = 0; ks < kGBK; ks += 8) {
-            bf16x4 af[2], bf[2];
+        for (int ks = 0; ks < kGBK; ks += 16) {
+            bf16x8 af[2], bf[2];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                af[i] = *reinterpret_cast<const bf16x4 *>(&As[cur][(wm + 32 * i + fr) * kGLD + ks + fk]);
-                bf[i] = *reinterpret_cast<const bf16x4 *>(&Bs[cur][(wn + 32 * i + fr) * kGLD + ks + fk]);
+                af[i] = *reinterpret_cast<const bf16x8 *>(&As[cur][(wm + 32 * i + fr) * kGLD + ks + fk]);
+                bf[i] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][(wn + 32 * i + fr) * kGLD + ks + fk]);
             }
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(af[i], bf[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
         // the other buffer was last read in the previous slice, before its closing barrier
         if (kt + 1 < nk) stash(cur ^ 1);
@@ -149,18 +151,40 @@ __global__ __launch_bounds__(256) void k_gemm_nt(NtArgs g) {
 
 // out[c][r] = in[r][c] for a bf16 [rows][cols] matrix (row stride ld_in) into [cols][ld_out], rows
 // rows..rows_pad-1 of the result zero (64 x 64 tiles through LDS).
+// Each thread moves groups of 4 elements (8-byte loads along a row, 8-byte stores along an output
+// row) where the group lies inside the matrix and is 8-byte aligned, single elements otherwise.
 __global__ __launch_bounds__(256) void k_transpose_bf16(const unsigned short *in, int rows, int cols, long long ld_in,
                                                         unsigned short *out, long long ld_out, int rows_pad) {
-    __shared__ unsigned short tile[64][66];
+    __shared__ unsigned short tile[64][68];
     const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
-    for (int q = threadIdx.x; q < 64 * 64; q += 256) {
-        const int r = q >> 6, c = q & 63;
-        tile[r][c] = r0 + r < rows && c0 + c < cols ? in[(long long)(r0 + r) * ld_in + c0 + c] : (unsigned short)0;
+    for (int q = threadIdx.x; q < 64 * 16; q += 256) {
+        const int r = q >> 4, c = (q & 15) * 4;
+        const unsigned short *src = in + (long long)(r0 + r) * ld_in + c0 + c;
+        if (r0 + r < rows && c0 + c + 3 < cols && ((uintptr_t)src & 7) == 0) {
+            const uint2 u = *reinterpret_cast<const uint2 *>(src);
+            tile[r][c] = (unsigned short)(u.x & 0xffffu);
+            tile[r][c + 1] = (unsigned short)(u.x >> 16);
+            tile[r][c + 2] = (unsigned short)(u.y & 0xffffu);
+            tile[r][c + 3] = (unsigned short)(u.y >> 16);
+        } else {
+            for (int e = 0; e < 4; e++)
+                tile[r][c + e] = r0 + r < rows && c0 + c + e < cols ? src[e] : (unsigned short)0;
+        }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < 64 * 64; q += 256) {
-        const int c = q >> 6, r = q & 63;
-        if (c0 + c < cols && r0 + r < rows_pad) out[(long long)(c0 + c) * ld_out + r0 + r] = tile[r][c];
+    for (int q = threadIdx.x; q < 64 * 16; q += 256) {
+        const int c = q >> 4, r = (q & 15) * 4;
+        if (c0 + c >= cols) continue;
+        unsigned short *dst = out + (long long)(c0 + c) * ld_out + r0 + r;
+        if (r0 + r + 3 < rows_pad && ((uintptr_t)dst & 7) == 0) {
+            uint2 u;
+            u.x = (unsigned)tile[r][c] | ((unsigned)tile[r + 1][c] << 16);
+            u.y = (unsigned)tile[r + 2][c] | ((unsigned)tile[r + 3][c] << 16);
+            *reinterpret_cast<uint2 *>(dst) = u;
+        } else {
+            for (int e = 0; e < 4; e++)
+                if (r0 + r + e < rows_pad) dst[e] = tile[r + e][c];
+        }
     }
 }
 
@@ -205,26 +229,35 @@ __global__ void k_gelu_back(const float *dx, const float *z, long long total, fl
 
 // db[n] = sum over samples of dZ[m][n] (fixed order per column).
 // Column sums of dz [rows][cols] (bias gradients) in a fixed order: block (x, y) sums rows
-// 256 y .. 256 y + 255 of columns 64 x .. 64 x + 63 (4 row lanes per column, then the 4 lanes in
-// order) into part[y][col]; k_colsum_final adds the row blocks in order.
+// kColRows y .. kColRows (y + 1) - 1 of columns 64 x .. 64 x + 63 (4 row lanes per column, then the
+// 4 lanes in order) into part[y][col]; k_colsum_final adds the row blocks in order.  32 rows per
+// block: a 4096-sample batch spreads over 512 blocks instead of idling most of the chip.
+constexpr int kColRows = 32;
 __global__ __launch_bounds__(256) void k_colsum_partial(const float *dz, int rows, int cols, float *part) {
     __shared__ float red[4][64];
     const int cl = threadIdx.x & 63, rl = threadIdx.x >> 6, col = blockIdx.x * 64 + cl;
     float s = 0.0f;
     if (col < cols)
-        for (int r = blockIdx.y * 256 + rl; r < rows && r < (blockIdx.y + 1) * 256; r += 4)
+        for (int r = blockIdx.y * kColRows + rl; r < rows && r < (blockIdx.y + 1) * kColRows; r += 4)
             s += dz[(long long)r * cols + col];
     red[rl][cl] = s;
     __syncthreads();
     if (rl == 0 && col < cols) part[(long long)blockIdx.y * cols + col] = ((red[0][cl] + red[1][cl]) + red[2][cl]) + red[3][cl];
 }
 
-__global__ void k_colsum_final(const float *part, int nblk, int cols, float *db) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// Block of 256 threads = 32 columns x 8 lanes: lane l sums row blocks l, l + 8, ... in order, then
+// the 8 lane sums are added pairwise in a fixed tree.
+__global__ __launch_bounds__(256) void k_colsum_final(const float *part, int nblk, int cols, float *db) {
+    __shared__ float red[8][32];
+    const int cl = threadIdx.x & 31, l = threadIdx.x >> 5, c = blockIdx.x * 32 + cl;
     float s = 0.0f;
-    for (int b = 0; b < nblk; b++) s += part[(long long)b * cols + c];
-    db[c] = s;
+    if (c < cols)
+        for (int b = l; b < nblk; b += 8) s += part[(long long)b * cols + c];
+    red[l][cl] = s;
+    __syncthreads();
+    if (l == 0 && c < cols)
+        db[c] = ((red[0][cl] + red[1][cl]) + (red[2][cl] + red[3][cl])) +
+                ((red[4][cl] + red[5][cl]) + (red[6][cl] + red[7][cl]));
 }
 
 // C[i] = sum over z (in order) of ws[z * mn + i]: the split-K partial products.
@@ -420,7 +453,7 @@ int ensure_capacity(fdtd_mlp *m, int n) {
     FDTD_TRY(dev_malloc(&m->xt, (size_t)m->npad * (m->kin > m->width ? m->kin : m->width) * sizeof(unsigned short)));
     const size_t fmax = (size_t)(m->kin > wmax ? m->kin : wmax);
     FDTD_TRY(dev_malloc(&m->ws, (size_t)kMaxSplit * wmax * fmax * sizeof(float)));
-    FDTD_TRY(dev_malloc(&m->colpart, (size_t)((n + 255) / 256) * wmax * sizeof(float)));
+    FDTD_TRY(dev_malloc(&m->colpart, (size_t)((n + kColRows - 1) / kColRows) * wmax * sizeof(float)));
     FDTD_TRY(dev_malloc(&m->xf, N * m->n_in * sizeof(float)));
     FDTD_TRY(dev_malloc(&m->tf, N * m->n_out * sizeof(float)));
     FDTD_TRY(dev_malloc(&m->partial, (N * m->n_out / 256 + 2) * sizeof(double)));
@@ -473,10 +506,11 @@ int backward(fdtd_mlp *m, int n, double *loss) {
         rc = gemm(m, EPI_STORE, m->dzt, m->npad, m->xt, m->npad, fo, fi, m->npad, m->g + m->woff[l], fi, nullptr,
                   nullptr, 0);
         if (rc) return rc;
-        const int nblk = (n + 255) / 256;
+        const int nblk = (n + kColRows - 1) / kColRows;
         hipLaunchKernelGGL(k_colsum_partial, dim3((unsigned)((fo + 63) / 64), (unsigned)nblk), dim3(256), 0, m->stream,
                            (const float *)m->dz, n, fo, m->colpart);
-        hipLaunchKernelGGL(k_colsum_final, dim3(grid1(fo)), dim3(256), 0, m->stream, (const float *)m->colpart, nblk, fo,
+        hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)((fo + 31) / 32)), dim3(256), 0, m->stream,
+                           (const float *)m->colpart, nblk, fo,
                            m->g + m->boff[l]);
         FDTD_TRY(hipGetLastError());
         if (l == 0) break;

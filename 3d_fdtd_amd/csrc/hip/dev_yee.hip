@@ -128,6 +128,128 @@ __global__ void k_mur_face(FaceArgs fa, int a, int side, int field, int k0, int 
     }
 }
 
+// All six Mur faces of one field in one launch, with the result of the reference's sequential
+// face order (x0, y0, z0, x_end, y_end, z_end; later faces overwrite the edge and corner values of
+// earlier ones, and read the values earlier faces wrote next to them).
+//   One thread per (face, face cell); of the cell's two tangential components it writes those whose
+// last writer in that order is its face.  That value is the face's formula applied to the inner
+// neighbour's value as it stood when the face ran: if an earlier face covers the inner neighbour
+// with the component tangential, that face's formula at the neighbour (and so on inward: a cell
+// lies on at most three faces, so the chain has at most three levels), else the neighbour's value
+// in memory.  Such a base value is never written by any face (were a face to write it, the same
+// face would also cover this chain's first cell later in the order), so the threads do not race,
+// and the float operations are those of the per-face kernels in the same order.
+// Element i of (v0, v1, v2) by selects: indexing a small local array with a runtime index would
+// place it in scratch memory.
+template <typename T>
+__device__ __forceinline__ T sel3(int i, T v0, T v1, T v2) {
+    return i == 0 ? v0 : (i == 1 ? v1 : v2);
+}
+
+template <typename T>
+__device__ __forceinline__ T sel6(int i, const T *v) {
+    return i < 3 ? sel3(i, v[0], v[1], v[2]) : sel3(i - 3, v[3], v[4], v[5]);
+}
+
+// Face g (0..5 = x0, y0, z0, x_end, y_end, z_end) covers cell (x0, x1, x2) of an N0 x N1 x N2 grid.
+__device__ __forceinline__ bool on_face(int x0, int x1, int x2, int g, int N0, int N1, int N2) {
+    const int ag = g % 3;
+    return sel3(ag, x0, x1, x2) == (g < 3 ? 0 : sel3(ag, N0, N1, N2) - 1);
+}
+
+// The latest face before `before` that covers (x0, x1, x2) with component m tangential, or -1.
+__device__ __forceinline__ int earlier_face(int x0, int x1, int x2, int before, int m, int N0, int N1, int N2) {
+    for (int g = before - 1; g >= 0; g--)
+        if (g % 3 != m && on_face(x0, x1, x2, g, N0, N1, N2)) return g;
+    return -1;
+}
+
+// Cell (x0, x1, x2) moved one cell inward along face g's normal.
+__device__ __forceinline__ void inward(int g, int N0, int N1, int N2, int &x0, int &x1, int &x2) {
+    const int ag = g % 3, v = g < 3 ? 1 : sel3(ag, N0, N1, N2) - 2;
+    x0 = ag == 0 ? v : x0;
+    x1 = ag == 1 ? v : x1;
+    x2 = ag == 2 ? v : x2;
+}
+
+// Face g's Mur update of component m at its boundary cell (x0, x1, x2), given the inner
+// neighbour's current value R: sh1 + s (R - sh0) from the shells and the cell's coefficient.
+template <typename MatT>
+__device__ __forceinline__ float mur_formula(const FaceArgs &fa, long long a0, long long a1, long long a2, int field,
+                                             int g, int m, int x0, int x1, int x2, float R) {
+    const int N0 = fa.N[0], N1 = fa.N[1];
+    const int ag = g % 3, qg = m == (ag + 1) % 3 ? 0 : 1;
+    // tangential enumeration axes (face_enum_axes): x faces (z... y, z), y faces (x, z), z faces (x, y)
+    const int u = ag == 0 ? x1 : x0, v = ag == 2 ? x1 : x2, Nu = ag == 0 ? N1 : N0;
+    const long long ar = sel3(ag, a0, a1, a2);
+    const float *sh0 = fa.shell + sel6(g, fa.shell_off) + u + (long long)Nu * v;
+    const float *sh1 = sh0 + 4 * ar;
+    const long long cl = (long long)x0 + (long long)N0 * ((long long)x1 + (long long)N1 * x2);
+    const float cf = fa.coef[(long long)mat_at<MatT>(fa.mat, cl) * kCoefStride + 18 + 6 * field + 2 * ag + qg];
+    return sh1[(2 * field + qg) * ar] + cf * (R - sh0[(2 * field + qg) * ar]);
+}
+
+// All six Mur faces of one field in one launch, with the result of the reference's sequential
+// face order (x0, y0, z0, x_end, y_end, z_end; later faces overwrite the edge and corner values of
+// earlier ones, and read the values earlier faces wrote next to them).
+//   One thread per (face, face cell); of the cell's two tangential components it writes those whose
+// last writer in that order is its face.  That value is the face's formula applied to the inner
+// neighbour's value as it stood when the face ran: if an earlier face covers the inner neighbour
+// with the component tangential, that face's formula at the neighbour (and so on inward: each level
+// moves one more coordinate inward, so the chain has at most three levels), else the neighbour's
+// value in memory.  Such a base value is never written by any face (were a face to write it, the
+// same face would also cover this chain's first cell later in the order), so the threads do not
+// race, and the float operations are those of the per-face kernels in the same order.
+template <typename MatT>
+__global__ void k_mur_all(FaceArgs fa, int field, long long a0, long long a1, long long a2) {
+    const int N0 = fa.N[0], N1 = fa.N[1], N2 = fa.N[2];
+    long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int f = 0;
+    while (f < 6 && r >= sel3(f % 3, a0, a1, a2)) {
+        r -= sel3(f % 3, a0, a1, a2);
+        f++;
+    }
+    if (f >= 6) return;
+    const int a = f % 3;
+    // enumeration (face_enum_axes): u fastest in memory -- y on x faces, x on y and z faces
+    const int Nu = a == 0 ? N1 : N0;
+    const int u = (int)(r % Nu), v = (int)(r / Nu);
+    const int bnd = f < 3 ? 0 : sel3(a, N0, N1, N2) - 1;
+    const int b0 = a == 0 ? bnd : u;
+    const int b1 = a == 1 ? bnd : (a == 0 ? u : v);
+    const int b2 = a == 2 ? bnd : v;
+    const long long cb = (long long)b0 + (long long)N0 * ((long long)b1 + (long long)N1 * b2);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int m = (a + 1 + q) % 3;
+        bool last = true;
+        for (int g = f + 1; g < 6; g++)
+            if (g % 3 != m && on_face(b0, b1, b2, g, N0, N1, N2)) last = false;
+        if (!last) continue;
+        // level 1 (the inner neighbour of this cell) and level 2 (its inner neighbour)
+        int c0 = b0, c1 = b1, c2 = b2;
+        inward(f, N0, N1, N2, c0, c1, c2);
+        const int g1 = earlier_face(c0, c1, c2, f, m, N0, N1, N2);
+        int d0 = c0, d1 = c1, d2 = c2, g2 = -1;
+        if (g1 >= 0) {
+            inward(g1, N0, N1, N2, d0, d1, d2);
+            g2 = earlier_face(d0, d1, d2, g1, m, N0, N1, N2);
+        }
+        int e0 = d0, e1 = d1, e2 = d2;
+        if (g2 >= 0) inward(g2, N0, N1, N2, e0, e1, e2);
+        // base: the neighbour beyond the last level, then the formulas from the innermost out
+        const int z0 = g1 < 0 ? c0 : (g2 < 0 ? d0 : e0);
+        const int z1 = g1 < 0 ? c1 : (g2 < 0 ? d1 : e1);
+        const int z2 = g1 < 0 ? c2 : (g2 < 0 ? d2 : e2);
+        float *Fm = sel6(3 * field + m, fa.F);
+        float R = Fm[(long long)z0 + (long long)N0 * ((long long)z1 + (long long)N1 * z2)];
+        if (g2 >= 0) R = mur_formula<MatT>(fa, a0, a1, a2, field, g2, m, d0, d1, d2, R);
+        if (g1 >= 0) R = mur_formula<MatT>(fa, a0, a1, a2, field, g1, m, c0, c1, c2, R);
+        R = mur_formula<MatT>(fa, a0, a1, a2, field, f, m, b0, b1, b2, R);
+        Fm[cb] = R;
+    }
+}
+
 FaceArgs face_args(fdtd_dev *d, int scheme) {
     FaceArgs fa;
     for (int m = 0; m < 6; m++) fa.F[m] = d->F[m];
@@ -363,6 +485,22 @@ int launch_mur(fdtd_dev *d, double dt, int k0, int k1, int field) {
     k1 = k1 > d->K ? d->K : k1;
     FaceArgs fa = face_args(d, 0);
     const int N[3] = {d->I, d->J, d->K};
+    if (k0 == 0 && k1 == d->K && d->I >= 3 && d->J >= 3 && d->K >= 3) {
+        // the whole boundary: one launch for the six faces
+        const long long ar[3] = {(long long)N[1] * N[2], (long long)N[2] * N[0], (long long)N[0] * N[1]};
+        const long long total = 2 * (ar[0] + ar[1] + ar[2]);
+        const unsigned nb = (unsigned)((total + kBlock - 1) / kBlock);
+        if (d->mat_bytes == 1)
+            hipLaunchKernelGGL(k_mur_all<uint8_t>, dim3(nb), dim3(kBlock), 0, d->stream, fa, field, ar[0], ar[1], ar[2]);
+        else if (d->mat_bytes == 2)
+            hipLaunchKernelGGL(k_mur_all<uint16_t>, dim3(nb), dim3(kBlock), 0, d->stream, fa, field, ar[0], ar[1],
+                               ar[2]);
+        else
+            hipLaunchKernelGGL(k_mur_all<uint32_t>, dim3(nb), dim3(kBlock), 0, d->stream, fa, field, ar[0], ar[1],
+                               ar[2]);
+        FDTD_TRY(hipGetLastError());
+        return 0;
+    }
     // face order x0, y0, z0, x_end, y_end, z_end: later faces overwrite edges and corners
     for (int side = 0; side < 2; side++) {
         for (int a = 0; a < 3; a++) {

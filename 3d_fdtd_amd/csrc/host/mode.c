@@ -248,6 +248,26 @@ complex mode_H_inner_product(mode psi, mode phi) {
     return complex_smul(psi.disc * psi.disc, s);
 }
 
+/* The complex helpers of complex.c, inlined for the per-cell loops below: the same double
+ * operations in the same order (so the same bits), without a call per operation; every drop-in
+ * source and overlap call power-normalises its mode, so these loops run several times a step. */
+static inline complex c_mul(complex x, complex y) {
+    complex z = {x.re * y.re - x.im * y.im, x.re * y.im + x.im * y.re};
+    return z;
+}
+static inline complex c_conj(complex x) {
+    complex z = {x.re, -x.im};
+    return z;
+}
+static inline complex c_add(complex x, complex y) {
+    complex z = {x.re + y.re, x.im + y.im};
+    return z;
+}
+static inline complex c_neg0(complex x) { /* 0 - x, as complex_sub(complex_zero(), x) */
+    complex z = {0.0 - x.re, 0.0 - x.im};
+    return z;
+}
+
 /* -dx^2/2 sum conj(Hx_psi)Ey_phi - conj(Hy_psi)Ex_phi + Hx_phi conj(Ey_psi) - Hy_phi conj(Ex_psi),
  * i outer and j inner. */
 complex mode_power_inner_product(mode psi, mode phi) {
@@ -257,11 +277,11 @@ complex mode_power_inner_product(mode psi, mode phi) {
         for (int jj = 0; jj < psi.j; jj++) {
             const complex *a = psi.fields.m + 6 * (ii + jj * psi.i);
             const complex *b = phi.fields.m + 6 * (ii + jj * psi.i);
-            complex p1 = complex_mul(complex_conj(a[3]), b[1]);
-            complex p2 = complex_sub(cz(), complex_mul(complex_conj(a[4]), b[0]));
-            complex p3 = complex_mul(b[3], complex_conj(a[1]));
-            complex p4 = complex_sub(cz(), complex_mul(b[4], complex_conj(a[0])));
-            s = complex_add(s, complex_add(complex_add(p1, p2), complex_add(p3, p4)));
+            complex p1 = c_mul(c_conj(a[3]), b[1]);
+            complex p2 = c_neg0(c_mul(c_conj(a[4]), b[0]));
+            complex p3 = c_mul(b[3], c_conj(a[1]));
+            complex p4 = c_neg0(c_mul(b[4], c_conj(a[0])));
+            s = c_add(s, c_add(c_add(p1, p2), c_add(p3, p4)));
         }
     return complex_smul(-0.5 * psi.disc * psi.disc, s);
 }
@@ -275,7 +295,10 @@ static void scale_fields(mode psi, double norm) {
     /* x * 1.0 == x for every double, so a mode already normalised (the state of every call after
      * the first in a driver's loop) is left as the pass would leave it, without the pass */
     if (inv == 1.0) return;
-    for (int q = 0; q < 6 * psi.i * psi.j; q++) psi.fields.m[q] = complex_smul(inv, psi.fields.m[q]);
+    for (int q = 0; q < 6 * psi.i * psi.j; q++) {
+        psi.fields.m[q].re = inv * psi.fields.m[q].re;
+        psi.fields.m[q].im = inv * psi.fields.m[q].im;
+    }
 }
 
 void mode_normalize(mode psi) { scale_fields(psi, mode_norm(psi)); }

@@ -88,3 +88,18 @@ def loss_grad(p, x, t, dims, bf16_ops=True):
         dx = dzq @ W
         dz = dx * gelu_grad(cache[l - 1][2])
     return loss, g
+
+
+def adam_update(p, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8, bias_correction=True):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1) in FP32, in the order the device evaluates it:
+    m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, p -= lr (m / c1) / (sqrt(v / c2) + eps) with
+    c1 = 1 - b1^step, c2 = 1 - b2^step.  Returns (p, m, v).  bias_correction=False drops c1 and c2
+    (a planted mistake for the tests)."""
+    f = np.float32
+    p, g, m, v = (np.asarray(a, np.float32) for a in (p, g, m, v))
+    m = f(b1) * m + (f(1) - f(b1)) * g
+    v = f(b2) * v + (f(1) - f(b2)) * g * g
+    c1 = f(1) - f(np.power(f(b1), f(step))) if bias_correction else f(1)
+    c2 = f(1) - f(np.power(f(b2), f(step))) if bias_correction else f(1)
+    p = p - f(lr) * (m / c1) / (np.sqrt(v / c2) + f(eps))
+    return p.astype(np.float32), m, v

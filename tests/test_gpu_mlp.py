@@ -100,3 +100,50 @@ def test_resident_training_equals_step_by_step():
     assert last_b < first_b
     a.free()
     b.free()
+
+
+def test_gradients_match_oracle_at_training_batch():
+    """The training batch of 4096 designs: the weight-gradient GEMMs split K = 4096 over many
+    blocks and reduce the partial sums in a fixed order; every layer's gradient still equals the
+    oracle's FP64 sums to the gate."""
+    p = M.init_params(*DIMS, seed=11)
+    x, t = _case(4096, 12)
+    net = fd.Mlp(*DIMS, params=p)
+    loss, g = net.loss_grad(x, t)
+    loss_o, g_o = M.loss_grad(p, x, t, DIMS)
+    say("batch 4096: loss %.6e vs %.6e" % (loss, loss_o))
+    assert abs(loss - loss_o) <= 1e-3 * abs(loss_o)
+    lay, _ = M.layout(*DIMS)
+    for l, (fin, fout, wo, bo) in enumerate(lay):
+        ew = rel(g[wo:wo + fin * fout], g_o[wo:wo + fin * fout])
+        eb = rel(g[bo:bo + fout], g_o[bo:bo + fout])
+        say("layer", l, "dW rel L2 %.2e, db rel L2 %.2e" % (ew, eb))
+        assert ew <= 1e-2 and eb <= 1e-2
+    g_again = net.loss_grad(x, t)[1]
+    assert np.array_equal(g, g_again)  # deterministic reductions
+    net.free()
+
+
+def test_adam_steps_match_oracle_restatement():
+    """Two Adam steps on the GPU (batch 4096) against the FP32 restatement of Adam fed with the GPU's
+    own gradients (read with loss_grad right before each step): the parameters agree to the last
+    bit or two of FP32.  The first step is also the closed form p - lr g / (|g| + eps)."""
+    p0 = M.init_params(*DIMS, seed=13)
+    x, t = _case(4096, 14)
+    net = fd.Mlp(*DIMS, params=p0)
+    lr = 2e-3
+    g1 = net.loss_grad(x, t)[1]
+    net.adam_step(x, t, lr=lr)
+    p1 = net.params()
+    g2 = net.loss_grad(x, t)[1]
+    net.adam_step(x, t, lr=lr)
+    p2 = net.params()
+    z = np.zeros(p0.size, np.float32)
+    r1, m1, v1 = M.adam_update(p0, g1, z, z, 1, lr)
+    r2, _, _ = M.adam_update(r1, g2, m1, v1, 2, lr)
+    e1 = float(np.abs(p1.astype(np.float64) - r1).max())
+    e2 = float(np.abs(p2.astype(np.float64) - r2).max())
+    closed = float(np.abs(p1.astype(np.float64) - (p0 - lr * g1.astype(np.float64) / (np.abs(g1) + 1e-8))).max())
+    say("adam: step 1 max |dp| %.2e (closed form %.2e), step 2 %.2e" % (e1, closed, e2))
+    assert e1 <= 4e-7 and e2 <= 4e-7 and closed <= 4e-7
+    net.free()

@@ -50,3 +50,23 @@ def test_backward_matches_finite_differences():
     finally:
         M.gelu_grad = saved
     assert np.abs(gbad[idx] - fdg).max() / np.abs(fdg).max() > 1e-2
+
+
+def test_adam_restatement_closed_forms():
+    """The first Adam step from zero moments moves each parameter by lr g / (|g| + eps), whatever g's
+    scale (the bias-corrected moments are g and g^2); with a constant gradient every later step
+    does the same.  Dropping the bias correction (a planted mistake) moves the first step by a
+    factor (1 - b1) / sqrt(1 - b2) = 3.16 instead."""
+    rng = np.random.default_rng(21)
+    p0 = rng.uniform(-1, 1, 1000).astype(np.float32)
+    g = (rng.normal(size=1000) * 10.0 ** rng.integers(-4, 3, 1000)).astype(np.float32)
+    lr = 1e-3
+    want = p0.astype(np.float64) - lr * g / (np.abs(g) + 1e-8)
+    p1, m, v = M.adam_update(p0, g, np.zeros(1000), np.zeros(1000), 1, lr)
+    assert np.abs(p1 - want).max() <= 2e-7
+    p, mm, vv = p1, m, v
+    for step in range(2, 6):
+        p, mm, vv = M.adam_update(p, g, mm, vv, step, lr)
+    assert np.abs((p1 - p) - 4 * (p0 - p1)).max() <= 1e-6
+    bad, _, _ = M.adam_update(p0, g, np.zeros(1000), np.zeros(1000), 1, lr, bias_correction=False)
+    assert np.abs(bad - want).max() > 5e-4
