@@ -494,6 +494,12 @@ class DevGrid:
                   "overlap result")
         return cum[0::2] + 1j * cum[1::2], complex(last[0], last[1])
 
+    def dft_box(self, bid, ncells, nfreq=1):
+        """The box's FP64 DFT accumulators as interleaved (re, im) doubles (fdtd_dev_run_dft_result)."""
+        out = np.zeros(2 * 3 * ncells * nfreq)
+        dev_check(lib().fdtd_dev_run_dft_result(self.d, bid, out.ctypes.data_as(C.POINTER(C.c_double))), "dft result")
+        return out
+
     def fields(self):
         out = np.zeros(6 * self.I * self.J * self.K)
         dev_check(lib().fdtd_dev_get_fields(self.d, out.ctypes.data_as(C.POINTER(C.c_double))), "get_fields")
