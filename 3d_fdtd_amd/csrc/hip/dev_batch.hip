@@ -291,12 +291,18 @@ fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n) {
         set_error("batch: no grids");
         return nullptr;
     }
+    int mb = 1;
     for (int g = 0; g < n; g++) {
-        if (!grids[g] || grids[g]->mat_bytes != grids[0]->mat_bytes || grids[g]->mat_bytes > 2) {
-            set_error("batch: null grid, grids with different material-id widths, or more than 65536 materials "
-                      "in a grid");
+        if (!grids[g] || grids[g]->mat_bytes > 2) {
+            set_error("batch: null grid, or more than 65536 materials in a grid");
             return nullptr;
         }
+        mb = grids[g]->mat_bytes > mb ? grids[g]->mat_bytes : mb;
+    }
+    // one id width for the whole batch: grids with 1-byte ids join with 2-byte ids
+    for (int g = 0; g < n; g++)
+        if (widen_material_ids(grids[g], mb)) return nullptr;
+    for (int g = 0; g < n; g++) {
         if (grids[g]->N >= kSmallGridCells) {
             set_error("batch: every grid must have fewer than 2^30 cells");
             return nullptr;

@@ -210,6 +210,34 @@ const ModeCache *mode_cached(const fdtd_dev *d, const double *mode, int mi, int 
     return nullptr;
 }
 
+// Rewrite the id volume with wider ids (the table outgrew them, or a batch needs one width).
+int widen_material_ids(fdtd_dev *d, int bytes) {
+    if (bytes <= d->mat_bytes) return 0;
+    FDTD_TRY(hipStreamSynchronize(d->stream));  // kernels already enqueued may still read the ids
+    const unsigned nbN = (unsigned)((d->N + kBlock - 1) / kBlock);
+    void *w = nullptr;
+    FDTD_TRY(dev_malloc(&w, (size_t)d->N * bytes));
+    if (d->mat_bytes == 1 && bytes == 2)
+        hipLaunchKernelGGL((k_widen_ids<uint8_t, uint16_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
+                           static_cast<const uint8_t *>(d->mat), static_cast<uint16_t *>(w), d->N);
+    else if (d->mat_bytes == 1)
+        hipLaunchKernelGGL((k_widen_ids<uint8_t, uint32_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
+                           static_cast<const uint8_t *>(d->mat), static_cast<uint32_t *>(w), d->N);
+    else
+        hipLaunchKernelGGL((k_widen_ids<uint16_t, uint32_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
+                           static_cast<const uint16_t *>(d->mat), static_cast<uint32_t *>(w), d->N);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess) {
+        (void)dev_free(w);
+        return fail("widen material ids", e);
+    }
+    (void)dev_free(d->mat);
+    d->mat = w;
+    d->mat_bytes = bytes;
+    return 0;
+}
+
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -425,6 +453,7 @@ int expand_materials(fdtd_dev *d, float *ep, float *mu, float *sig) {
 // its repainted tuple (appending the new ones), widens the id volume when the table outgrows it,
 // and a kernel rewrites the ids of the region's cells: no per-cell upload.
 int paint(fdtd_dev *d, PaintArgs p, float ep, float mu, bool set_mu) {
+    int rc;
     if (!d->have_mats) {
         set_error("paint: the grid has no materials yet");
         return -2;
@@ -462,29 +491,7 @@ int paint(fdtd_dev *d, PaintArgs p, float ep, float mu, bool set_mu) {
     }
     const int nmat = (int)ids.size();
     const int bytes = nmat <= 256 ? 1 : (nmat <= 65536 ? 2 : 4);
-    const unsigned nbN = (unsigned)((d->N + kBlock - 1) / kBlock);
-    if (bytes > d->mat_bytes) {
-        void *w = nullptr;
-        FDTD_TRY(dev_malloc(&w, (size_t)d->N * bytes));
-        if (d->mat_bytes == 1 && bytes == 2)
-            hipLaunchKernelGGL((k_widen_ids<uint8_t, uint16_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
-                               static_cast<const uint8_t *>(d->mat), static_cast<uint16_t *>(w), d->N);
-        else if (d->mat_bytes == 1)
-            hipLaunchKernelGGL((k_widen_ids<uint8_t, uint32_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
-                               static_cast<const uint8_t *>(d->mat), static_cast<uint32_t *>(w), d->N);
-        else
-            hipLaunchKernelGGL((k_widen_ids<uint16_t, uint32_t>), dim3(nbN), dim3(kBlock), 0, d->stream,
-                               static_cast<const uint16_t *>(d->mat), static_cast<uint32_t *>(w), d->N);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
-        if (e != hipSuccess) {
-            (void)dev_free(w);
-            return fail("paint: widen ids", e);
-        }
-        (void)dev_free(d->mat);
-        d->mat = w;
-        d->mat_bytes = bytes;
-    }
+    if (bytes > d->mat_bytes && (rc = widen_material_ids(d, bytes))) return rc;
     uint32_t *drm = nullptr;
     FDTD_TRY(dev_malloc(&drm, remap.size() * sizeof(uint32_t)));
     hipError_t e = hipMemcpyAsync(drm, remap.data(), remap.size() * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream);

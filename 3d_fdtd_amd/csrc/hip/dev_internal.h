@@ -24,6 +24,8 @@ inline hipError_t dev_malloc(T **p, size_t bytes) {
     return dev_malloc_bytes(reinterpret_cast<void **>(p), bytes);
 }
 hipError_t dev_free(void *p);
+// Rewrite grid d's material-id volume with `bytes`-byte ids (no-op when already that wide).
+int widen_material_ids(fdtd_dev *d, int bytes);
 hipError_t dev_stream_acquire(hipStream_t *s);
 hipError_t dev_stream_release(hipStream_t s);
 

@@ -21,12 +21,45 @@ from scipy.interpolate import CubicSpline
 fd = importlib.import_module("3d_fdtd_amd")
 
 
+def port_cross_section(fam, wcells):
+    """Permittivity [j][i][3] (complex) of a straight strip port of wcells cells' width in family
+    fam's cross-section (the fill-fraction raster with whole-cell edges)."""
+    x = np.arange(fam.I) - fam.I / 2
+    w = float(wcells)
+    frac = np.clip(np.minimum(x + 1, w / 2) - np.maximum(x, -w / 2), 0, 1)
+    sec = np.full((fam.J, fam.I, 3), fam.eps_clad, np.complex128)
+    j0 = fam.J // 2 - fam.core // 2
+    sec[j0:j0 + fam.core] = (fam.eps_clad + frac * (fam.eps_core - fam.eps_clad))[None, :, None]
+    return sec
+
+
+def solve_port_mode(fam, wcells, margin):
+    """TE0 (the mode solver's fundamental mode, zero fields on the window edge) of a port of
+    wcells cells at the band centre, on the cross-section window inside `margin` cells of each
+    edge, power-normalised, as the interleaved complex array the run API takes; and n_eff."""
+    sec = port_cross_section(fam, wcells)
+    mi, mj = fam.I - 2 * margin, fam.J - 2 * margin
+    i0, j0 = fam.I // 2 - mi // 2, fam.J // 2 - mj // 2
+    beta, md = fd.solve_mode(sec[j0:j0 + mj, i0:i0 + mi], fam.disc, fam.w0, tol=1e-9, size=20, confinement=2)
+    arr = fd.normalised_mode(mi, mj, fam.disc, fd.mode_fields(md).copy())
+    fd.lib().mode_free(md)
+    return arr, (beta / fam.w0).real
+
+
 class TaperFamily:
-    """Geometry and excitation of a family of taper designs (library axes: k = propagation)."""
+    """Geometry and excitation of a family of taper designs (library axes: k = propagation).
+
+    ports="gaussian": source and monitors use one Gaussian-profile mode of the window;
+    ports="te0": the source and the launched-side monitor use TE0 of the input port (the first
+    control width, rounded to whole cells) and the transmitted-side monitor TE0 of the output port
+    (the last control width), from the mode solver (solve_port_mode, cached per width)."""
 
     def __init__(self, disc=0.05, I=48, J=20, core_cells=4, npml=6, port=4, wmin=0.4, wmax=2.0, lmin=1.5,
-                 lmax=4.0, steps=1200, lam=(1.50, 1.60), nfreq=16):
+                 lmax=4.0, steps=1200, lam=(1.50, 1.60), nfreq=16, ports="gaussian", fixed_ports=None):
         self.disc, self.I, self.J, self.core, self.npml, self.port = disc, I, J, core_cells, npml, port
+        self.ports = ports
+        self.fixed_ports = fixed_ports  # (w_in, w_out): random designs keep the end widths
+        self._te0 = {}
         self.wmin, self.wmax, self.lmin, self.lmax = wmin, wmax, lmin, lmax
         self.steps = steps
         self.eps_core, self.eps_clad = 3.48 ** 2, 1.44 ** 2
@@ -38,17 +71,66 @@ class TaperFamily:
         t = np.arange(steps) * self.dt
         self.amp = np.exp(-(t - 4 * sig_t) ** 2 / (2 * sig_t ** 2))
         self.phase = w0 * t
-        self.mi, self.mj = I - 2 * npml - 4, J - 2 * npml - 2
-        md = fd.lib().mode_gaussian(self.mi, self.mj, disc, fd.Complex(1.44 ** 2, 0), fd.Complex(1.0, 0), w0,
-                                    0.15 * self.mi, 0)
-        self.mode = fd.normalised_mode(self.mi, self.mj, disc, fd.mode_fields(md).copy())
-        fd.lib().mode_free(md)
+        self.w0 = w0
+        if ports == "te0":
+            self.margin = npml + 4
+            self.mi, self.mj = I - 2 * self.margin, J - 2 * self.margin
+            self.mode = None
+        else:
+            self.mi, self.mj = I - 2 * npml - 4, J - 2 * npml - 2
+            md = fd.lib().mode_gaussian(self.mi, self.mj, disc, fd.Complex(1.44 ** 2, 0), fd.Complex(1.0, 0), w0,
+                                        0.15 * self.mi, 0)
+            self.mode = fd.normalised_mode(self.mi, self.mj, disc, fd.mode_fields(md).copy())
+            fd.lib().mode_free(md)
+
+    def port_width_cells(self, w):
+        return int(round(float(w) / self.disc))
+
+    def te0(self, wcells):
+        """TE0 of a port of wcells cells (cached)."""
+        if wcells not in self._te0:
+            self._te0[wcells] = solve_port_mode(self, wcells, self.margin)[0]
+        return self._te0[wcells]
+
+    def port_modes(self, p):
+        """(input-side mode, output-side mode) of design p."""
+        if self.ports != "te0":
+            return self.mode, self.mode
+        return self.te0(self.port_width_cells(p[0])), self.te0(self.port_width_cells(p[7]))
 
     def random(self, rng, n):
-        """n random designs: widths ~ U[wmin, wmax], length ~ U[lmin, lmax]."""
+        """n random designs: widths ~ U[wmin, wmax], length ~ U[lmin, lmax] (end widths fixed when
+        the family has fixed ports)."""
         w = rng.uniform(self.wmin, self.wmax, (n, 8))
         L = rng.uniform(self.lmin, self.lmax, (n, 1))
+        if self.fixed_ports is not None:
+            w[:, 0], w[:, 7] = self.fixed_ports
         return np.hstack([w, L])
+
+    def precompute_ports(self, designs, workers=16):
+        """Solve the TE0 port modes every design needs, in `workers` host processes (the mode
+        solver is host code; each width is solved once)."""
+        if self.ports != "te0":
+            return
+        need = sorted({self.port_width_cells(w) for p in np.asarray(designs) for w in (p[0], p[7])} - set(self._te0))
+        if not need:
+            return
+        if workers <= 1 or len(need) == 1:
+            for wc in need:
+                self.te0(wc)
+            return
+        import multiprocessing as mp
+        ctx = mp.get_context("spawn")  # fresh interpreters: no state of this process is shared
+        with ctx.Pool(min(workers, len(need))) as pool:
+            res = pool.starmap(_solve_port_worker, [(self._spec(), wc) for wc in need])
+        for wc, arr in zip(need, res):
+            self._te0[wc] = arr
+
+    def _spec(self):
+        return dict(disc=self.disc, I=self.I, J=self.J, core_cells=self.core, npml=self.npml, port=self.port,
+                    wmin=self.wmin, wmax=self.wmax, lmin=self.lmin, lmax=self.lmax, steps=1,
+                    lam=(2 * math.pi / self.omegas[0], 2 * math.pi / self.omegas[-1]), nfreq=len(self.omegas),
+                    ports=self.ports)
 
     def normalise(self, p):
         """Surrogate inputs in [-1, 1]."""
@@ -101,20 +183,23 @@ class TaperFamily:
         permittivity, grid, time step, CPML depth, mode window, source schedule, wavelengths,
         source plane and the launched / transmitted monitor planes."""
         ep, K = self.permittivity(p)
-        return dict(ep=ep, I=self.I, J=self.J, K=K, disc=self.disc, dt=self.dt, npml=self.npml, mode=self.mode,
+        m_in, m_out = self.port_modes(p)
+        return dict(ep=ep, I=self.I, J=self.J, K=K, disc=self.disc, dt=self.dt, npml=self.npml, mode=m_in,
                     mi=self.mi, mj=self.mj, amp=self.amp, phase=self.phase, omegas=self.omegas,
-                    src_k=self.npml + 1, mon_ks=(self.npml + 4, K - self.npml - 2))
+                    src_k=self.npml + 1, mon_ks=(self.npml + 4, K - self.npml - 2),
+                    out_mode=(m_out, self.mi, self.mj))
 
     def grid(self, p):
         """A device grid for design p with its source and the launched / transmitted monitors."""
         ex, K = self.extruded(p)
+        m_in, m_out = self.port_modes(p)
         g = fd.DevGrid(self.I, self.J, K, self.disc, None, extruded=ex)
         g.cpml((self.npml,) * 6, 3, 1e-8, 1.0, 0.05)
         ci, cj = self.I // 2, self.J // 2
         ks = self.npml + 1
-        g.add_source(ci, cj, ks, self.mode, self.mi, self.mj, self.disc, self.amp, self.phase)
-        g.mon_in = g.add_overlap(ci, cj, ks + 3, self.mode, self.mi, self.mj, self.disc, self.omegas)
-        g.mon_out = g.add_overlap(ci, cj, K - self.npml - 2, self.mode, self.mi, self.mj, self.disc, self.omegas)
+        g.add_source(ci, cj, ks, m_in, self.mi, self.mj, self.disc, self.amp, self.phase)
+        g.mon_in = g.add_overlap(ci, cj, ks + 3, m_in, self.mi, self.mj, self.disc, self.omegas)
+        g.mon_out = g.add_overlap(ci, cj, K - self.npml - 2, m_out, self.mi, self.mj, self.disc, self.omegas)
         return g
 
     def spline_basis(self, K):
@@ -136,6 +221,7 @@ class TaperFamily:
         holds each edge grows by (eps_core - eps_clad) dw / (2 dx) (the raster is the fill-fraction
         one of permittivity()); the spline basis carries d T / d w(k) to the control points."""
         ex, K = self.extruded(p)
+        m_in, m_out = self.port_modes(p)
         ci, cj = self.I // 2, self.J // 2
         ks, km = self.npml + 1, K - self.npml - 2
         j0 = self.J // 2 - self.core // 2
@@ -144,9 +230,9 @@ class TaperFamily:
         nf = len(self.omegas)
         g = fd.DevGrid(self.I, self.J, K, self.disc, None, extruded=ex)
         g.cpml((self.npml,) * 6, 3, 1e-8, 1.0, 0.05)
-        g.add_source(ci, cj, ks, self.mode, self.mi, self.mj, self.disc, self.amp, self.phase)
-        mon_in = g.add_overlap(ci, cj, ks + 3, self.mode, self.mi, self.mj, self.disc, self.omegas)
-        mon_out = g.add_overlap(ci, cj, km, self.mode, self.mi, self.mj, self.disc, self.omegas)
+        g.add_source(ci, cj, ks, m_in, self.mi, self.mj, self.disc, self.amp, self.phase)
+        mon_in = g.add_overlap(ci, cj, ks + 3, m_in, self.mi, self.mj, self.disc, self.omegas)
+        mon_out = g.add_overlap(ci, cj, km, m_out, self.mi, self.mj, self.disc, self.omegas)
         bid = g.add_dft_box(box, 7, self.omegas)
         g.run(self.dt, 0, self.steps)
         a_in, _ = g.overlap(mon_in, nf)
@@ -154,7 +240,7 @@ class TaperFamily:
         ga = fd.DevGrid(self.I, self.J, K, self.disc, None, extruded=ex)
         ga.cpml((self.npml,) * 6, 3, 1e-8, 1.0, 0.05)
         w0 = self.phase[1] / self.dt
-        ga.add_adjoint_source(ci, cj, km, self.mode, self.mi, self.mj, self.disc, w0 * self.dt, self.amp, self.phase)
+        ga.add_adjoint_source(ci, cj, km, m_out, self.mi, self.mj, self.disc, w0 * self.dt, self.amp, self.phase)
         aid = ga.add_dft_box(box, 7, self.omegas)
         ga.run(self.dt, 0, self.steps, 1.0)
         t = a_out / a_in
@@ -200,6 +286,11 @@ class TaperFamily:
             for g in grids:
                 g.free()
         return np.array(out)
+
+
+def _solve_port_worker(spec, wcells):
+    fam = TaperFamily(**spec)
+    return solve_port_mode(fam, wcells, fam.margin)[0]
 
 
 def adjoint_scale(amp, w, w0, dt):

@@ -95,3 +95,45 @@ def test_batch_matches_single_grids(single_results, calls, shape):
     b.free()
     for g, _, _ in built:
         g.free()
+
+
+def test_batch_of_mixed_material_id_widths():
+    """A grid with more than 256 distinct materials (2-byte ids) batched with grids of 1-byte ids:
+    the batch widens the narrow grids' ids to one width, and every grid still equals its own
+    single-grid run."""
+    def make(q, K, graded):
+        g, mons, nf = _make(K, 1 + q)
+        if graded:  # 400 distinct permittivities in the core rows
+            I = J = 20
+            ep = np.full((K, J, I, 3), 2.0, np.float32)
+            ep[:, 8:12, :, :] = (2.0 + np.arange(K * 4 * I).reshape(K, 4, I, 1) % 400 / 100.0).astype(np.float32)
+            g.free()
+            g = fd.DevGrid(I, J, K, GC.DISC, ep.reshape(-1))
+            g.cpml((5, 5, 5, 5, 5, 5))
+            mode = fd.normalised_mode(GC.MI, GC.MJ, GC.DISC, GC.mode_array())
+            amp, phase = _schedule()
+            g.add_source(I // 2, J // 2, 7, mode, GC.MI, GC.MJ, GC.DISC, amp, phase)
+            oms = GC.OMEGA * np.array([0.9, 1.0, 1.1])
+            mons = [g.add_overlap(I // 2, J // 2, K - 8, mode, GC.MI, GC.MJ, GC.DISC, oms),
+                    g.add_overlap(I // 2, J // 2, 10, mode, GC.MI, GC.MJ, GC.DISC, oms, reverse=1)]
+        return g, mons, nf
+
+    spec = [(0, 30, False), (1, 47, True), (2, 64, False)]
+    ref = []
+    for q, K, graded in spec:
+        g, mons, nf = make(q, K, graded)
+        g.run(GC.DT, 0, NSTEPS)
+        ref.append(_results(g, mons, nf))
+        g.free()
+    built = [make(q, K, graded) for q, K, graded in spec]
+    b = fd.Batch([g for g, _, _ in built])
+    b.run(GC.DT, 0, NSTEPS)
+    for (g, mons, nf), (mon_s, f_s) in zip(built, ref):
+        mon_b, f_b = _results(g, mons, nf)
+        ef = float(np.linalg.norm(f_b - f_s) / np.linalg.norm(f_s))
+        em = float(np.linalg.norm(mon_b - mon_s) / np.linalg.norm(mon_s))
+        say("mixed ids: fields rel %.2e, monitors rel %.2e" % (ef, em))
+        assert ef <= 1e-6 and em <= 1e-6
+    b.free()
+    for g, _, _ in built:
+        g.free()
