@@ -295,7 +295,10 @@ def bench_design(fd, args):
     designs meet it 1 time in 20).  Strict acceptance adds the surrogate gate: max over the 16
     wavelengths of |T_surrogate - T_FDTD| <= 1e-3 (T = |t|^2) on the accepted design.  The plain loop
     has no surrogate, so its accepted designs count under both.  value = the surrogate loop's
-    accepted designs per hour after training (the training set's FDTD time is reported beside)."""
+    accepted designs per hour after training (the training set's FDTD time is reported beside).
+
+    The two loops can run in separate calls (--design-iters 0 or --plain-seconds 0): each saves its
+    part to DESIGN_DATA, and the line is printed once both parts exist."""
     design = importlib.import_module("3d_fdtd_amd.design")
     fam = design_family(design)
     files = sorted(f for f in os.listdir(DESIGN_DATA) if f.startswith("cfg5_train_")) if os.path.isdir(
@@ -310,49 +313,69 @@ def bench_design(fd, args):
     spec = float(np.quantile(T_train, 0.95))
     print(f"bench: {len(train_p)} training designs ({t_data:.0f} s of FDTD); spec band T >= {spec:.4f}",
           file=sys.stderr, flush=True)
-    rng = np.random.default_rng(WL.TAPER_SEED + 7)
+    # every port width the families can draw, solved once in parallel host processes
+    wc = np.arange(fam.port_width_cells(fam.wmin), fam.port_width_cells(fam.wmax) + 1) * fam.disc
     t0 = time.perf_counter()
-    net = design.fit(fam, train_p, train_t, steps=args.fit_steps)
-    t_fit = time.perf_counter() - t0
-    loop = design.SurrogateLoop(fam, net, train_p, train_t, spec=spec, seed=WL.TAPER_SEED + 1, min_hit=0.0)
-    print("bench: surrogate loop", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    it = 0
-    while it < args.design_iters:
-        cand = fam.random(loop.rng, loop.pool)
-        fam.precompute_ports(cand[design.screen(fam, net, cand, loop.verify)[0]])
-        loop.rng = np.random.default_rng(WL.TAPER_SEED + 100 + it)  # the same candidates again in step()
-        loop.step()
-        it += 1
-        print(f"bench: iteration {it}: {len(loop.accepted)} accepted of {loop.verified} verified, "
-              f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
-    t_sur = time.perf_counter() - t0
-    err = loop.surrogate_error()
-    acc = len(loop.accepted)
-    strict = int((err <= 1e-3).sum())
-    print("bench: plain loop", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    plain_ok, iters, starts = 0, 0, 0
-    plain_acc = []
-    while time.perf_counter() - t0 < args.plain_seconds:
-        p0 = fam.random(rng, 1)[0]
-        fam.precompute_ports(p0[None], workers=1)
-        p, hist = design.adjoint_optimize(fam, p0, 12, spec)
-        starts += 1
-        iters += len(hist)
-        if hist[-1] >= spec:
-            plain_ok += 1
-            plain_acc.append(p)
-        print(f"bench: plain start {starts}: T {hist[0]:.3f} -> {hist[-1]:.3f} in {len(hist)} iterations, "
-              f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
-    t_plain = time.perf_counter() - t0
-    net.free()
+    fam.precompute_ports(np.column_stack([wc] + [wc] * 8))
+    print(f"bench: {len(wc)} port modes in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     os.makedirs(DESIGN_DATA, exist_ok=True)
-    np.savez(os.path.join(DESIGN_DATA, "cfg5_accepted.npz"), p=np.array(loop.accepted), spec=spec,
-             fdtd=np.array(loop.spec_fdtd), sur=np.array(loop.spec_sur), plain=np.array(plain_acc))
-    rate = acc / t_sur * 3600
-    rate_strict = strict / t_sur * 3600
-    rate_plain = plain_ok / t_plain * 3600
+    if args.design_iters > 0:
+        t0 = time.perf_counter()
+        net = design.fit(fam, train_p, train_t, steps=args.fit_steps)
+        t_fit = time.perf_counter() - t0
+        loop = design.SurrogateLoop(fam, net, train_p, train_t, spec=spec, seed=WL.TAPER_SEED + 1, min_hit=0.0)
+        print("bench: surrogate loop", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for it in range(args.design_iters):
+            loop.step()
+            print(f"bench: iteration {it + 1}: {len(loop.accepted)} accepted of {loop.verified} verified, "
+                  f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+        t_sur = time.perf_counter() - t0
+        err = loop.surrogate_error()
+        net.free()
+        acc, strict = len(loop.accepted), int((err <= 1e-3).sum())
+        np.savez(os.path.join(DESIGN_DATA, "cfg5_accepted.npz"), p=np.array(loop.accepted), spec=spec,
+                 fdtd=np.array(loop.spec_fdtd), sur=np.array(loop.spec_sur))
+        part = {"accepted": acc, "accepted_strict": strict, "verified": loop.verified, "iterations": loop.iterations,
+                "seconds": round(t_sur, 1), "designs_per_hour": round(acc / t_sur * 3600, 1),
+                "designs_per_hour_strict": round(strict / t_sur * 3600, 1), "training_set_fdtd_s": round(t_data, 1),
+                "training_s": round(t_fit, 1),
+                "designs_per_hour_including_training_set": round(acc / (t_sur + t_data + t_fit) * 3600, 1),
+                "surrogate_max_dT_on_accepted": {"median": round(float(np.median(err)), 4) if acc else None,
+                                                 "max": round(float(err.max()), 4) if acc else None,
+                                                 "within_1e-3": strict},
+                "train_designs": len(train_p), "fit_steps": args.fit_steps, "spec": spec,
+                "screened_per_iteration": loop.pool, "verified_per_iteration": loop.verify}
+        with open(os.path.join(DESIGN_DATA, "cfg5_surrogate.json"), "w") as fh:
+            json.dump(part, fh)
+    if args.plain_seconds > 0:
+        rng = np.random.default_rng(WL.TAPER_SEED + 7)
+        print("bench: plain loop", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        plain_ok, iters, starts = 0, 0, 0
+        plain_acc = []
+        while time.perf_counter() - t0 < args.plain_seconds:
+            p0 = fam.random(rng, 1)[0]
+            p, hist = design.adjoint_optimize(fam, p0, 12, spec)
+            starts += 1
+            iters += len(hist)
+            if hist[-1] >= spec:
+                plain_ok += 1
+                plain_acc.append(p)
+            print(f"bench: plain start {starts}: T {hist[0]:.3f} -> {hist[-1]:.3f} in {len(hist)} iterations, "
+                  f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+        t_plain = time.perf_counter() - t0
+        part = {"accepted": plain_ok, "starts": starts, "iterations": iters, "fdtd_runs": 2 * iters,
+                "seconds": round(t_plain, 1), "designs_per_hour": round(plain_ok / t_plain * 3600, 1), "spec": spec,
+                "accepted_designs": [[round(float(v), 4) for v in q] for q in plain_acc]}
+        with open(os.path.join(DESIGN_DATA, "cfg5_plain.json"), "w") as fh:
+            json.dump(part, fh)
+    paths = [os.path.join(DESIGN_DATA, f"cfg5_{w}.json") for w in ("surrogate", "plain")]
+    if not all(os.path.exists(q) for q in paths):
+        print("bench: one loop's part saved; the line is printed once both parts exist", file=sys.stderr, flush=True)
+        return
+    sur, plain = (json.load(open(q)) for q in paths)
+    rate, rate_strict, rate_plain = sur["designs_per_hour"], sur["designs_per_hour_strict"], plain["designs_per_hour"]
     out = {"metric": METRIC, "value": rate, "unit": "verified taper designs/hour", "n_gpus": 1, "steps": fam.steps,
            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 fields, f64 monitors, bf16 surrogate", "data": "synthetic",
@@ -361,23 +384,13 @@ def bench_design(fd, args):
                                   "ports from the mode solver); surrogate MLP 9-256x4-32 GELU (BF16 MFMA, FP32 "
                                   "accumulate) trained on the FDTD training set; 10^5 screened and 64 verified by "
                                   "batched FDTD per iteration",
-                      "train_designs": len(train_p), "fit_steps": args.fit_steps, "spec": f"band |t|^2 >= {spec:.4f} "
-                      "(95th percentile of the training set)", "screened_per_iteration": loop.pool,
-                      "verified_per_iteration": loop.verify, "iterations": loop.iterations},
-           "design_loop": {
-               "surrogate": {"accepted": acc, "accepted_strict": strict, "verified": loop.verified,
-                             "seconds": round(t_sur, 1), "designs_per_hour": round(rate, 1),
-                             "designs_per_hour_strict": round(rate_strict, 1),
-                             "training_set_fdtd_s": round(t_data, 1), "training_s": round(t_fit, 1),
-                             "designs_per_hour_including_training_set": round(acc / (t_sur + t_data + t_fit) * 3600, 1),
-                             "surrogate_max_dT_on_accepted": {
-                                 "median": round(float(np.median(err)), 4) if acc else None,
-                                 "max": round(float(err.max()), 4) if acc else None,
-                                 "within_1e-3": strict}},
-               "plain": {"accepted": plain_ok, "starts": starts, "iterations": iters, "fdtd_runs": 2 * iters,
-                         "seconds": round(t_plain, 1), "designs_per_hour": round(rate_plain, 1)},
-               "ratio": round(rate / rate_plain, 2) if rate_plain else None,
-               "ratio_strict": round(rate_strict / rate_plain, 2) if rate_plain else None}}
+                      "train_designs": sur["train_designs"], "fit_steps": sur["fit_steps"],
+                      "spec": f"band |t|^2 >= {sur['spec']:.4f} (95th percentile of the training set)",
+                      "screened_per_iteration": sur["screened_per_iteration"],
+                      "verified_per_iteration": sur["verified_per_iteration"]},
+           "design_loop": {"surrogate": sur, "plain": {k: v for k, v in plain.items() if k != "accepted_designs"},
+                           "ratio": round(rate / rate_plain, 2) if rate_plain else None,
+                           "ratio_strict": round(rate_strict / rate_plain, 2) if rate_plain else None}}
     print(json.dumps(out), flush=True)
 
 
