@@ -111,7 +111,7 @@ int ensure_coefficients(fdtd_dev *d, double dt) {
         return -1;
     }
     if (d->coef_dt == dt) return 0;
-    std::vector<float> tab((size_t)2 * d->nmat * kCoefStride, 0.0f);
+    std::vector<float> tab(coef_table_floats(d->nmat), 0.0f);
     const double inv_dx = 1 / d->disc;
     for (int id = 0; id < d->nmat; id++) {
         const float *pr = &d->props[(size_t)id * kPropStride];
@@ -151,7 +151,26 @@ int ensure_coefficients(fdtd_dev *d, double dt) {
                 }
             }
         }
+        const float *s = &tab[(size_t)id * kCoefStride];  // semi-implicit record
+        float *pk = &tab[coef_packed_offset(d->nmat) + (size_t)id * kCoefPacked];
+        for (int m = 0; m < 3; m++) {
+            pk[m] = s[m];           // caE
+            pk[3 + m] = s[3 + m];   // cbE
+            pk[6 + m] = s[9 + m];   // caH
+            pk[9 + m] = s[12 + m];  // cbH
+        }
+        tab[coef_iso_offset(d->nmat) + id] = s[3];
     }
+    // isotropic lossless table: caE = caH = 1, cbE equal over the components, cbH one value
+    bool iso = d->nmat > 0;
+    const float cbh = d->nmat > 0 ? tab[12] : 0.0f;
+    for (int id = 0; id < d->nmat && iso; id++) {
+        const float *s = &tab[(size_t)id * kCoefStride];
+        for (int m = 0; m < 3; m++)
+            iso = iso && s[m] == 1.0f && s[9 + m] == 1.0f && s[3 + m] == s[3] && s[12 + m] == cbh;
+    }
+    d->coef_iso = iso;
+    d->coef_cbh = cbh;
     FDTD_TRY(hipMemcpyAsync(d->d_coef, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
     FDTD_TRY(hipStreamSynchronize(d->stream));
     d->coef_dt = dt;
@@ -264,6 +283,8 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->d_props = nullptr;
     d->d_coef = nullptr;
     d->coef_dt = NAN;
+    d->coef_iso = false;
+    d->coef_cbh = 0.0f;
     d->have_mats = false;
     d->fourier = nullptr;
     d->partials = nullptr;
@@ -289,6 +310,8 @@ fdtd_dev *fdtd_dev_create(int I, int J, int K, double disc) {
     d->fused_kchunk = 8;
     d->split_kchunk = 0;
     d->leap_kc = 0;
+    d->onepass_tw = 0;
+    d->onepass_kc = 0;
     d->finite_every = 0;
     d->finite_buf = nullptr;
     d->stream_mode = -1;
@@ -414,7 +437,7 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
         d->d_props = nullptr;
         d->d_coef = nullptr;
         FDTD_TRY(dev_malloc(&d->d_props, (size_t)nmat * kPropStride * sizeof(float)));
-        FDTD_TRY(dev_malloc(&d->d_coef, (size_t)2 * nmat * kCoefStride * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->d_coef, coef_table_floats(nmat) * sizeof(float)));
     }
     FDTD_TRY(hipMemcpy(d->d_props, props.data(), props.size() * sizeof(float), hipMemcpyHostToDevice));
     d->props = std::move(props);
@@ -518,7 +541,7 @@ int paint(fdtd_dev *d, PaintArgs p, float ep, float mu, bool set_mu) {
         d->d_props = nullptr;
         d->d_coef = nullptr;
         FDTD_TRY(dev_malloc(&d->d_props, (size_t)nmat * kPropStride * sizeof(float)));
-        FDTD_TRY(dev_malloc(&d->d_coef, (size_t)2 * nmat * kCoefStride * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->d_coef, coef_table_floats(nmat) * sizeof(float)));
         FDTD_TRY(hipMemcpy(d->d_props, props.data(), props.size() * sizeof(float), hipMemcpyHostToDevice));
         d->props = std::move(props);
         d->nmat = nmat;
@@ -627,7 +650,7 @@ int fdtd_dev_set_materials_extruded(fdtd_dev *d, int nmat, const float *ep, cons
         d->d_props = nullptr;
         d->d_coef = nullptr;
         FDTD_TRY(dev_malloc(&d->d_props, (size_t)nmat * kPropStride * sizeof(float)));
-        FDTD_TRY(dev_malloc(&d->d_coef, (size_t)2 * nmat * kCoefStride * sizeof(float)));
+        FDTD_TRY(dev_malloc(&d->d_coef, coef_table_floats(nmat) * sizeof(float)));
     }
     FDTD_TRY(hipMemcpy(d->d_props, props.data(), props.size() * sizeof(float), hipMemcpyHostToDevice));
     d->props = std::move(props);
@@ -780,9 +803,20 @@ int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk) {
         d->leap_kc = kchunk;
         return 0;
     }
+    if (rows == -3) {  // single-pass full steps
+        if (kchunk < 0) {
+            set_error("single-pass step: z planes per chunk must be >= 0 (0 = automatic)");
+            return -2;
+        }
+        d->fused_rows = -3;
+        d->onepass_tw = 0;
+        d->onepass_kc = kchunk;
+        return 0;
+    }
     if (rows != -1 && rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16 && rows != 32) {
-        set_error("step kernel rows must be -2 (single-pass leap launches), -1 (fused columns), 0 (split kernels), "
-                  "1, 2, 4, 8 (fused register strips) or 16, 32 (fused LDS tiles of 8 / 16 rows)");
+        set_error("step kernel rows must be -3 (single-pass full steps), -2 (single-pass leap launches), -1 (fused "
+                  "columns), 0 (split kernels), 1, 2, 4, 8 (fused register strips) or 16, 32 (fused LDS tiles of 8 / 16 "
+                  "rows)");
         return -2;
     }
     if (rows != 0 && kchunk < 1) {

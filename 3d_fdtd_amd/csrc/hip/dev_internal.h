@@ -54,6 +54,14 @@ inline bool streaming_loads(long long cells, int mode) {
 //   [18..23] Mur s for E on faces x,y,z (two tangential comps each, order b=(a+1)%3, c=(a+2)%3),
 //   [24..29] Mur s for H, [30..31] pad.
 constexpr int kCoefStride = 32;
+// Behind the two scheme tables the same allocation holds a packed copy of the semi-implicit
+// update coefficients, kCoefPacked floats per material: caEx caEy caEz cbEx | cbEy cbEz caHx caHy |
+// caHz cbHx cbHy cbHz (three 16-byte loads per cell for the single-pass step).
+constexpr int kCoefPacked = 12;
+// Then one float per material, its cbE (meaningful when the table is isotropic, see coef_iso).
+inline size_t coef_table_floats(int nmat) { return (size_t)nmat * (2 * kCoefStride + kCoefPacked + 1); }
+inline size_t coef_packed_offset(int nmat) { return (size_t)2 * nmat * kCoefStride; }
+inline size_t coef_iso_offset(int nmat) { return (size_t)nmat * (2 * kCoefStride + kCoefPacked); }
 // Raw per-material properties kept for sources and reports: ep[3], mu[3], sig[6] (+4 pad).
 constexpr int kPropStride = 16;
 
@@ -180,6 +188,9 @@ struct fdtd_dev {
     // single-pass leap launches of device-resident runs (fused_rows == -2): z planes per chunk
     // (0 = chosen per grid)
     int leap_kc;
+    // single-pass full steps (fused_rows == -3): x-tile width (0: the default, 64) and z planes per
+    // chunk (0 = chosen per grid)
+    int onepass_tw, onepass_kc;
 
     // materials
     int nmat;
@@ -188,8 +199,13 @@ struct fdtd_dev {
     void *mat;            // per-cell id
     std::vector<float> props;  // nmat * kPropStride
     float *d_props;
-    float *d_coef;        // [2 schemes][nmat][kCoefStride]
+    float *d_coef;        // [2 schemes][nmat][kCoefStride], then the packed and isotropic tables
     double coef_dt;       // dt the coefficient table was built for (NaN = invalid)
+    // every material of the semi-implicit table has caE = caH = 1 on all components, one cbE for
+    // its three components and the grid one cbH (isotropic, lossless, mu uniform): the update then
+    // needs only cbE per material (coef_iso_offset) and the scalar cbh
+    bool coef_iso;
+    float coef_cbh;
     bool have_mats;
 
     // Mur shells: 6 faces x 2 layers x 4 comps x face area
@@ -273,6 +289,10 @@ int finite_watch_verdict(fdtd_dev *d);
 bool col_supported(const fdtd_dev *d);
 // The step kernel evaluates run overlap monitors itself (split H update or column step).
 bool step_folds_monitors(const fdtd_dev *d);
+// Monitor partial slots per step that a folding step kernel writes (one per block of a plane).
+int step_monitor_slots(const fdtd_dev *d);
+// True when fdtd_dev_step runs the single-pass full step (dev_onepass.hip).
+bool step_uses_onepass(const fdtd_dev *d);
 // The device source plane for z = k (created zeroed on first use), or null with *err set.
 SrcPlane *plane_for(fdtd_dev *d, int k, int *err);
 void run_release(fdtd_dev *d);

@@ -690,8 +690,9 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     // overlap monitors accumulate inside the H update when every one of them can
     bool fold = nmon > 0 && nmon <= kMaxRunMon && step_folds_monitors(d);
     for (auto &o : r->ovl) fold = fold && o.ring_free;
+    const int mslots = step_monitor_slots(d);  // partials per step that the folding kernel writes
     if (fold) {
-        int rc = ensure_monitor_parts(r, n, (int)nbp, d->stream);
+        int rc = ensure_monitor_parts(r, n, mslots, d->stream);
         if (rc) return rc;
     }
     for (int s = 0; s < n; s++) {
@@ -712,7 +713,7 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
             d->n_run_mon = nmon;
             for (int m = 0; m < nmon; m++)
                 d->run_mon[m] = RunMonArg{r->ovl[m].k, r->ovl[m].W,
-                                          r->mpart + ((size_t)m * r->mstep_cap + s) * nbp, 0};
+                                          r->mpart + ((size_t)m * r->mstep_cap + s) * mslots, 0};
         }
         int rc = fdtd_dev_step(d, dt, 0);
         d->n_run_src = 0;
@@ -743,7 +744,7 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     }
     if (fold) {
         hipLaunchKernelGGL(k_run_mon_steps, dim3((n + kBlock - 1) / kBlock, nmon), dim3(kBlock), 0, d->stream,
-                           (const double2 *)r->mpart, r->minst, n, (int)nbp, r->mstep_cap, scale);
+                           (const double2 *)r->mpart, r->minst, n, mslots, r->mstep_cap, scale);
         hipLaunchKernelGGL(k_run_mon_dft, dim3(nmon), dim3(kBlock), 0, d->stream, (const MonDev *)r->mons,
                            (const double2 *)r->minst, n, r->mstep_cap, (const double2 *)r->tw, nf);
         FDTD_TRY(hipGetLastError());

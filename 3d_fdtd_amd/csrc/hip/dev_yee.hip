@@ -10,6 +10,7 @@
 // E pass streams E in/out, H in and a 1-byte material id: 25 bytes of HBM traffic.
 #include <cmath>
 
+#include "dev_onepass.h"
 #include "dev_stencil.h"
 
 namespace fdtd {
@@ -540,11 +541,22 @@ bool step_uses_split(const fdtd_dev *d) {
     if (!d->cpml.on || d->mat_bytes == 4) return true;
     if (d->fused_rows > 0) return !fused_supported(d);
     if (d->fused_rows == -1) return !col_supported(d);
+    if (d->fused_rows == -3) return !onepass_supported(d);
     return true;  // split pair, or leap launches (a lone step is a split pair; runs use leap_run)
 }
 
+bool step_uses_onepass(const fdtd_dev *d) {
+    return d->cpml.on && d->mat_bytes <= 2 && d->fused_rows == -3 && onepass_supported(d);
+}
+
 bool step_folds_monitors(const fdtd_dev *d) {
-    return step_uses_split(d) || (d->cpml.on && d->fused_rows == -1 && col_supported(d));
+    return step_uses_split(d) || step_uses_onepass(d) || (d->cpml.on && d->fused_rows == -1 && col_supported(d));
+}
+
+int step_monitor_slots(const fdtd_dev *d) {
+    if (step_uses_onepass(d))
+        return onepass_plane_slots(d, d->onepass_tw > 0 ? d->onepass_tw : onepass_default_width(d));
+    return (int)((d->P + kBlock - 1) / kBlock);
 }
 
 int fill_step_args(fdtd_dev *d, double dt, YeeArgs *a, PmlArgs *pm) {
@@ -634,6 +646,9 @@ int fdtd_dev_step(fdtd_dev *d, double dt, int scheme) {
     if (d->cpml.on && scheme == 0) {
         if (d->fused_rows > 0 && fused_supported(d)) return fused_step(d, dt, d->fused_rows, d->fused_kchunk);
         if (d->fused_rows == -1 && col_supported(d)) return col_step(d, dt, d->fused_kchunk);
+        if (step_uses_onepass(d))
+            return onepass_step(d, dt, d->onepass_tw > 0 ? d->onepass_tw : onepass_default_width(d),
+                                d->onepass_kc > 0 ? d->onepass_kc : onepass_default_chunk(d));
         if ((rc = fdtd_dev_update_E_cpml(d, dt))) return rc;
         return fdtd_dev_update_H_cpml(d, dt);
     }

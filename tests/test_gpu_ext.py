@@ -67,13 +67,13 @@ def test_cpml_step_vs_oracle(npml):
 
 
 @pytest.mark.parametrize("rows,kchunk", [(-1, 1), (-1, 2), (-1, 4), (2, 3), (4, 16), (8, 5), (16, 3), (16, 7),
-                                         (32, 4)])
+                                         (32, 4), (-3, 0), (-3, 1), (-3, 4), (-3, 7), (-3, 3)])
 def test_fused_step_vs_split_and_oracle(rows, kchunk):
-    """Fused one-kernel step (ping-pong buffers: columns, register strips or LDS tiles) vs the
-    split E/H kernels and the FP64 oracle, on an odd-sized grid (partial blocks, strips, tiles and
-    chunks) with conductivity, a plane source and PML on some faces; switches kernels mid-run to
-    exercise the buffer resynchronisation."""
-    rng = np.random.default_rng((rows + 2) * 100 + kchunk)
+    """Fused one-kernel step (ping-pong buffers: columns, register strips, LDS tiles or the
+    single-pass full step) vs the split E/H kernels and the FP64 oracle, on an odd-sized grid
+    (partial blocks, strips, tiles and chunks) with conductivity, a plane source and PML on some
+    faces; switches kernels mid-run to exercise the buffer resynchronisation."""
+    rng = np.random.default_rng(abs(rows + 2) * 100 + kchunk)
     I, J, K, disc = 70, 23, 29, 0.02
     dt = 0.99 * disc / np.sqrt(3)
     npml = (5, 4, 0, 3, 6, 0)
@@ -207,7 +207,7 @@ def test_run_monitor_fold_matches_separate_kernels(golden_dir):
     assert err <= 1e-10
 
 
-@pytest.mark.parametrize("rows", [-1, 8, 16])
+@pytest.mark.parametrize("rows", [-1, 8, 16, -3])
 def test_run_api_fused_matches_split(golden_dir, rows):
     """Device-resident run under the PML with a fused step kernel (run sources evaluated inside
     the fused kernel; monitors inside the column kernel, by the separate kernels for strips and
@@ -229,7 +229,7 @@ def test_run_api_fused_matches_split(golden_dir, rows):
         g.push()
         npml = (C.c_int * 6)(6, 6, 6, 6, 6, 6)
         fd.check(L().fdtd3d_ext_set_cpml(g.tf, npml, 3, 1e-6, 1.0, 0.05), "cpml")
-        fd.check(L().fdtd3d_ext_set_step_kernel(g.tf, kernel, {0: 0, -1: 2}.get(kernel, 5)), "kernel")
+        fd.check(L().fdtd3d_ext_set_step_kernel(g.tf, kernel, {0: 0, -1: 2, -3: 0}.get(kernel, 5)), "kernel")
         m1 = fd.mode_from_array(30, 30, float(z["disc"]), z["mode1"])
         assert L().fdtd3d_ext_run_add_mode_source(g.tf, I // 2, J // 2, 10, m1, 0, 0, n, amp.ctypes.data_as(DP),
                                                   phase.ctypes.data_as(DP)) >= 0
