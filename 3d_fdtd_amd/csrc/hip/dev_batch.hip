@@ -102,7 +102,8 @@ struct Shape {
     bool unroll, persistent;
 };
 const Shape kTune[] = {{2, true, true},  {4, true, true},  {2, false, true},  {4, false, true},  {8, false, true},
-                       {2, true, false}, {4, true, false}, {2, false, false}, {4, false, false}, {8, false, false}};
+                       {2, true, false}, {4, true, false}, {2, false, false}, {4, false, false}, {8, false, false},
+                       {3, true, false}};
 
 long long batch_layout(fdtd_dev_batch *b, int kc) {
     long long vb = 0;

@@ -270,13 +270,13 @@ class TaperFamily:
         return self.spline_basis(K) @ dT_dw
 
     def simulate(self, designs, batch=64):
-        """Complex transmission t [n][nfreq] of each design, by batched FDTD on the GPU."""
+        """Complex transmission t [n][nfreq] of each design, by batched FDTD on the GPU (the
+        batch's launch shape tuned on its first steps, which are real steps)."""
         out = []
         designs = np.asarray(designs)
         for s in range(0, len(designs), batch):
             grids = [self.grid(p) for p in designs[s:s + batch]]
             b = fd.Batch(grids)
-            b.set_shape(2, True, True)
             b.run(self.dt, 0, self.steps)
             for g in grids:
                 a_in, _ = g.overlap(g.mon_in, len(self.omegas))

@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 STEP_BYTES = 48  # algorithmic bytes per cell-update: 6 components x (4 B read + 4 B written)
 HALF_STEP_BYTES = STEP_BYTES // 2  # one split E or H launch: half a cell-update
 SPLIT_KERNEL_BYTES = 36  # what one split launch itself reads and writes (3 comps r+w, 3 read)
-TUNE_STEPS = 64  # the batch autotuner's window (10 candidates x 5 steps) on the first run call
+TUNE_STEPS = 64  # the batch autotuner's window (11 candidates x 5 steps) on the first run call
 
 WL = importlib.import_module("3d_fdtd_amd.workloads")
 DISC = WL.DISC
