@@ -298,7 +298,8 @@ def bench_design(fd, args):
     accepted designs per hour after training (the training set's FDTD time is reported beside).
 
     The two loops can run in separate calls (--design-iters 0 or --plain-seconds 0): each saves its
-    part to DESIGN_DATA, and the line is printed once both parts exist."""
+    part to DESIGN_DATA, and the line is printed once both parts exist.  --plain-append adds a
+    call's plain-loop starts (seeded after the saved ones) to the saved plain part."""
     design = importlib.import_module("3d_fdtd_amd.design")
     fam = design_family(design)
     files = sorted(f for f in os.listdir(DESIGN_DATA) if f.startswith("cfg5_train_")) if os.path.isdir(
@@ -349,7 +350,10 @@ def bench_design(fd, args):
         with open(os.path.join(DESIGN_DATA, "cfg5_surrogate.json"), "w") as fh:
             json.dump(part, fh)
     if args.plain_seconds > 0:
-        rng = np.random.default_rng(WL.TAPER_SEED + 7)
+        plain_path = os.path.join(DESIGN_DATA, "cfg5_plain.json")
+        prev = json.load(open(plain_path)) if args.plain_append and os.path.exists(plain_path) else None
+        # --plain-append continues a saved part with the starts after its own (the seed advances)
+        rng = np.random.default_rng(WL.TAPER_SEED + 7 + (prev["starts"] if prev else 0))
         print("bench: plain loop", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         plain_ok, iters, starts = 0, 0, 0
@@ -365,10 +369,15 @@ def bench_design(fd, args):
             print(f"bench: plain start {starts}: T {hist[0]:.3f} -> {hist[-1]:.3f} in {len(hist)} iterations, "
                   f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
         t_plain = time.perf_counter() - t0
+        acc_list = [[round(float(v), 4) for v in q] for q in plain_acc]
+        if prev:
+            plain_ok, starts, iters = plain_ok + prev["accepted"], starts + prev["starts"], iters + prev["iterations"]
+            t_plain += prev["seconds"]
+            acc_list = prev["accepted_designs"] + acc_list
         part = {"accepted": plain_ok, "starts": starts, "iterations": iters, "fdtd_runs": 2 * iters,
                 "seconds": round(t_plain, 1), "designs_per_hour": round(plain_ok / t_plain * 3600, 1), "spec": spec,
-                "accepted_designs": [[round(float(v), 4) for v in q] for q in plain_acc]}
-        with open(os.path.join(DESIGN_DATA, "cfg5_plain.json"), "w") as fh:
+                "accepted_designs": acc_list}
+        with open(plain_path, "w") as fh:
             json.dump(part, fh)
     paths = [os.path.join(DESIGN_DATA, f"cfg5_{w}.json") for w in ("surrogate", "plain")]
     if not all(os.path.exists(q) for q in paths):
@@ -963,6 +972,8 @@ def main():
     ap.add_argument("--chunks", type=int, default=5, help="design data: chunks of the training set")
     ap.add_argument("--design-iters", type=int, default=10, help="design workload: surrogate-loop iterations")
     ap.add_argument("--plain-seconds", type=float, default=360.0, help="design workload: plain-loop wall time")
+    ap.add_argument("--plain-append", action="store_true",
+                    help="design workload: add this call's plain-loop starts to the saved plain part")
     ap.add_argument("--fit-steps", type=int, default=1500, help="design workload: surrogate training steps")
     ap.add_argument("--tapers", type=int, default=64, help="number of taper grids of the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
