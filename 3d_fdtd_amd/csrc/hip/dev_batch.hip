@@ -34,23 +34,37 @@ __device__ __forceinline__ int batch_grid(const BatchGeom *__restrict__ geo, int
     return lo;
 }
 
-// NT: streaming own-cell loads (the mean grid's fields exceed the last-level cache).
-template <typename MatT, int KC, bool NT>
+// NT: streaming own-cell loads (the mean grid's fields exceed the last-level cache).  LOOP: a
+// persistent grid-stride launch; else one block per virtual block, with no state carried from
+// one virtual block to the next (fewer registers).
+template <typename MatT, int KC, bool NT, bool LOOP>
 __global__ __launch_bounds__(kBlock) void k_batch_E(const BatchGeom *__restrict__ geo, int G, long long total) {
-    for (long long vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    auto block = [&](long long vb) {
         const long long lb = xcd_logical_block_ll(vb, total);
         const BatchGeom &B = geo[batch_grid(geo, G, lb)];
         update_E_block<MatT, true, KC, uint32_t, NT>(B.a, B.pm, (int)(lb - B.vb0));
+    };
+    if constexpr (LOOP) {
+        for (long long vb = blockIdx.x; vb < total; vb += gridDim.x) block(vb);
+    } else if (blockIdx.x < total) {
+        block(blockIdx.x);
     }
 }
 
-template <typename MatT, bool MON, int KC, bool NT>
+template <typename MatT, bool MON, int KC, bool NT, bool LOOP>
 __global__ __launch_bounds__(kBlock) void k_batch_H(const BatchGeom *__restrict__ geo, int G, long long total) {
-    for (long long vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    auto block = [&](long long vb) {
         const long long lb = xcd_logical_block_ll(vb, total);
         const BatchGeom &B = geo[batch_grid(geo, G, lb)];
         update_H_block<MatT, true, MON, KC, uint32_t, NT>(B.a, B.pm, (int)(lb - B.vb0));
-        if (MON) __syncthreads();  // the next virtual block reuses the reduction slots
+    };
+    if constexpr (LOOP) {
+        for (long long vb = blockIdx.x; vb < total; vb += gridDim.x) {
+            block(vb);
+            if (MON) __syncthreads();  // the next virtual block reuses the reduction slots
+        }
+    } else if (blockIdx.x < total) {
+        block(blockIdx.x);
     }
 }
 
@@ -147,7 +161,7 @@ void batch_events(fdtd_dev_batch *b, int which, hipEvent_t *t0, hipEvent_t *t1) 
     *t1 = y;
 }
 
-template <typename MatT, int KC, bool NT>
+template <typename MatT, int KC, bool NT, bool LOOP>
 void launch_step_typed(fdtd_dev_batch *b) {
     const int G = (int)b->host.size();
     const dim3 grid(b->grid_blocks), block(kBlock);
@@ -156,26 +170,35 @@ void launch_step_typed(fdtd_dev_batch *b) {
         hipEvent_t e0, e1, h0, h1;
         batch_events(b, 0, &e0, &e1);
         batch_events(b, 1, &h0, &h1);
-        hipExtLaunchKernelGGL((k_batch_E<MatT, KC, NT>), grid, block, 0, b->stream, e0, e1, 0, geo, G, b->total);
+        hipExtLaunchKernelGGL((k_batch_E<MatT, KC, NT, LOOP>), grid, block, 0, b->stream, e0, e1, 0, geo, G, b->total);
         if (b->mon)
-            hipExtLaunchKernelGGL((k_batch_H<MatT, true, KC, NT>), grid, block, 0, b->stream, h0, h1, 0, geo, G, b->total);
+            hipExtLaunchKernelGGL((k_batch_H<MatT, true, KC, NT, LOOP>), grid, block, 0, b->stream, h0, h1, 0, geo, G,
+                                  b->total);
         else
-            hipExtLaunchKernelGGL((k_batch_H<MatT, false, KC, NT>), grid, block, 0, b->stream, h0, h1, 0, geo, G, b->total);
+            hipExtLaunchKernelGGL((k_batch_H<MatT, false, KC, NT, LOOP>), grid, block, 0, b->stream, h0, h1, 0, geo, G,
+                                  b->total);
     } else {
-        hipLaunchKernelGGL((k_batch_E<MatT, KC, NT>), grid, block, 0, b->stream, geo, G, b->total);
+        hipLaunchKernelGGL((k_batch_E<MatT, KC, NT, LOOP>), grid, block, 0, b->stream, geo, G, b->total);
         if (b->mon)
-            hipLaunchKernelGGL((k_batch_H<MatT, true, KC, NT>), grid, block, 0, b->stream, geo, G, b->total);
+            hipLaunchKernelGGL((k_batch_H<MatT, true, KC, NT, LOOP>), grid, block, 0, b->stream, geo, G, b->total);
         else
-            hipLaunchKernelGGL((k_batch_H<MatT, false, KC, NT>), grid, block, 0, b->stream, geo, G, b->total);
+            hipLaunchKernelGGL((k_batch_H<MatT, false, KC, NT, LOOP>), grid, block, 0, b->stream, geo, G, b->total);
     }
+}
+
+template <typename MatT, int KC, bool NT>
+void launch_step_loop(fdtd_dev_batch *b) {
+    // the grid covers every virtual block exactly when it is not capped at the persistent size
+    if (b->grid_blocks < b->total) launch_step_typed<MatT, KC, NT, true>(b);
+    else launch_step_typed<MatT, KC, NT, false>(b);
 }
 
 template <int KC>
 void launch_step_kc(fdtd_dev_batch *b) {
-    if (b->u8 && b->nt) launch_step_typed<uint8_t, KC, true>(b);
-    else if (b->u8) launch_step_typed<uint8_t, KC, false>(b);
-    else if (b->nt) launch_step_typed<uint16_t, KC, true>(b);
-    else launch_step_typed<uint16_t, KC, false>(b);
+    if (b->u8 && b->nt) launch_step_loop<uint8_t, KC, true>(b);
+    else if (b->u8) launch_step_loop<uint8_t, KC, false>(b);
+    else if (b->nt) launch_step_loop<uint16_t, KC, true>(b);
+    else launch_step_loop<uint16_t, KC, false>(b);
     hipLaunchKernelGGL(k_step_tick, dim3(1), dim3(64), 0, b->stream, b->step);
 }
 

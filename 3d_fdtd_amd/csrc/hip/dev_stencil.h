@@ -73,6 +73,18 @@ int run_batch_finish(fdtd_dev *d, int n0, int n, hipStream_t st);
 
 namespace {
 
+// Global-memory view of a device pointer.  A pointer that a kernel loads from memory rather than
+// receives as an argument (the batched kernels' per-grid descriptor table) is generic to the
+// compiler, which then emits flat memory instructions for it: they need a 64-bit vector address
+// per access (no scalar base), and they count against the scalar-load counter too, so each wait
+// for a descriptor field also drains the field loads in flight.  Every pointer the helpers below
+// take points into device memory, so they access it through this view.
+#define FDTD_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ FDTD_GLOBAL T *gview(T *p) {
+    return (FDTD_GLOBAL T *)p;
+}
+
 // Cell offsets inside one grid's arrays.  Off = uint32_t when every array of the grid is below
 // 4 GiB (fewer than 2^30 cells): a load is then a uniform 64-bit base in scalar registers plus a
 // 32-bit byte offset in one vector register, and fixed neighbour deltas (c - 1, c - I) become
@@ -81,18 +93,20 @@ namespace {
 template <typename Off, typename T>
 __device__ __forceinline__ T ld(const T *base, Off i) {
     if constexpr (sizeof(Off) == 4)
-        return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)i * (uint32_t)sizeof(T));
+        return *reinterpret_cast<const FDTD_GLOBAL T *>(reinterpret_cast<const FDTD_GLOBAL char *>(gview(base)) +
+                                                         (uint32_t)i * (uint32_t)sizeof(T));
     else
-        return base[i];
+        return gview(base)[i];
 }
 
 template <typename Off, typename T>
 __device__ __forceinline__ void st(T *base, Off i, T v) {
     // streaming stores: the updated fields are read again only by the next launch
     if constexpr (sizeof(Off) == 4)
-        __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)i * (uint32_t)sizeof(T)));
+        __builtin_nontemporal_store(v, reinterpret_cast<FDTD_GLOBAL T *>(reinterpret_cast<FDTD_GLOBAL char *>(gview(base)) +
+                                                                          (uint32_t)i * (uint32_t)sizeof(T)));
     else
-        __builtin_nontemporal_store(v, base + i);
+        __builtin_nontemporal_store(v, gview(base) + i);
 }
 
 // Load of a value no other thread of the launch reads (a cell's own old E in the E update, own
@@ -105,10 +119,10 @@ __device__ __forceinline__ T ld_own(const T *base, Off i) {
     if constexpr (!NT) {
         return ld(base, i);
     } else if constexpr (sizeof(Off) == 4) {
-        return __builtin_nontemporal_load(
-            reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)i * (uint32_t)sizeof(T)));
+        return __builtin_nontemporal_load(reinterpret_cast<const FDTD_GLOBAL T *>(
+            reinterpret_cast<const FDTD_GLOBAL char *>(gview(base)) + (uint32_t)i * (uint32_t)sizeof(T)));
     } else {
-        return __builtin_nontemporal_load(base + i);
+        return __builtin_nontemporal_load(gview(base) + i);
     }
 }
 
@@ -129,7 +143,7 @@ __device__ __forceinline__ bool source3(const YeeArgs &a, int k, Off p, int m0, 
         if (r.k == k && ld(r.win, p)) {
             float c1 = r.c1, c2 = r.c2;
             if (r.tab) {
-                const int s = *a.step;
+                const int s = *gview(a.step);
                 const float2 c = s >= 0 && s < r.ntab ? r.tab[s] : make_float2(0.0f, 0.0f);
                 c1 = c.x;
                 c2 = c.y;
@@ -236,20 +250,20 @@ __device__ __forceinline__ void update_E_block(const YeeArgs &a, const PmlArgs &
     if (PML) {
         sx = slab_index(i, a.I, pm.nlo[0], pm.nhi[0]);
         sy = slab_index(j, a.J, pm.nlo[1], pm.nhi[1]);
-        if (sx >= 0) bx = pm.tab[0][i], ax = pm.tab[0][a.I + i], kx = pm.tab[0][2 * a.I + i];
-        if (sy >= 0) by = pm.tab[1][j], ay = pm.tab[1][a.J + j], ky = pm.tab[1][2 * a.J + j];
+        if (sx >= 0) bx = gview(pm.tab[0])[i], ax = gview(pm.tab[0])[a.I + i], kx = gview(pm.tab[0])[2 * a.I + i];
+        if (sy >= 0) by = gview(pm.tab[1])[j], ay = gview(pm.tab[1])[a.J + j], ky = gview(pm.tab[1])[2 * a.J + j];
     }
     // one cell-update of plane k from its loaded inputs; stores only when `live`
     auto cell = [&](int k, bool live, Off c, float hx, float hy, float hz, float hy_im, float hz_im, float hx_jm,
                     float hz_jm, float hx_km, float hy_km, float eox, float eoy, float eoz, int m, const float *pv) {
-        const float *cf = a.coef + m * kCoefStride;
+        const FDTD_GLOBAL float *cf = gview(a.coef) + m * kCoefStride;
         // curl_x = dHz/dy - dHy/dz, curl_y = dHx/dz - dHz/dx, curl_z = dHy/dx - dHx/dy
         float x1 = hz - hz_jm, x2 = hy - hy_km;
         float y1 = hx - hx_km, y2 = hz - hz_im;
         float z1 = hy - hy_im, z2 = hx - hx_jm;
         const int sz = PML ? slab_index(k, a.K, pm.nlo[2], pm.nhi[2]) : -1;
         if (PML && (sx >= 0 || sy >= 0 || sz >= 0)) {
-            const float kz = sz >= 0 ? pm.tab[2][2 * a.K + k] : 1.0f;
+            const float kz = sz >= 0 ? gview(pm.tab[2])[2 * a.K + k] : 1.0f;
             float tx1 = x1 * ky, tx2 = x2 * kz, ty1 = y1 * kz, ty2 = y2 * kx, tz1 = z1 * kx, tz2 = z2 * ky;
             if (sx >= 0) {
                 const Off q = psi_x<Off>(pm, a, j, k, sx);
@@ -262,7 +276,7 @@ __device__ __forceinline__ void update_E_block(const YeeArgs &a, const PmlArgs &
                 tx1 = pml_term(pm.psiE[1][1], q, pv[3], by, ay, ky, x1, live);
             }
             if (sz >= 0) {
-                const float bz = pm.tab[2][k], az = pm.tab[2][a.K + k];
+                const float bz = gview(pm.tab[2])[k], az = gview(pm.tab[2])[a.K + k];
                 const Off q = psi_z<Off>(a, p, sz);
                 tx2 = pml_term(pm.psiE[2][0], q, pv[4], bz, az, kz, x2, live);
                 ty1 = pml_term(pm.psiE[2][1], q, pv[5], bz, az, kz, y1, live);
@@ -367,7 +381,7 @@ __device__ __forceinline__ void monitor_epilogue_on(const YeeArgs &a, const floa
             im = (hx * w[3] - hy * w[1]) + (ey * w[5] - ex * w[7]);
         }
         const double2 v = block_sum2(re, im);
-        double2 *part = a.step ? M.part + (long long)(*a.step - a.step0) * M.stride : M.part;
+        double2 *part = a.step ? M.part + (long long)(*gview(a.step) - a.step0) * M.stride : M.part;
         if (threadIdx.x == 0) part[pb] = v;
     }
 }
@@ -404,19 +418,19 @@ __device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &
         if (PML) {
             sx = slab_index(i, a.I, pm.nlo[0], pm.nhi[0]);
             sy = slab_index(j, a.J, pm.nlo[1], pm.nhi[1]);
-            if (sx >= 0) bx = pm.tab[0][3 * a.I + i], ax = pm.tab[0][4 * a.I + i], kx = pm.tab[0][5 * a.I + i];
-            if (sy >= 0) by = pm.tab[1][3 * a.J + j], ay = pm.tab[1][4 * a.J + j], ky = pm.tab[1][5 * a.J + j];
+            if (sx >= 0) bx = gview(pm.tab[0])[3 * a.I + i], ax = gview(pm.tab[0])[4 * a.I + i], kx = gview(pm.tab[0])[5 * a.I + i];
+            if (sy >= 0) by = gview(pm.tab[1])[3 * a.J + j], ay = gview(pm.tab[1])[4 * a.J + j], ky = gview(pm.tab[1])[5 * a.J + j];
         }
         auto cell = [&](int k, bool live, Off c, float ex, float ey, float ez, float ex_kp, float ey_kp, float ey_ip,
                         float ez_ip, float ex_jp, float ez_jp, float hox, float hoy, float hoz, int m, const float *pv) {
-            const float *cf = a.coef + m * kCoefStride;
+            const FDTD_GLOBAL float *cf = gview(a.coef) + m * kCoefStride;
             // curl_x = dEz/dy - dEy/dz, curl_y = dEx/dz - dEz/dx, curl_z = dEy/dx - dEx/dy
             float x1 = ez_jp - ez, x2 = ey_kp - ey;
             float y1 = ex_kp - ex, y2 = ez_ip - ez;
             float z1 = ey_ip - ey, z2 = ex_jp - ex;
             const int sz = PML ? slab_index(k, a.K, pm.nlo[2], pm.nhi[2]) : -1;
             if (PML && (sx >= 0 || sy >= 0 || sz >= 0)) {
-                const float kz = sz >= 0 ? pm.tab[2][5 * a.K + k] : 1.0f;
+                const float kz = sz >= 0 ? gview(pm.tab[2])[5 * a.K + k] : 1.0f;
                 float tx1 = x1 * ky, tx2 = x2 * kz, ty1 = y1 * kz, ty2 = y2 * kx, tz1 = z1 * kx, tz2 = z2 * ky;
                 if (sx >= 0) {
                     const Off q = psi_x<Off>(pm, a, j, k, sx);
@@ -429,7 +443,7 @@ __device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &
                     tx1 = pml_term(pm.psiH[1][1], q, pv[3], by, ay, ky, x1, live);
                 }
                 if (sz >= 0) {
-                    const float bz = pm.tab[2][3 * a.K + k], az = pm.tab[2][4 * a.K + k];
+                    const float bz = gview(pm.tab[2])[3 * a.K + k], az = gview(pm.tab[2])[4 * a.K + k];
                     const Off q = psi_z<Off>(a, p, sz);
                     tx2 = pml_term(pm.psiH[2][0], q, pv[4], bz, az, kz, x2, live);
                     ty1 = pml_term(pm.psiH[2][1], q, pv[5], bz, az, kz, y1, live);
