@@ -144,9 +144,10 @@ __device__ __forceinline__ bool source3(const YeeArgs &a, int k, Off p, int m0, 
             float c1 = r.c1, c2 = r.c2;
             if (r.tab) {
                 const int s = *gview(a.step);
-                const float2 c = s >= 0 && s < r.ntab ? r.tab[s] : make_float2(0.0f, 0.0f);
-                c1 = c.x;
-                c2 = c.y;
+                const FDTD_GLOBAL float *t = gview(reinterpret_cast<const float *>(r.tab));  // (c1, c2) pairs
+                const bool in = s >= 0 && s < r.ntab;
+                c1 = in ? t[2 * s] : 0.0f;
+                c2 = in ? t[2 * s + 1] : 0.0f;
             }
             for (int m = 0; m < 3; m++)
                 v[m] = c1 * ld(r.PR + (long long)(m0 + m) * a.P, p) + c2 * ld(r.PI + (long long)(m0 + m) * a.P, p);
@@ -382,7 +383,11 @@ __device__ __forceinline__ void monitor_epilogue_on(const YeeArgs &a, const floa
         }
         const double2 v = block_sum2(re, im);
         double2 *part = a.step ? M.part + (long long)(*gview(a.step) - a.step0) * M.stride : M.part;
-        if (threadIdx.x == 0) part[pb] = v;
+        if (threadIdx.x == 0) {
+            FDTD_GLOBAL double *q = gview(reinterpret_cast<double *>(part + pb));
+            q[0] = v.x;
+            q[1] = v.y;
+        }
     }
 }
 
