@@ -72,6 +72,10 @@ __global__ void k_step_tick(int *step) {
     if (threadIdx.x == 0) *step += 1;
 }
 
+__global__ void k_step_set(int *step, int v) {
+    if (threadIdx.x == 0) *step = v;
+}
+
 }  // namespace
 }  // namespace fdtd
 
@@ -81,6 +85,7 @@ struct fdtd_dev_batch {
     std::vector<fdtd_dev *> grids;
     hipStream_t stream = nullptr;
     int *step = nullptr;        // device step counter
+    BatchStage *stage = nullptr;  // pinned staging and device tables of a call's finish
     BatchGeom *geo = nullptr;   // device descriptor table
     std::vector<BatchGeom> host;
     long long total = 0;        // virtual blocks of the current chunk depth
@@ -244,8 +249,8 @@ int segment_graph(fdtd_dev_batch *b) {
 }
 
 int set_step(fdtd_dev_batch *b, int n0) {
-    FDTD_TRY(hipMemcpyAsync(b->step, &n0, sizeof(int), hipMemcpyHostToDevice, b->stream));
-    FDTD_TRY(hipStreamSynchronize(b->stream));  // n0 lives on the host stack
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, b->stream, b->step, n0);
+    FDTD_TRY(hipGetLastError());
     return 0;
 }
 
@@ -254,6 +259,7 @@ int set_step(fdtd_dev_batch *b, int n0) {
 int batch_chunk(fdtd_dev_batch *b, double dt, int n0, int n) {
     for (auto *d : b->grids) fields_changed(d);
     for (auto *d : b->grids) FDTD_TRY(hipStreamSynchronize(d->stream));
+    FDTD_TRY(hipStreamSynchronize(b->stream));  // nothing of an earlier call (failed or not) in flight
     b->mon = false;
     for (size_t g = 0; g < b->grids.size(); g++) {
         int rc = run_batch_prepare(b->grids[g], dt, n0, n, b->step, &b->host[g].a, &b->host[g].pm);
@@ -300,8 +306,7 @@ int batch_chunk(fdtd_dev_batch *b, double dt, int n0, int n) {
     }
     for (int s = segs * kSeg; s < left; s++)
         if ((rc = launch_step(b, b->kc, b->unroll))) return rc;
-    for (auto *d : b->grids)
-        if ((rc = run_batch_finish(d, n0, n, b->stream))) return rc;
+    if ((rc = run_batch_finish(b->grids.data(), (int)b->grids.size(), dt, n0, n, b->stream, b->stage))) return rc;
     FDTD_TRY(hipStreamSynchronize(b->stream));
     return 0;
 }
@@ -343,6 +348,7 @@ fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n) {
     long long cells = 0;
     for (int g = 0; g < n; g++) cells += grids[g]->N;
     b->nt = streaming_loads(cells / n, grids[0]->stream_mode);
+    b->stage = batch_stage_new();
     hipError_t e = dev_stream_acquire(&b->stream);
     if (e == hipSuccess) e = dev_malloc(&b->step, sizeof(int));
     if (e == hipSuccess) e = dev_malloc(&b->geo, (size_t)n * sizeof(BatchGeom));
@@ -364,6 +370,7 @@ void fdtd_dev_batch_free(fdtd_dev_batch *b) {
         for (hipEvent_t e : v) (void)hipEventDestroy(e);
     (void)dev_free(b->step);
     (void)dev_free(b->geo);
+    batch_stage_free(b->stage);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)dev_stream_release(b->stream);

@@ -60,6 +60,7 @@ struct Run {
     int partial_cap = 0;
     MonDev *mons = nullptr;       // device copy of the monitor descriptors
     int mon_cap = 0;
+    std::vector<MonDev> mons_host;  // what `mons` holds (batched calls upload only a change)
     double2 *tw = nullptr;        // [step][total freq] twiddles for the current run() call
     long long tw_cap = 0;
     // monitors evaluated inside the H update: per-step block partials [monitor][step][block]
@@ -184,29 +185,31 @@ __global__ __launch_bounds__(kBlock) void k_run_overlap_final_multi(const double
     }
 }
 
-// Overlap per step from the block partials the H update stored: inst[m][s] = scale * sum_b
-// part[m][s][b], blocks summed in index order.
-__global__ void k_run_mon_steps(const double2 *part, double2 *inst, int n, int nbp, long long cap, double scale) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
-    if (s >= n) return;
-    const double2 *pp = part + ((long long)m * cap + s) * nbp;
+// Overlap of one monitor at step s from the block partials the H update stored: inst[s] = scale *
+// sum_b part[s][b], blocks summed in index order.
+__device__ __forceinline__ void mon_step_sum(const double2 *part, double2 *inst, int s, int nbp, double scale) {
+    const double2 *pp = part + (long long)s * nbp;
     double r = 0, q = 0;
     for (int b = 0; b < nbp; b++) {
         r += pp[b].x;
         q += pp[b].y;
     }
-    inst[(long long)m * cap + s] = make_double2(scale * r, scale * q);
+    inst[s] = make_double2(scale * r, scale * q);
 }
 
-// Running DFT of a run call's per-step overlaps, one block per monitor: acc[f] += sum_s tw[s][f]
-// inst[s].  Thread t sums a fixed contiguous range of steps, the block combines the 256 partial
-// sums in a fixed tree, so the result does not depend on scheduling.  Also leaves the last
-// step's overlap in *inst_last.
-__global__ __launch_bounds__(kBlock) void k_run_mon_dft(const MonDev *mons, const double2 *inst, int n, long long cap,
-                                                        const double2 *tw, int nf) {
+// Overlap per step of every monitor m of a grid: part [m][s][b] and inst [m][s], step capacity cap.
+__global__ void k_run_mon_steps(const double2 *part, double2 *inst, int n, int nbp, long long cap, double scale) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
+    if (s >= n) return;
+    mon_step_sum(part + (long long)m * cap * nbp, inst + (long long)m * cap, s, nbp, scale);
+}
+
+// Running DFT of a run call's per-step overlaps x[0..n) of monitor M, by one block: acc[f] +=
+// sum_s tw[s][f] x[s].  Thread t sums a fixed contiguous range of steps, the block combines the
+// partial sums in a fixed tree, so the result does not depend on scheduling.  Also leaves the
+// last step's overlap in *M.inst.
+__device__ __forceinline__ void mon_dft_block(const MonDev &M, const double2 *x, int n, const double2 *tw, int nf) {
     __shared__ double sre[kBlock], sim[kBlock];
-    const MonDev M = mons[blockIdx.x];
-    const double2 *x = inst + (long long)blockIdx.x * cap;
     const int per = (n + kBlock - 1) / kBlock;
     const int s0 = threadIdx.x * per, s1 = min(n, s0 + per);
     for (int f = 0; f < M.nfreq; f++) {
@@ -233,6 +236,40 @@ __global__ __launch_bounds__(kBlock) void k_run_mon_dft(const MonDev *mons, cons
         __syncthreads();
     }
     if (threadIdx.x == 0) *M.inst = x[n - 1];
+}
+
+// The running DFT of every monitor of a grid, one block per monitor (inst [m][s], capacity cap).
+__global__ __launch_bounds__(kBlock) void k_run_mon_dft(const MonDev *mons, const double2 *inst, int n, long long cap,
+                                                        const double2 *tw, int nf) {
+    const MonDev M = mons[blockIdx.x];
+    mon_dft_block(M, inst + (long long)blockIdx.x * cap, n, tw, nf);
+}
+
+// One overlap monitor of a batched run call: its block partials [s][b] and overlaps [s], its
+// device descriptor, and its grid's twiddle rows [s][nf].
+struct BatchMon {
+    const double2 *part;
+    double2 *inst;
+    const MonDev *mon;
+    const double2 *tw;
+    int nbp, nf;
+    double scale;
+};
+
+// The finish of a batched run call for every monitor of every grid in two launches: per-step
+// overlaps (blockIdx.y = monitor), then the running DFTs (one block per monitor).  The same
+// per-monitor arithmetic as k_run_mon_steps and k_run_mon_dft.
+__global__ void k_batch_mon_steps(const BatchMon *bm, int n) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const BatchMon &e = bm[blockIdx.y];
+    mon_step_sum(e.part, e.inst, s, e.nbp, e.scale);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_mon_dft(const BatchMon *bm, int n) {
+    const BatchMon &e = bm[blockIdx.x];
+    const MonDev M = *e.mon;
+    mon_dft_block(M, e.inst, n, e.tw, e.nf);
 }
 
 // acc[slot][f][q] += field_m(cell q of the box) * tw[f] for each selected component m (FP64).
@@ -665,6 +702,7 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
         }
         FDTD_TRY(hipMemcpyAsync(r->mons, md.data(), md.size() * sizeof(MonDev), hipMemcpyHostToDevice, d->stream));
         FDTD_TRY(hipStreamSynchronize(d->stream));
+        r->mons_host = md;
     }
     // source planes resolved once per call; the first kMaxRunSrc sources are evaluated inside
     // the Yee kernels, any further ones are written into their plane by a small kernel
@@ -760,9 +798,9 @@ int run_chunk(fdtd_dev *d, double dt, int n0, int n, double t_offset) {
     return 0;
 }
 
-// Twiddles e^{i omega (step + t_offset) dt} of steps n0..n0+n-1 for every monitor frequency of the
-// run (overlaps first, then DFT boxes), uploaded to r->tw; returns the row length.
-int run_twiddles(fdtd_dev *d, Run *r, double dt, int n0, int n, double t_offset, int *nf_out) {
+// Offsets of every monitor's frequencies in a twiddle row (overlaps first, then DFT boxes);
+// returns the row length.
+int run_freq_offsets(Run *r) {
     int nf = 0;
     for (auto &o : r->ovl) {
         o.freq_off = nf;
@@ -772,9 +810,11 @@ int run_twiddles(fdtd_dev *d, Run *r, double dt, int n0, int n, double t_offset,
         q.freq_off = nf;
         nf += q.nfreq;
     }
-    *nf_out = nf;
-    if (nf == 0) return 0;
-    std::vector<double2> tw((size_t)n * nf);
+    return nf;
+}
+
+// Twiddle rows e^{i omega (step + t_offset) dt} of steps n0..n0+n-1, nf per row, into tw.
+void twiddle_rows(const Run *r, double dt, int n0, int n, double t_offset, int nf, double2 *tw) {
     for (int s = 0; s < n; s++) {
         double t = ((double)(n0 + s) + t_offset) * dt;
         int f = 0;
@@ -783,16 +823,6 @@ int run_twiddles(fdtd_dev *d, Run *r, double dt, int n0, int n, double t_offset,
         for (auto &q : r->dft)
             for (double w : q.omega) tw[(size_t)s * nf + f++] = make_double2(std::cos(w * t), std::sin(w * t));
     }
-    if ((long long)tw.size() > r->tw_cap) {
-        FDTD_TRY(hipStreamSynchronize(d->stream));
-        (void)dev_free(r->tw);
-        r->tw = nullptr;
-        FDTD_TRY(dev_malloc(&r->tw, tw.size() * sizeof(double2)));
-        r->tw_cap = (long long)tw.size();
-    }
-    FDTD_TRY(hipMemcpyAsync(r->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, d->stream));
-    FDTD_TRY(hipStreamSynchronize(d->stream));
-    return 0;
 }
 
 }  // namespace
@@ -812,8 +842,9 @@ int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, Ye
             set_error("batched runs need overlap monitors clear of the boundary ring");
             return -2;
         }
-    int nf = 0, rc = run_twiddles(d, r, dt, n0, n, 0.0, &nf);
-    if (rc) return rc;
+    // the twiddle rows themselves are formed by run_batch_finish, while the steps run
+    (void)run_freq_offsets(r);
+    int rc = 0;
     const unsigned nbp = (unsigned)((d->P + kBlock - 1) / kBlock);
     const int nmon = (int)r->ovl.size();
     if (nmon > 0) {
@@ -822,13 +853,18 @@ int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, Ye
             const RunOverlap &o = r->ovl[m];
             md[m] = MonDev{(long long)o.k * d->P, o.W, o.inst, o.acc, o.nfreq, o.freq_off};
         }
-        if (nmon > r->mon_cap) {
-            (void)dev_free(r->mons);
-            r->mons = nullptr;
-            FDTD_TRY(dev_malloc(&r->mons, (size_t)nmon * sizeof(MonDev)));
-            r->mon_cap = nmon;
+        // MonDev has no padding, so equal bytes mean equal descriptors
+        if (md.size() != r->mons_host.size() || std::memcmp(md.data(), r->mons_host.data(), md.size() * sizeof(MonDev))) {
+            if (nmon > r->mon_cap) {
+                (void)dev_free(r->mons);
+                r->mons = nullptr;
+                r->mons_host.clear();
+                FDTD_TRY(dev_malloc(&r->mons, (size_t)nmon * sizeof(MonDev)));
+                r->mon_cap = nmon;
+            }
+            FDTD_TRY(hipMemcpy(r->mons, md.data(), md.size() * sizeof(MonDev), hipMemcpyHostToDevice));
+            r->mons_host = md;
         }
-        FDTD_TRY(hipMemcpy(r->mons, md.data(), md.size() * sizeof(MonDev), hipMemcpyHostToDevice));
         if ((rc = ensure_monitor_parts(r, n, (int)nbp, d->stream))) return rc;
     }
     for (auto &s : r->src) {
@@ -854,27 +890,86 @@ int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, Ye
     return 0;
 }
 
-int run_batch_finish(fdtd_dev *d, int n0, int n, hipStream_t st) {
-    Run *r = run_of(d);
-    const unsigned nbp = (unsigned)((d->P + kBlock - 1) / kBlock);
-    const int nmon = (int)r->ovl.size();
-    int nf = 0;
-    for (auto &o : r->ovl) nf += o.nfreq;
-    if (nmon > 0) {
-        hipLaunchKernelGGL(k_run_mon_steps, dim3((n + kBlock - 1) / kBlock, nmon), dim3(kBlock), 0, st,
-                           (const double2 *)r->mpart, r->minst, n, (int)nbp, r->mstep_cap, -0.5 * d->disc * d->disc);
-        hipLaunchKernelGGL(k_run_mon_dft, dim3(nmon), dim3(kBlock), 0, st, (const MonDev *)r->mons,
-                           (const double2 *)r->minst, n, r->mstep_cap, (const double2 *)r->tw, nf);
+static_assert(sizeof(MonDev) == 40, "MonDev is compared bytewise: no padding");
+
+struct BatchStage {
+    char *host = nullptr;  // pinned: one call's twiddle rows of every grid, then its monitor table
+    size_t host_cap = 0;
+    char *dev = nullptr;   // device copy of the same bytes
+    size_t dev_cap = 0;
+};
+
+BatchStage *batch_stage_new() { return new BatchStage(); }
+
+void batch_stage_free(BatchStage *S) {
+    if (!S) return;
+    if (S->host) (void)hipHostFree(S->host);
+    (void)dev_free(S->dev);
+    delete S;
+}
+
+int run_batch_finish(fdtd_dev *const *grids, int ng, double dt, int n0, int n, hipStream_t st, BatchStage *S) {
+    // staging layout: [twiddle rows of grid 0][grid 1]...[BatchMon table]
+    std::vector<size_t> off((size_t)ng + 1, 0);
+    std::vector<int> nfs((size_t)ng, 0);
+    int nmon_all = 0;
+    for (int g = 0; g < ng; g++) {
+        Run *r = run_of(grids[g]);
+        for (auto &o : r->ovl) nfs[g] += o.nfreq;
+        off[g + 1] = off[g] + (size_t)n * nfs[g] * sizeof(double2);
+        nmon_all += (int)r->ovl.size();
+    }
+    const size_t tab_off = off[ng], need = tab_off + (size_t)nmon_all * sizeof(BatchMon);
+    if (nmon_all > 0) {
+        // the previous call's copies from the staging ended with that call's synchronisation
+        if (need > S->host_cap) {
+            if (S->host) (void)hipHostFree(S->host);
+            S->host = nullptr;
+            S->host_cap = 0;
+            FDTD_TRY(hipHostMalloc((void **)&S->host, need));
+            S->host_cap = need;
+        }
+        if (need > S->dev_cap) {
+            FDTD_TRY(hipStreamSynchronize(st));  // this call's steps do not read it; earlier finishes did
+            (void)dev_free(S->dev);
+            S->dev = nullptr;
+            S->dev_cap = 0;
+            FDTD_TRY(dev_malloc(&S->dev, need));
+            S->dev_cap = need;
+        }
+        BatchMon *tab = reinterpret_cast<BatchMon *>(S->host + tab_off);
+        int q = 0;
+        for (int g = 0; g < ng; g++) {
+            fdtd_dev *d = grids[g];
+            Run *r = run_of(d);
+            if (r->ovl.empty()) continue;
+            twiddle_rows(r, dt, n0, n, 0.0, nfs[g], reinterpret_cast<double2 *>(S->host + off[g]));
+            const long long nbp = (d->P + kBlock - 1) / kBlock;
+            for (int m = 0; m < (int)r->ovl.size(); m++)
+                tab[q++] = BatchMon{r->mpart + (size_t)m * r->mstep_cap * nbp, r->minst + (size_t)m * r->mstep_cap,
+                                    r->mons + m, reinterpret_cast<const double2 *>(S->dev + off[g]), (int)nbp, nfs[g],
+                                    -0.5 * d->disc * d->disc};
+        }
+        FDTD_TRY(hipMemcpyAsync(S->dev, S->host, need, hipMemcpyHostToDevice, st));
+        const BatchMon *dtab = reinterpret_cast<const BatchMon *>(S->dev + tab_off);
+        hipLaunchKernelGGL(k_batch_mon_steps, dim3((n + kBlock - 1) / kBlock, q), dim3(kBlock), 0, st, dtab, n);
+        hipLaunchKernelGGL(k_batch_mon_dft, dim3(q), dim3(kBlock), 0, st, dtab, n);
         FDTD_TRY(hipGetLastError());
     }
-    for (auto &s : r->src) {
-        int err;
-        SrcPlane *sp = plane_for(d, s.k, &err);
-        if (!sp) return err;
-        const int last = n0 + n - 1;
-        const float2 c = last >= 0 && last < s.nsteps ? s.coef[last] : make_float2(0.0f, 0.0f);
-        hipLaunchKernelGGL(k_run_source, dim3(nbp), dim3(kBlock), 0, st, sp->J, sp->stride, s.PR, s.PI, s.win, d->P, c);
-        FDTD_TRY(hipGetLastError());
+    for (int g = 0; g < ng; g++) {
+        fdtd_dev *d = grids[g];
+        Run *r = run_of(d);
+        const unsigned nbp = (unsigned)((d->P + kBlock - 1) / kBlock);
+        for (auto &s : r->src) {
+            int err;
+            SrcPlane *sp = plane_for(d, s.k, &err);
+            if (!sp) return err;
+            const int last = n0 + n - 1;
+            const float2 c = last >= 0 && last < s.nsteps ? s.coef[last] : make_float2(0.0f, 0.0f);
+            hipLaunchKernelGGL(k_run_source, dim3(nbp), dim3(kBlock), 0, st, sp->J, sp->stride, s.PR, s.PI, s.win,
+                               d->P, c);
+            FDTD_TRY(hipGetLastError());
+        }
     }
     return 0;
 }

@@ -64,12 +64,17 @@ inline void set_source_range(YeeArgs &a) {
 int fill_step_args(fdtd_dev *d, double dt, YeeArgs *a, PmlArgs *pm);
 
 // Batched runs (dev_run.hip).  run_batch_prepare readies steps n0..n0+n-1 of grid d's run for
-// launches on stream st that read the step index from the device counter `step` (twiddles,
-// device source-coefficient tables, monitor partial buffers) and fills d's launch arguments with
-// them; run_batch_finish forms the monitors' overlaps and running DFTs of those steps on st and
-// leaves each source plane as the last step wrote it.
+// launches that read the step index from the device counter `step` (device source-coefficient
+// tables, monitor descriptors and partial buffers) and fills d's launch arguments with them.
+// run_batch_finish, called once the steps of all ng grids are queued on st, forms every
+// monitor's per-step overlaps and running DFTs of those steps on st (the twiddles are computed on
+// the host meanwhile and go up with one copy from the stage's pinned buffer, which must not be
+// reused before st has finished) and leaves each source plane as the last step wrote it.
+struct BatchStage;
+BatchStage *batch_stage_new();
+void batch_stage_free(BatchStage *s);
 int run_batch_prepare(fdtd_dev *d, double dt, int n0, int n, const int *step, YeeArgs *a, PmlArgs *pm);
-int run_batch_finish(fdtd_dev *d, int n0, int n, hipStream_t st);
+int run_batch_finish(fdtd_dev *const *grids, int ng, double dt, int n0, int n, hipStream_t st, BatchStage *s);
 
 namespace {
 
