@@ -137,3 +137,54 @@ def test_batch_of_mixed_material_id_widths():
     b.free()
     for g, _, _ in built:
         g.free()
+
+
+def test_batch_finish_with_uneven_monitor_sets():
+    """Grids of one batch with two monitors of 3 wavelengths, none, and one of 5 wavelengths: the
+    batched finish (one twiddle block per grid of its own row length, one table of every grid's
+    monitors) gives each grid the monitors and fields of its own single-grid run, over two calls
+    of which the second is the longer (the staging grows between them)."""
+    def make(q, K):
+        I = J = 20
+        ep = np.full((K, J, I, 3), 2.0, np.float32)
+        ep[:, 8:12, 9 - q:11 + q, :] = 6.0
+        g = fd.DevGrid(I, J, K, GC.DISC, ep.reshape(-1))
+        g.cpml((5, 5, 5, 5, 5, 5))
+        mode = fd.normalised_mode(GC.MI, GC.MJ, GC.DISC, GC.mode_array())
+        amp, phase = _schedule()
+        g.add_source(I // 2, J // 2, 7, mode, GC.MI, GC.MJ, GC.DISC, amp, phase)
+        if q == 0:
+            oms = GC.OMEGA * np.array([0.9, 1.0, 1.1])
+            mons = [g.add_overlap(I // 2, J // 2, K - 8, mode, GC.MI, GC.MJ, GC.DISC, oms),
+                    g.add_overlap(I // 2, J // 2, 10, mode, GC.MI, GC.MJ, GC.DISC, oms, reverse=1)]
+        elif q == 1:
+            oms, mons = np.zeros(0), []
+        else:
+            oms = GC.OMEGA * np.array([0.8, 0.9, 1.0, 1.1, 1.2])
+            mons = [g.add_overlap(I // 2, J // 2, K - 9, mode, GC.MI, GC.MJ, GC.DISC, oms)]
+        return g, mons, len(oms)
+
+    spec = [(0, 33), (1, 41), (2, 52)]
+    ref = []
+    for q, K in spec:
+        g, mons, nf = make(q, K)
+        g.run(GC.DT, 0, NSTEPS)
+        ref.append(_results(g, mons, nf))
+        g.free()
+    built = [make(q, K) for q, K in spec]
+    b = fd.Batch([g for g, _, _ in built])
+    b.set_shape(2, True, False)
+    b.run(GC.DT, 0, 61)
+    b.run(GC.DT, 61, NSTEPS - 61)
+    for (g, mons, nf), (mon_s, f_s) in zip(built, ref):
+        mon_b, f_b = _results(g, mons, nf)
+        ef = float(np.linalg.norm(f_b - f_s) / np.linalg.norm(f_s))
+        say("uneven monitors: grid K %d, %d monitors, fields rel %.2e" % (g.K, len(mons), ef))
+        assert ef <= 1e-6
+        if mons:
+            em = float(np.abs(mon_b - mon_s).max() / np.abs(mon_s).max())
+            say("   monitors max rel %.2e" % em)
+            assert np.abs(mon_s).max() > 0 and em <= 1e-6
+    b.free()
+    for g, _, _ in built:
+        g.free()
