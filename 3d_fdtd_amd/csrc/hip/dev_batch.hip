@@ -94,12 +94,13 @@ struct fdtd_dev_batch {
     int kc = 0;
     bool unroll = true, persistent = true;
     bool u8 = true, mon = false;
+    bool ext = false;  // every grid extruded: ids from the K x I tables, not the id volumes
     bool nt = false;  // streaming own-cell loads: the mean grid's fields exceed the cache
     unsigned grid_blocks = 0;
     // captured segment of kSeg steps
     hipGraphExec_t exec = nullptr;
     int exec_kc = -1;
-    bool exec_unroll = false, exec_persistent = false, exec_mon = false;
+    bool exec_unroll = false, exec_persistent = false, exec_mon = false, exec_ext = false;
     long long exec_total = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // per-kernel timing: while on, steps run without the captured graph and every E / H launch
@@ -200,7 +201,9 @@ void launch_step_loop(fdtd_dev_batch *b) {
 
 template <int KC>
 void launch_step_kc(fdtd_dev_batch *b) {
-    if (b->u8 && b->nt) launch_step_loop<uint8_t, KC, true>(b);
+    if (b->ext && b->nt) launch_step_loop<ExtIds, KC, true>(b);
+    else if (b->ext) launch_step_loop<ExtIds, KC, false>(b);
+    else if (b->u8 && b->nt) launch_step_loop<uint8_t, KC, true>(b);
     else if (b->u8) launch_step_loop<uint8_t, KC, false>(b);
     else if (b->nt) launch_step_loop<uint16_t, KC, true>(b);
     else launch_step_loop<uint16_t, KC, false>(b);
@@ -221,7 +224,7 @@ int launch_step(fdtd_dev_batch *b, int kc, bool unroll) {
 // The captured segment for the current layout (re-captured when the kernel choice changes).
 int segment_graph(fdtd_dev_batch *b) {
     if (b->exec && b->exec_kc == b->kc && b->exec_unroll == b->unroll && b->exec_persistent == b->persistent &&
-        b->exec_mon == b->mon && b->exec_total == b->total)
+        b->exec_mon == b->mon && b->exec_ext == b->ext && b->exec_total == b->total)
         return 0;
     if (b->exec) {
         (void)hipGraphExecDestroy(b->exec);
@@ -244,6 +247,7 @@ int segment_graph(fdtd_dev_batch *b) {
     b->exec_unroll = b->unroll;
     b->exec_persistent = b->persistent;
     b->exec_mon = b->mon;
+    b->exec_ext = b->ext;
     b->exec_total = b->total;
     return 0;
 }
@@ -261,6 +265,8 @@ int batch_chunk(fdtd_dev_batch *b, double dt, int n0, int n) {
     for (auto *d : b->grids) FDTD_TRY(hipStreamSynchronize(d->stream));
     FDTD_TRY(hipStreamSynchronize(b->stream));  // nothing of an earlier call (failed or not) in flight
     b->mon = false;
+    b->ext = true;
+    for (auto *d : b->grids) b->ext = b->ext && d->ext_ids != nullptr;
     for (size_t g = 0; g < b->grids.size(); g++) {
         int rc = run_batch_prepare(b->grids[g], dt, n0, n, b->step, &b->host[g].a, &b->host[g].pm);
         if (rc) return rc;

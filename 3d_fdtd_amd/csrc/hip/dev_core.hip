@@ -103,6 +103,13 @@ __global__ void k_soa_to_aos(const float *__restrict__ base, double *__restrict_
     for (int m = 0; m < 6; m++) aos[6 * c + m] = (double)base[m * Nal + c];
 }
 
+// The id volume is about to change: the extruded table no longer describes it (no kernel still
+// queued reads it: batched calls end synchronised, and the single-grid kernels read the volume).
+void drop_extruded_ids(fdtd_dev *d) {
+    (void)dev_free(d->ext_ids);
+    d->ext_ids = nullptr;
+}
+
 }  // namespace
 
 int ensure_coefficients(fdtd_dev *d, double dt) {
@@ -354,6 +361,7 @@ void fdtd_dev_destroy(fdtd_dev *d) {
     (void)dev_free(d->field_base);
     (void)dev_free(d->alt_base);
     (void)dev_free(d->mat);
+    (void)dev_free(d->ext_ids);
     (void)dev_free(d->d_props);
     (void)dev_free(d->d_coef);
     (void)dev_free(d->shell);
@@ -415,6 +423,7 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
     // and a table as long as the grid, and runs on the split kernels
     int nmat = (int)ids.size();
     int bytes = nmat <= 256 ? 1 : (nmat <= 65536 ? 2 : 4);
+    drop_extruded_ids(d);
     if (d->mat && bytes != d->mat_bytes) {
         FDTD_TRY(dev_free(d->mat));
         d->mat = nullptr;
@@ -515,6 +524,7 @@ int paint(fdtd_dev *d, PaintArgs p, float ep, float mu, bool set_mu) {
     const int nmat = (int)ids.size();
     const int bytes = nmat <= 256 ? 1 : (nmat <= 65536 ? 2 : 4);
     if (bytes > d->mat_bytes && (rc = widen_material_ids(d, bytes))) return rc;
+    drop_extruded_ids(d);
     uint32_t *drm = nullptr;
     FDTD_TRY(dev_malloc(&drm, remap.size() * sizeof(uint32_t)));
     hipError_t e = hipMemcpyAsync(drm, remap.data(), remap.size() * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream);
@@ -642,8 +652,15 @@ int fdtd_dev_set_materials_extruded(fdtd_dev *d, int nmat, const float *ep, cons
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);  // the host id array may go right after
-    (void)dev_free(dids);
-    if (e != hipSuccess) return fail("fdtd_dev_set_materials_extruded", e);
+    drop_extruded_ids(d);
+    if (e != hipSuccess) {
+        (void)dev_free(dids);
+        return fail("fdtd_dev_set_materials_extruded", e);
+    }
+    d->ext_ids = dids;  // the volume now equals the table
+    d->ext_j0 = j0;
+    d->ext_j1 = j1;
+    d->ext_bg = background;
     if (nmat != d->nmat || !d->d_props) {
         (void)dev_free(d->d_props);
         (void)dev_free(d->d_coef);

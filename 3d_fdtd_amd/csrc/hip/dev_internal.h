@@ -197,6 +197,11 @@ struct fdtd_dev {
     int mat_bytes;        // 1, 2 or 4 (4: split kernels only)
     long long mat_uploads = 0, mat_paints = 0;  // full material uploads, device repaints
     void *mat;            // per-cell id
+    // an extruded geometry's ids (set_materials_extruded): ids[k][i] in rows [ext_j0, ext_j1),
+    // ext_bg elsewhere; kept while the id volume equals it (the batched kernels then read this
+    // K x I table instead of the volume), null once a full upload or a repaint changes the ids
+    uint16_t *ext_ids = nullptr;
+    int ext_j0 = 0, ext_j1 = 0, ext_bg = 0;
     std::vector<float> props;  // nmat * kPropStride
     float *d_props;
     float *d_coef;        // [2 schemes][nmat][kCoefStride], then the packed and isotropic tables

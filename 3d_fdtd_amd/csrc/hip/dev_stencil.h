@@ -10,6 +10,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "dev_internal.h"
 
@@ -20,6 +21,8 @@ struct YeeArgs {
     float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
     const void *mat;
     const float *coef;  // [nmat][kCoefStride] for the chosen scheme
+    const uint16_t *ext;  // extruded ids [K][I] of rows [ext_j0, ext_j1), ext_bg elsewhere, or null
+    int ext_j0, ext_j1, ext_bg;
     int I, J, K;
     long long P;
     int k0, k1;          // z range updated, already clipped to [1, K-1)
@@ -134,6 +137,21 @@ __device__ __forceinline__ T ld_own(const T *base, Off i) {
 template <typename MatT, typename Off>
 __device__ __forceinline__ int mat_at(const void *mat, Off c) {
     return (int)ld(static_cast<const MatT *>(mat), c);
+}
+
+// Material ids taken from an extruded geometry's K x I table (YeeArgs::ext) instead of the
+// per-cell volume: the batched kernels' MatT when every grid of the batch is extruded.  The table
+// of one z plane is a few hundred bytes that neighbouring blocks share through the cache, so a
+// cell-update no longer reads its 1- or 2-byte id from HBM.
+struct ExtIds {};
+
+// Material id of cell c = (i, j, k).
+template <typename MatT, typename Off>
+__device__ __forceinline__ int material(const YeeArgs &a, Off c, int i, int j, int k) {
+    if constexpr (std::is_same<MatT, ExtIds>::value)
+        return j >= a.ext_j0 && j < a.ext_j1 ? (int)gview(a.ext)[k * a.I + i] : a.ext_bg;
+    else
+        return mat_at<MatT>(a.mat, c);
 }
 
 // Source components m0..m0+2 of plane cell p at plane k: tabulated run source inside its mode
@@ -318,7 +336,7 @@ __device__ __forceinline__ void update_E_block(const YeeArgs &a, const PmlArgs &
             hy_im[t] = ld(Hy, c - 1), hz_im[t] = ld(Hz, c - 1);
             hx_jm[t] = ld(Hx, c - I), hz_jm[t] = ld(Hz, c - I);
             eox[t] = ld_own<NT>(Ex, c), eoy[t] = ld_own<NT>(Ey, c), eoz[t] = ld_own<NT>(Ez, c);
-            m[t] = mat_at<MatT>(a.mat, c);
+            m[t] = material<MatT>(a, c, i, j, kk);
             if (PML) psi_load(pm.psiE, pm, a, i, j, p, kk, sx, sy, pv[t]);
         }
 #pragma unroll
@@ -336,7 +354,7 @@ __device__ __forceinline__ void update_E_block(const YeeArgs &a, const PmlArgs &
             float pv[6];
             if (PML) psi_load(pm.psiE, pm, a, i, j, p, k, sx, sy, pv);
             cell(k, true, c, hx, hy, hz, ld(Hy, c - 1), ld(Hz, c - 1), ld(Hx, c - I), ld(Hz, c - I), hx_km, hy_km,
-                 ld_own<NT>(Ex, c), ld_own<NT>(Ey, c), ld_own<NT>(Ez, c), mat_at<MatT>(a.mat, c), pv);
+                 ld_own<NT>(Ex, c), ld_own<NT>(Ey, c), ld_own<NT>(Ez, c), material<MatT>(a, c, i, j, k), pv);
             hx_km = hx;
             hy_km = hy;
         }
@@ -490,7 +508,7 @@ __device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &
                 ey_ip[t] = ld(Ey, c + 1), ez_ip[t] = ld(Ez, c + 1);
                 ex_jp[t] = ld(Ex, c + I), ez_jp[t] = ld(Ez, c + I);
                 hox[t] = ld_own<NT>(Hx, c), hoy[t] = ld_own<NT>(Hy, c), hoz[t] = ld_own<NT>(Hz, c);
-                m[t] = mat_at<MatT>(a.mat, c);
+                m[t] = material<MatT>(a, c, i, j, kk);
                 if (PML) psi_load(pm.psiH, pm, a, i, j, p, kk, sx, sy, pv[t]);
             }
 #pragma unroll
@@ -508,7 +526,7 @@ __device__ __forceinline__ void update_H_block(const YeeArgs &a, const PmlArgs &
                 float pv[6];
                 if (PML) psi_load(pm.psiH, pm, a, i, j, p, k, sx, sy, pv);
                 cell(k, true, c, ex, ey, ld(Ez, c), ex_kp, ey_kp, ld(Ey, c + 1), ld(Ez, c + 1), ld(Ex, c + I),
-                     ld(Ez, c + I), ld_own<NT>(Hx, c), ld_own<NT>(Hy, c), ld_own<NT>(Hz, c), mat_at<MatT>(a.mat, c),
+                     ld(Ez, c + I), ld_own<NT>(Hx, c), ld_own<NT>(Hy, c), ld_own<NT>(Hz, c), material<MatT>(a, c, i, j, k),
                      pv);
                 ex = ex_kp;
                 ey = ey_kp;

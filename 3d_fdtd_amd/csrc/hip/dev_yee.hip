@@ -407,6 +407,10 @@ int base_args(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool pml, YeeA
     a.Hz = d->F[HZ];
     a.mat = d->mat;
     a.coef = d->d_coef + (size_t)scheme * d->nmat * kCoefStride;
+    a.ext = d->ext_ids;
+    a.ext_j0 = d->ext_j0;
+    a.ext_j1 = d->ext_j1;
+    a.ext_bg = d->ext_bg;
     a.I = d->I;
     a.J = d->J;
     a.K = d->K;
