@@ -34,7 +34,7 @@ __device__ __forceinline__ int batch_grid(const BatchGeom *__restrict__ geo, int
     return lo;
 }
 
-// NT: streaming own-cell loads (the mean grid's fields exceed the last-level cache).  LOOP: a
+// NT: streaming own-cell loads (the batch's fields exceed the last-level cache).  LOOP: a
 // persistent grid-stride launch; else one block per virtual block, with no state carried from
 // one virtual block to the next (fewer registers).
 template <typename MatT, int KC, bool NT, bool LOOP>
@@ -95,7 +95,7 @@ struct fdtd_dev_batch {
     bool unroll = true, persistent = true;
     bool u8 = true, mon = false;
     bool ext = false;  // every grid extruded: ids from the K x I tables, not the id volumes
-    bool nt = false;  // streaming own-cell loads: the mean grid's fields exceed the cache
+    bool nt = false;  // streaming own-cell loads: the batch's fields exceed the cache
     unsigned grid_blocks = 0;
     // captured segment of kSeg steps
     hipGraphExec_t exec = nullptr;
@@ -348,12 +348,12 @@ fdtd_dev_batch *fdtd_dev_batch_new(fdtd_dev *const *grids, int n) {
     b->host.resize((size_t)n);
     std::memset((void *)b->host.data(), 0, b->host.size() * sizeof(BatchGeom));
     b->u8 = grids[0]->mat_bytes == 1;
-    // the launches sweep the grids one after another, so the cache sees one grid at a time:
-    // decide on the mean grid (64 tapers of 3.7-15 M cells, mean 9.5 M: 58.2 G cell-updates/s
-    // streaming, 58.5 G plain)
+    // every launch sweeps all grids, so nothing one launch reads is still cached at the next
+    // unless the whole batch fits: decide on the batch's total (64 tapers, 608 M cells, chosen
+    // launch shape: 9.61-9.65 ms per step streaming against 9.69-9.72 plain)
     long long cells = 0;
     for (int g = 0; g < n; g++) cells += grids[g]->N;
-    b->nt = streaming_loads(cells / n, grids[0]->stream_mode);
+    b->nt = streaming_loads(cells, grids[0]->stream_mode);
     b->stage = batch_stage_new();
     hipError_t e = dev_stream_acquire(&b->stream);
     if (e == hipSuccess) e = dev_malloc(&b->step, sizeof(int));
