@@ -188,3 +188,59 @@ def test_batch_finish_with_uneven_monitor_sets():
     b.free()
     for g, _, _ in built:
         g.free()
+
+
+def _extruded(q, K):
+    """A 20 x 20 x K grid rasterised on the device from an extruded table: core rows 8..11 take
+    ids[k][i] (core where |i - 10| < w(k)), every other cell the cladding."""
+    I = J = 20
+    table = np.array([[2.0] * 3, [6.0] * 3], np.float32)
+    ids = np.zeros((K, I), np.uint16)
+    for k in range(K):
+        w = 1 + q + (k * 3) // K
+        ids[k, 10 - w:10 + w] = 1
+    g = fd.DevGrid(I, J, K, GC.DISC, None, extruded=(table, ids, 8, 12, 0))
+    g.cpml((5, 5, 5, 5, 5, 5))
+    mode = fd.normalised_mode(GC.MI, GC.MJ, GC.DISC, GC.mode_array())
+    amp, phase = _schedule()
+    g.add_source(I // 2, J // 2, 7, mode, GC.MI, GC.MJ, GC.DISC, amp, phase)
+    oms = GC.OMEGA * np.array([0.9, 1.0, 1.1])
+    mons = [g.add_overlap(I // 2, J // 2, K - 8, mode, GC.MI, GC.MJ, GC.DISC, oms)]
+    return g, mons, len(oms)
+
+
+@pytest.mark.parametrize("repaint", [False, True])
+def test_batch_of_extruded_grids(repaint):
+    """Grids rasterised from extruded tables: the batch takes their ids from the K x I tables
+    (no id volume read) and equals each grid's single-grid run, which reads the volume.  With one
+    grid repainted after the raster (a prism of another permittivity), that grid's table no longer
+    describes its volume and is dropped, the batch falls back to the volumes, and it still equals
+    the single-grid runs of the same repainted grids."""
+    L = fd.lib()
+    spec = [(0, 30), (1, 44), (2, 57)]
+
+    def build():
+        out = [_extruded(q, K) for q, K in spec]
+        if repaint:
+            fd.dev_check(L.fdtd_dev_paint_prism(out[1][0].d, 4.0, 1.0, 0, 4, 9, 6, 14, 12, 20), "paint")
+        return out
+
+    ref = []
+    for g, mons, nf in build():
+        g.run(GC.DT, 0, NSTEPS)
+        ref.append(_results(g, mons, nf))
+        g.free()
+    built = build()
+    b = fd.Batch([g for g, _, _ in built])
+    b.set_shape(2, True, False)
+    b.run(GC.DT, 0, 70)
+    b.run(GC.DT, 70, NSTEPS - 70)
+    for (g, mons, nf), (mon_s, f_s) in zip(built, ref):
+        mon_b, f_b = _results(g, mons, nf)
+        ef = float(np.linalg.norm(f_b - f_s) / np.linalg.norm(f_s))
+        em = float(np.abs(mon_b - mon_s).max() / np.abs(mon_s).max())
+        say("extruded (repaint %s): grid K %d fields rel %.2e, monitors max rel %.2e" % (repaint, g.K, ef, em))
+        assert np.abs(mon_s).max() > 0 and ef <= 1e-6 and em <= 1e-6
+    b.free()
+    for g, _, _ in built:
+        g.free()
