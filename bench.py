@@ -112,8 +112,9 @@ def bench_tapers(fd, args):
            "config": {"workload": f"batched tapers: {args.tapers} Si tapers (seed {WL.TAPER_SEED}), lengths "
                                   "U[3,15] um, 8 spline control-point widths U[0.4,3.2] um, 200x100 cross-section "
                                   "of 20 nm cells, 8-cell CPML, mode source, FP32 fields, FP64 overlap monitor x 16 "
-                                  "wavelengths per taper; all grids advance in lockstep (persistent batched "
-                                  "kernels, hipGraph-captured steps)",
+                                  "wavelengths per taper; all grids advance in lockstep (batched E/H kernels "
+                                  "over a per-grid descriptor table, launch shape autotuned, hipGraph-captured "
+                                  "steps)",
                       "grids": args.tapers, "cells_total": cells, "setup_s": round(setup_s, 2),
                       "tuning_steps_before_warmup": TUNE_STEPS,
                       "lengths_cells": [WL.taper_length_cells(x) for x in lengths]},
