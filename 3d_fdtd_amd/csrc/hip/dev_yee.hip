@@ -349,8 +349,8 @@ PmlArgs pml_args(fdtd_dev *d) {
 // mb: material id bytes; 4-byte ids (per-cell media) take the runtime-chunk kernels only (KC 0),
 // without the streaming hint.
 template <int KC, typename Off>
-void launch_split(bool e_field, bool mon, int mb, bool pml, bool nt, dim3 grid, hipStream_t st, hipEvent_t t0,
-                  hipEvent_t t1, const YeeArgs &a, const PmlArgs &pm) {
+void launch_split(bool e_field, bool mon, int mb, bool ext, bool pml, bool nt, dim3 grid, hipStream_t st,
+                  hipEvent_t t0, hipEvent_t t1, const YeeArgs &a, const PmlArgs &pm) {
 #define FDTD_GO(KER) hipExtLaunchKernelGGL(KER, grid, dim3(kBlock), 0, st, t0, t1, 0, a, pm)
 #define FDTD_PML(KERN, MT)                                  \
     do {                                                    \
@@ -367,6 +367,12 @@ void launch_split(bool e_field, bool mon, int mb, bool pml, bool nt, dim3 grid, 
             else FDTD_GO((k_update_H_mon<uint32_t, false, 0, Off, false>));
             return;
         }
+    }
+    if (ext && pml) {  // extruded ids from the K x I table (PML grids: the batched and run paths)
+        if (e_field) FDTD_PML(k_update_E, ExtIds);
+        else if (!mon) FDTD_PML(k_update_H, ExtIds);
+        else FDTD_PML(k_update_H_mon, ExtIds);
+        return;
     }
     const bool u8 = mb == 1;
     if (e_field) {
@@ -463,19 +469,20 @@ int launch_update(fdtd_dev *d, double dt, int k0, int k1, int scheme, bool e_fie
     dim3 grid((unsigned)(a.plane_blocks * zchunks));
     const int mb = d->mat_bytes;
     const bool nt = pml && streaming_loads(d->N, d->stream_mode);
+    const bool ext = d->ext_ids != nullptr;
     hipEvent_t t0, t1;
     timing_events(d, e_field ? 0 : 1, &t0, &t1);
     if (d->N < kSmallGridCells && mb != 4) {
         switch (a.kchunk) {
-            case 2: launch_split<2, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm); break;
-            case 3: launch_split<3, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm); break;
-            case 4: launch_split<4, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm); break;
-            default: launch_split<0, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            case 2: launch_split<2, uint32_t>(e_field, mon, mb, ext, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            case 3: launch_split<3, uint32_t>(e_field, mon, mb, ext, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            case 4: launch_split<4, uint32_t>(e_field, mon, mb, ext, pml, nt, grid, d->stream, t0, t1, a, pm); break;
+            default: launch_split<0, uint32_t>(e_field, mon, mb, ext, pml, nt, grid, d->stream, t0, t1, a, pm); break;
         }
     } else if (d->N < kSmallGridCells) {
-        launch_split<0, uint32_t>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm);
+        launch_split<0, uint32_t>(e_field, mon, mb, false, pml, nt, grid, d->stream, t0, t1, a, pm);
     } else {
-        launch_split<0, long long>(e_field, mon, mb, pml, nt, grid, d->stream, t0, t1, a, pm);
+        launch_split<0, long long>(e_field, mon, mb, false, pml, nt, grid, d->stream, t0, t1, a, pm);
     }
     FDTD_TRY(hipGetLastError());
     return 0;
