@@ -247,9 +247,10 @@ DESIGN_DATA = os.path.join(ROOT, "3d_fdtd_amd", "data")
 
 
 def design_family(design):
-    """The Cfg-4 taper family of the design loop: 20 nm cells, 200 x 100 cross-section, 8-cell CPML,
-    lengths U[3, 15] um, 8 spline control widths U[0.4, 3.2] um (fill-fraction raster), 6000 steps,
-    16 wavelengths 1.50-1.60 um, TE0 port modes from the mode solver."""
+    """The taper family of the design loop (that of the batched 64-taper set): 20 nm cells, 200 x 100
+    cross-section, 8-cell CPML, lengths U[3, 15] um, 8 spline control widths U[0.4, 3.2] um
+    (fill-fraction raster), 6000 steps, 16 wavelengths 1.50-1.60 um, TE0 port modes from the mode
+    solver."""
     return design.TaperFamily(disc=DISC, I=WL.TAPER_I, J=WL.TAPER_J, core_cells=WL.TAPER_CORE, npml=WL.TAPER_NPML,
                               port=0, wmin=0.4, wmax=3.2, lmin=3.0, lmax=15.0, steps=6000, lam=(1.50, 1.60),
                               nfreq=16, ports="te0")
@@ -286,7 +287,7 @@ def design_data(fd, args):
 
 
 def bench_design(fd, args):
-    """Verified taper designs per hour on the Cfg-4 family (design_family) at the surrogate
+    """Verified taper designs per hour on the batched set's taper family (design_family) at the surrogate
     configuration: an MLP 9 -> 256 x 4 -> 32 (BF16 MFMA) trained on the 4096-design FDTD training set
     (generated beforehand in chunks by --design-phase data), screening 10^5 candidates per
     iteration, the top 64 verified by batched FDTD; against the plain loop (gradient ascent with
@@ -389,7 +390,7 @@ def bench_design(fd, args):
     out = {"metric": METRIC, "value": rate, "unit": "verified taper designs/hour", "n_gpus": 1, "steps": fam.steps,
            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 fields, f64 monitors, bf16 surrogate", "data": "synthetic",
-           "config": {"workload": "Cfg-4 taper family (20 nm cells, 200x100 cross-section, 8-cell CPML, lengths U[3,15] "
+           "config": {"workload": "taper family of the batched 64-taper set (20 nm cells, 200x100 cross-section, 8-cell CPML, lengths U[3,15] "
                                   "um, 8 spline widths U[0.4,3.2] um, 6000 steps, 16 wavelengths 1.50-1.60 um, TE0 "
                                   "ports from the mode solver); surrogate MLP 9-256x4-32 GELU (BF16 MFMA, FP32 "
                                   "accumulate) trained on the FDTD training set; 10^5 screened and 64 verified by "

@@ -1,4 +1,4 @@
-"""GPU test of the surrogate design loop on the Cfg-4 taper family at reduced counts: 20 nm cells,
+"""GPU test of the surrogate design loop on the batched taper set's family at reduced counts: 20 nm cells,
 200 x 100 cross-section, 8-cell CPML, 6000 steps, 16 wavelengths, TE0 port modes from the mode
 solver (mode_solve_beta_squared), the fill-fraction raster on the device.  Reduced: lengths
 U[3, 4] um, fixed end widths (1.0 and 2.0 um, so two port modes), 32 training designs, 2000
