@@ -3,12 +3,14 @@
 //
 // Each grid keeps its own device state (fdtd_dev) and run configuration (tabulated sources,
 // overlap monitors).  A device table holds one descriptor per grid (the split-kernel arguments
-// plus the grid's first virtual block); the launches are persistent grid-stride kernels over the
-// concatenated virtual blocks of all grids, remapped so that each XCD walks a contiguous range
-// (consecutive z chunks of one grid share that XCD's L2).  Per-step values (source amplitudes,
-// monitor partial slots) are read through a device step counter, so a segment of steps (E, H,
-// counter tick) is captured once in a hipGraph and replayed.  The z-chunk depth of the launches
-// is picked by a small autotuner that times real steps of the run with each candidate.
+// plus the grid's first virtual block); the launches cover the concatenated virtual blocks of all
+// grids, one block per virtual block or a persistent grid striding over them, remapped so that
+// each XCD walks a contiguous range (consecutive z chunks of one grid share that XCD's L2).
+// Per-step values (source amplitudes, monitor partial slots) are read through a device step
+// counter, so a segment of steps (E, H, counter tick) is captured once in a hipGraph and
+// replayed.  The launch shape (z-chunk depth, unrolled or looped chunk, persistent or not) is
+// picked by a small autotuner that times real steps of the run with each candidate.  When every
+// grid is an extruded geometry the kernels read material ids from the grids' K x I tables.
 #include <cstring>
 #include <vector>
 
