@@ -38,7 +38,10 @@ int fdtd_dev_set_materials(fdtd_dev *d, const float *ep, const float *mu, const 
  * sig 6 per entry or NULL for 1 / 0) and, for the rows j0 <= j < j1, one id per (k, i) in ids[k * I
  * + i]; every other cell takes `background`.  This is the structure of the reference's waveguide
  * and taper setters (timefield3d.c:1458-1535) and of adjoint3d_taper_insert: the host uploads
- * K * I ids instead of 3 * N permittivities. */
+ * K * I ids instead of 3 * N permittivities.  The grid keeps the K * I table on the device while
+ * its id volume equals it: the split updates of CPML grids and batched runs of such grids then
+ * read each cell's id from the table, not from the volume.  A later fdtd_dev_set_materials or
+ * paint drops the table. */
 int fdtd_dev_set_materials_extruded(fdtd_dev *d, int nmat, const float *ep, const float *mu, const float *sig,
                                     const uint16_t *ids, int j0, int j1, int background);
 
