@@ -302,6 +302,25 @@ def adjoint_scale(amp, w, w0, dt):
     return 2.0 * (np.exp(-1j * w * dt) - 1.0) / (dt * E0)
 
 
+def plain_part(accepted, starts, iterations, seconds, spec, accepted_designs):
+    """The saved record of a plain-loop campaign: counts, wall time, the rate of accepted designs
+    per hour and the accepted designs (rounded to 1e-4)."""
+    return {"accepted": accepted, "starts": starts, "iterations": iterations, "fdtd_runs": 2 * iterations,
+            "seconds": round(seconds, 1), "designs_per_hour": round(accepted / seconds * 3600, 1) if seconds > 0 else 0.0,
+            "spec": spec, "accepted_designs": [[round(float(v), 4) for v in q] for q in accepted_designs]}
+
+
+def merge_plain_parts(prev, part):
+    """A saved plain-loop record with a later call's record added (the calls draw their random
+    starts from different seeds): counts and wall time summed, the rate taken over the total.  The
+    two must share the acceptance spec (the same training set's percentile)."""
+    if prev["spec"] != part["spec"]:
+        raise ValueError(f"plain-loop parts of different specs ({prev['spec']} and {part['spec']})")
+    return plain_part(prev["accepted"] + part["accepted"], prev["starts"] + part["starts"],
+                      prev["iterations"] + part["iterations"], prev["seconds"] + part["seconds"], prev["spec"],
+                      prev["accepted_designs"] + part["accepted_designs"])
+
+
 def band_transmission(t):
     """Band-averaged power transmission mean_lambda |t|^2."""
     return np.mean(np.abs(np.asarray(t)) ** 2, axis=-1)

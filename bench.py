@@ -371,14 +371,9 @@ def bench_design(fd, args):
             print(f"bench: plain start {starts}: T {hist[0]:.3f} -> {hist[-1]:.3f} in {len(hist)} iterations, "
                   f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
         t_plain = time.perf_counter() - t0
-        acc_list = [[round(float(v), 4) for v in q] for q in plain_acc]
+        part = design.plain_part(plain_ok, starts, iters, t_plain, spec, plain_acc)
         if prev:
-            plain_ok, starts, iters = plain_ok + prev["accepted"], starts + prev["starts"], iters + prev["iterations"]
-            t_plain += prev["seconds"]
-            acc_list = prev["accepted_designs"] + acc_list
-        part = {"accepted": plain_ok, "starts": starts, "iterations": iters, "fdtd_runs": 2 * iters,
-                "seconds": round(t_plain, 1), "designs_per_hour": round(plain_ok / t_plain * 3600, 1), "spec": spec,
-                "accepted_designs": acc_list}
+            part = design.merge_plain_parts(prev, part)
         with open(plain_path, "w") as fh:
             json.dump(part, fh)
     paths = [os.path.join(DESIGN_DATA, f"cfg5_{w}.json") for w in ("surrogate", "plain")]
