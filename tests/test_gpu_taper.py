@@ -115,3 +115,38 @@ def test_device_raster_equals_host_raster(fam):
     bad[0, 0] = table.shape[0]
     with pytest.raises(RuntimeError, match="outside the table"):
         fd.DevGrid(fam.I, fam.J, K, fam.disc, None, extruded=(table, bad, j0, j1, bg))
+
+
+def test_full_upload_after_device_raster_drops_the_id_table(fam):
+    """A grid rasterised from an extruded table, then given a different permittivity by a full
+    upload (fdtd_dev_set_materials): the table no longer describes the grid and must not be read.
+    Its run equals that of a grid built from the second permittivity alone."""
+    fd = design.fd
+    rng = np.random.default_rng(53)
+    p1, p2 = fam.random(rng, 2)
+    ep1, K1 = fam.permittivity(p1)
+    ep2, K2 = fam.permittivity(p2)
+    K = min(K1, K2)
+    ex1, _ = fam.extruded(p1)
+    table, ids, j0, j1, bg = ex1
+    ex1 = (table, ids[:K], j0, j1, bg)
+    ep2 = ep2.reshape(K2, fam.J, fam.I, 3)[:K].reshape(-1)
+    runs = []
+    for build in ("raster-then-upload", "upload"):
+        if build == "upload":
+            g = fd.DevGrid(fam.I, fam.J, K, fam.disc, ep2)
+        else:
+            g = fd.DevGrid(fam.I, fam.J, K, fam.disc, None, extruded=ex1)
+            fd.dev_check(fd.lib().fdtd_dev_set_materials(g.d, np.ascontiguousarray(ep2, np.float32).ctypes.data_as(
+                fd.C.POINTER(fd.C.c_float)), None, None), "set_materials")
+        g.cpml((fam.npml,) * 6, 3, 1e-8, 1.0, 0.05)
+        ci, cj, ks = fam.I // 2, fam.J // 2, fam.npml + 1
+        g.add_source(ci, cj, ks, fam.mode, fam.mi, fam.mj, fam.disc, fam.amp, fam.phase)
+        mon = g.add_overlap(ci, cj, K - fam.npml - 2, fam.mode, fam.mi, fam.mj, fam.disc, fam.omegas)
+        g.run(fam.dt, 0, 300)
+        runs.append((g.fields(), g.overlap(mon, len(fam.omegas))[0]))
+        g.free()
+        say("full upload after raster:", build, "done")
+    assert np.abs(runs[0][0]).max() > 0
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
