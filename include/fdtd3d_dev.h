@@ -139,8 +139,9 @@ int fdtd_dev_run_overlap_result(fdtd_dev *d, int id, double *cumulative, double 
 int fdtd_dev_run_dft_result(fdtd_dev *d, int id, double *out);
 
 /* Batched runs (extension): the grids advance in lockstep, one E and one H launch per step over
- * all of them (persistent kernels over a device table of per-grid descriptors; steps replayed
- * from a captured hipGraph).  Every grid runs under the PML boundary with the split kernels and
+ * all of them (kernels over a device table of per-grid descriptors; steps replayed from a
+ * captured hipGraph; the monitors' per-step overlaps and running DFTs of every grid are formed in
+ * two launches when the call ends).  Every grid runs under the PML boundary with the split kernels and
  * holds its run configuration (at most 2 run sources and 4 overlap monitors clear of the boundary
  * ring, no DFT monitors).  fdtd_dev_batch_run(b, dt, n0, n) advances steps n0..n0+n-1 of every
  * grid's run; results are read per grid (fdtd_dev_run_overlap_result).  The launch shape (z
