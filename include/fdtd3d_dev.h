@@ -90,7 +90,10 @@ int fdtd_dev_cpml_off(fdtd_dev *d);
 int fdtd_dev_update_E_cpml(fdtd_dev *d, double dt);
 int fdtd_dev_update_H_cpml(fdtd_dev *d, double dt);
 /* Kernel used by fdtd_dev_step under the PML boundary: rows = 0 selects the split E / H pair
- * (kchunk > 0 fixes their z planes per thread, 0 lets each launch choose), rows = -2 the
+ * (kchunk > 0 fixes their z planes per thread, 0 lets each launch choose), rows = -3 the
+ * single-pass full step (E then H of one step in one z-march of 64 x 4 tiles, one launch per
+ * slab class, ping-pong fields and CPML memories; kchunk = z planes per chunk, 0 automatic),
+ * rows = -2 the
  * single-pass leap launches for device-resident runs (fdtd_dev_run: H of step s and E of step
  * s + 1 in one launch, each field component read and written once; kchunk = z planes per chunk,
  * 0 automatic; a lone fdtd_dev_step stays a split pair), rows = -1 the fused
@@ -103,7 +106,8 @@ int fdtd_dev_set_step_kernel(fdtd_dev *d, int rows, int kchunk);
 /* Cache policy of the split updates under the PML: the loads of each cell's own old field may
  * carry the streaming (non-temporal) hint, which pays off once the fields exceed the last-level
  * cache and costs grids that fit it.  mode -1 (default): on when the six field arrays exceed
- * 240 MB; 0 off; 1 on. */
+ * 240 MB; 0 off; 1 on.  A batch decides once, on its grids' total field bytes (every batched
+ * launch sweeps all grids), with the first grid's mode. */
 int fdtd_dev_set_streaming_loads(fdtd_dev *d, int mode);
 
 /* Device-resident run (extension).  A run holds mode sources whose per-step amplitude and
